@@ -1,0 +1,189 @@
+// Device-side types and helpers of libsphnj (gfx950, wave64).
+//
+// Precision policies: P64 computes everything in double; M32 computes pair sums, rates,
+// pressure views and stress in float on cell-relative float coordinates, and keeps the
+// absolute positions, velocities and densities (the integrated state) in double.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sphnj {
+
+struct P64 {
+  using R = double;
+  static constexpr bool MIXED = false;
+};
+struct M32 {
+  using R = float;
+  static constexpr bool MIXED = true;
+};
+
+// ---- per-case constants (one entry per case of a batch) ---------------------------------
+template <class R>
+struct CaseDev {
+  // kernel (Kernel::initiate, src/kernel.cpp:25-41)
+  R rc2, hinv2, kdc, k2c, eR, h, eps_h2;  // eps_h2 = 0.01 h^2
+  R quarter_rc2;
+  // fluid per phase: pair-side constants in R
+  R c[2], cinv[2], mu8[2], rho0[2];
+  // fluid per phase: integration-side constants in double
+  double rho0_d[2], rho0_inv_d[2], gam_d[2], ginv_d[2], rho2p_d[2], p2rho_d[2], rho_min_d[2];
+  double p_min_d, p_bg_d;
+  R p_min;
+  R gx, gy;
+  double gx_d, gy_d, dt, dt_s;
+  R shift_k;     // particle_shift_coef * h * dt
+  R umax_cap;    // 0.1 * min(c_L, c_G)
+  // domain / cell grid (Domain::initiate, src/domain.cpp:37-48)
+  double cxmin, cymin, cdx, cdy, inv_cdx, inv_cdy;
+  R cdx_r, cdy_r;
+  double dxmin, dymin, dxmax, dymax;
+  int nx, ny, cell_off, ncell;
+  int steady, viscid;
+  // solid (SolidProperties, src/properties.cpp:69-89,118-161)
+  R lambda, mu_s, c_s, alpha, beta;
+  R rho_s_inv, rho_aux_inv;
+  double fixed_u, fixed_v;
+  R aux_mod, yield_stress, harden_strain, harden_coef;
+  int plastic;
+  // particle offsets of this case in the concatenated species arrays
+  int f_off, f_n, s_off, s_n, w_off, w_n;
+};
+
+// ---- device buffers --------------------------------------------------------------------
+// Fluid arrays are in cell-sorted slot order and re-permuted every step.  "view" arrays are
+// what neighbour loops read: for P64 they alias the double masters; for M32 they are float
+// copies with positions relative to the particle's own cell origin.  FluidBuf holds the
+// fields that ping-pong between two buffers (permute, pair+update, shift passes).
+template <class Pol>
+struct FluidBuf {
+  using R = typename Pol::R;
+  double *x, *y, *u, *v, *rho;  // masters
+  R *p;
+  R *vx, *vy, *vu, *vv, *vrho;  // views
+};
+
+template <class Pol>
+struct Dev {
+  using R = typename Pol::R;
+  const CaseDev<R> *cases;
+  int ncase;
+  int total_cells;
+  const int *f_case_off, *s_case_off, *w_case_off;  // ncase + 1 each
+  int nf, ns, nw;
+  // fluid
+  FluidBuf<Pol> fa, fb;  // cur (a) / alt (b)
+  int8_t *f_phase, *f_phase2, *f_cls, *f_cls2;  // cur / alt (swap in the permute only)
+  int *f_orig;           // slot -> global original index (sorted values)
+  int *f_key;            // slot -> global cell key (sorted keys)
+  int *f_cxy;            // slot -> (cx << 16) | cy
+  int *f_key_by_orig, *f_slot_of_orig;
+  R *f_col;
+  int8_t *f_near;
+  R *f_L;  // 4 per slot
+  int2 *f_range;  // per global cell: [start, end) slot range
+  unsigned long long *umax2;  // per case: bit pattern of max |u|^2 (double, >= 0)
+  // solid (original order, never permuted)
+  double *sx, *sy, *su, *sv;     // cur
+  double *sx2, *sy2, *su2, *sv2; // alt (sub-step ping-pong)
+  double *sX, *sY;
+  int8_t *s_mat, *s_bc;
+  R *s_vx, *s_vy, *s_vx2, *s_vy2;  // views (M32 cell-relative; P64 alias sx/sy)
+  R *s_p, *s_n;                    // pressure, normal (2 per particle)
+  R *s_F, *s_L, *s_P, *s_PL, *s_M; // 4 per particle
+  R *s_Lref, *s_Bt;                // 4 per particle: B^-1 (or I) and B of the reference configuration
+  int8_t *s_degen;                 // 1 where det(B) == 0 (L = I, F = F~)
+  double *s_a;                     // du_dt, 2 per particle
+  R *s_cp;                         // couple_dudt, 2 per particle
+  int *s_key, *s_cxy, *s_sorted;   // current hash, (cell, orig)-sorted index
+  int2 *s_range;
+  int *s_ref_off, *s_ref_j;        // reference-configuration lists (global solid indices)
+  R *s_ref_g;                      // 2 per entry: w'(dX) dX, w'(dX) dY
+  // wall (sorted once at init)
+  double *wx, *wy;
+  R *w_vx, *w_vy;
+  R *w_u, *w_v, *w_p, *w_rho, *w_n;
+  int8_t *w_phase, *w_bc;
+  int *w_orig, *w_cxy;
+  int2 *w_range;
+  // status word (SPHNJ_ST_* bits)
+  unsigned *status;
+};
+
+// ---- helpers ----------------------------------------------------------------------------
+
+__device__ __forceinline__ int case_of(int i, const int *off, int ncase) {
+  int lo = 0, hi = ncase;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (off[mid] <= i)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
+// One axis of findPositionCell (src/pair.cu:8-18): truncation toward zero, clamp outside [0,n).
+__device__ __forceinline__ int axis_cell(double x, double x0, double inv, int n, unsigned *fl) {
+  const double t = (x - x0) * inv;
+  if (!(t > -1.0)) {
+    *fl |= 1u;
+    return 0;
+  }
+  if (!(t < (double)n)) {
+    *fl |= 1u;
+    return n - 1;
+  }
+  return (int)t;
+}
+
+template <class R>
+__device__ __forceinline__ R kexp(R x);
+template <>
+__device__ __forceinline__ double kexp<double>(double x) {
+  return exp(x);
+}
+template <>
+__device__ __forceinline__ float kexp<float>(float x) {
+  // v_exp_f32 computes 2^x; arguments here lie in [-R^2 log2 e, 0]
+  return __builtin_amdgcn_exp2f(x * 1.4426950408889634f);
+}
+
+template <class R>
+__device__ __forceinline__ R krsqrt(R x);
+template <>
+__device__ __forceinline__ double krsqrt<double>(double x) {
+  return 1.0 / sqrt(x);
+}
+template <>
+__device__ __forceinline__ float krsqrt<float>(float x) {
+  return __builtin_amdgcn_rsqf(x);
+}
+
+template <class R>
+__device__ __forceinline__ R krcp(R x);
+template <>
+__device__ __forceinline__ double krcp<double>(double x) {
+  return 1.0 / x;
+}
+template <>
+__device__ __forceinline__ float krcp<float>(float x) {
+  return __builtin_amdgcn_rcpf(x);
+}
+
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long w = __shfl_xor(v, o, 64);
+    v = w > v ? w : v;
+  }
+  return v;
+}
+
+__device__ __forceinline__ unsigned wave_or(unsigned v) {
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+  return v;
+}
+
+}  // namespace sphnj

@@ -1,0 +1,194 @@
+// Stable LSD radix sort of (32-bit key, 32-bit value) pairs for gfx950, written for wave64.
+//
+// Replaces thrust::sort_by_key + reduce_by_key + exclusive_scan of the reference
+// (src/sort.cu:10-45).  One pass per 8-bit digit, three kernels per pass:
+//   upsweep    per-tile digit histogram: each wave finds the lanes sharing its digit with 8
+//              ballots (peer groups), and one lane per group adds the group size to an LDS
+//              counter, so equal digits (common: particles share cells) cost one LDS atomic
+//              per group instead of one per lane;
+//   rowscan    exclusive scan of each digit's row of tile counts (digit-major layout);
+//   downsweep  re-reads the tile, ranks each item inside its peer group (popcount of the
+//              lower lanes), turns per-wave digit counts into offsets in LDS, and scatters.
+// Items of a tile are visited round by round, wave by wave, lane by lane in index order, so
+// the sort is stable.  Values may be implicit (value = input index) on the first pass.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace sphnj {
+
+constexpr int kRadixBits = 8;
+constexpr int kRadix = 1 << kRadixBits;
+constexpr int kSortThreads = 256;
+constexpr int kSortItems = 16;
+constexpr int kSortTile = kSortThreads * kSortItems;
+
+__device__ __forceinline__ unsigned long long lanemask_lt() {
+  const unsigned lane = threadIdx.x & 63u;
+  return lane == 0 ? 0ull : (~0ull >> (64u - lane));
+}
+
+// Lanes of the wave whose digit equals this lane's digit (0 for invalid lanes).  Every lane
+// of the wave must call it (the ballots are wave-wide).
+__device__ __forceinline__ unsigned long long match_digit(unsigned d, bool valid) {
+  unsigned long long m = __ballot(valid ? 1 : 0);
+#pragma unroll
+  for (int b = 0; b < kRadixBits; ++b) {
+    const bool bit = (d >> b) & 1u;
+    const unsigned long long bal = __ballot(bit ? 1 : 0);
+    m &= bit ? bal : ~bal;
+  }
+  return valid ? m : 0ull;
+}
+
+// Exclusive scan over the 256 threads of a block; returns the exclusive prefix of v and
+// writes the block total to *total.  tmp holds 4 ints (one per wave).
+__device__ __forceinline__ int block_exscan256(int v, int *tmp, int *total) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  int inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int t = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += t;
+  }
+  if (lane == 63) tmp[w] = inc;
+  __syncthreads();
+  int wbase = 0, tot = 0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const int t = tmp[k];
+    if (k < w) wbase += t;
+    tot += t;
+  }
+  __syncthreads();
+  *total = tot;
+  return wbase + inc - v;
+}
+
+__global__ void __launch_bounds__(kSortThreads) radix_upsweep(const int *__restrict__ keys, int n, int shift,
+                                                              int nblocks, int *__restrict__ hist) {
+  __shared__ int cnt[kRadix];
+  const int t = threadIdx.x;
+  cnt[t] = 0;
+  __syncthreads();
+  const int base = blockIdx.x * kSortTile;
+  const unsigned long long lt = lanemask_lt();
+  for (int r = 0; r < kSortItems; ++r) {
+    const int idx = base + r * kSortThreads + t;
+    const bool valid = idx < n;
+    const unsigned d = valid ? (((unsigned)keys[idx]) >> shift) & (kRadix - 1) : 0u;
+    const unsigned long long peers = match_digit(d, valid);
+    if (valid && (peers & lt) == 0ull) atomicAdd(&cnt[d], (int)__popcll(peers));
+  }
+  __syncthreads();
+  hist[t * nblocks + blockIdx.x] = cnt[t];
+}
+
+__global__ void __launch_bounds__(kSortThreads) radix_rowscan(int *__restrict__ hist, int nblocks,
+                                                              int *__restrict__ digit_total) {
+  __shared__ int tmp[4];
+  int *row = hist + blockIdx.x * nblocks;
+  int carry = 0;
+  for (int s = 0; s < nblocks; s += kSortThreads) {
+    const int i = s + threadIdx.x;
+    const int v = i < nblocks ? row[i] : 0;
+    int tot;
+    const int ex = block_exscan256(v, tmp, &tot);
+    if (i < nblocks) row[i] = carry + ex;
+    carry += tot;
+  }
+  if (threadIdx.x == 0) digit_total[blockIdx.x] = carry;
+}
+
+__global__ void __launch_bounds__(kSortThreads) radix_downsweep(const int *__restrict__ kin,
+                                                                const int *__restrict__ vin,
+                                                                int *__restrict__ kout, int *__restrict__ vout,
+                                                                int n, int shift, int nblocks,
+                                                                const int *__restrict__ hist,
+                                                                const int *__restrict__ digit_total) {
+  __shared__ int base[kRadix];
+  __shared__ int woff[4][kRadix];
+  __shared__ int tmp[4];
+  const int t = threadIdx.x, w = t >> 6;
+  int tot;
+  const int ex = block_exscan256(digit_total[t], tmp, &tot);
+  base[t] = ex + hist[t * nblocks + blockIdx.x];
+  const int tile = blockIdx.x * kSortTile;
+  const unsigned long long lt = lanemask_lt();
+  for (int r = 0; r < kSortItems; ++r) {
+    const int idx = tile + r * kSortThreads + t;
+    const bool valid = idx < n;
+    const int k = valid ? kin[idx] : 0;
+    const int v = valid ? (vin ? vin[idx] : idx) : 0;
+    const unsigned d = (((unsigned)k) >> shift) & (kRadix - 1);
+    const unsigned long long peers = match_digit(d, valid);
+    const int rank = (int)__popcll(peers & lt);
+    woff[0][t] = 0;
+    woff[1][t] = 0;
+    woff[2][t] = 0;
+    woff[3][t] = 0;
+    __syncthreads();
+    if (valid && rank == 0) woff[w][d] = (int)__popcll(peers);
+    __syncthreads();
+    {
+      const int c0 = woff[0][t], c1 = woff[1][t], c2 = woff[2][t], c3 = woff[3][t];
+      const int b = base[t];
+      woff[0][t] = b;
+      woff[1][t] = b + c0;
+      woff[2][t] = b + c0 + c1;
+      woff[3][t] = b + c0 + c1 + c2;
+      base[t] = b + c0 + c1 + c2 + c3;
+    }
+    __syncthreads();
+    if (valid) {
+      const int pos = woff[w][d] + rank;
+      kout[pos] = k;
+      vout[pos] = v;
+    }
+    __syncthreads();
+  }
+}
+
+// Work buffers for one sorter instance (sized for n items).
+struct RadixSorter {
+  int n = 0, nblocks = 0, passes = 0;
+  int *k1 = nullptr, *v0 = nullptr, *v1 = nullptr, *hist = nullptr, *dtot = nullptr;
+};
+
+inline int radix_passes_for(int max_key) {
+  int bits = 0;
+  while (bits < 31 && (max_key >> bits) != 0) ++bits;
+  int p = (bits + kRadixBits - 1) / kRadixBits;
+  return p < 1 ? 1 : p;
+}
+
+// Sort (keys[i], i) for i < n.  keys is overwritten on even pass counts.  On return *out_keys /
+// *out_vals point at the sorted keys and values (inside keys / the sorter's buffers).
+inline hipError_t radix_sort_pairs(const RadixSorter &S, int *keys, int **out_keys, int **out_vals,
+                                   hipStream_t st) {
+  if (S.n == 0) {
+    *out_keys = keys;
+    *out_vals = S.v0;
+    return hipSuccess;
+  }
+  int *ka = keys, *kb = S.k1;
+  int *va = nullptr, *vb = S.v1;
+  for (int p = 0; p < S.passes; ++p) {
+    const int shift = p * kRadixBits;
+    radix_upsweep<<<S.nblocks, kSortThreads, 0, st>>>(ka, S.n, shift, S.nblocks, S.hist);
+    radix_rowscan<<<kRadix, kSortThreads, 0, st>>>(S.hist, S.nblocks, S.dtot);
+    radix_downsweep<<<S.nblocks, kSortThreads, 0, st>>>(ka, va, kb, vb, S.n, shift, S.nblocks, S.hist, S.dtot);
+    int *tk = ka;
+    ka = kb;
+    kb = tk;
+    int *tv = va;
+    va = vb;
+    vb = (p == 0) ? S.v0 : tv;
+  }
+  *out_keys = ka;
+  *out_vals = va;
+  return hipGetLastError();
+}
+
+}  // namespace sphnj

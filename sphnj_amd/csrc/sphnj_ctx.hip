@@ -1,0 +1,1197 @@
+// libsphnj context: device memory, step orchestration, hipGraph capture, and the C ABI of
+// include/sphnj.h.  Replaces Sim (include/sim.cuh), DeviceCalculation::calculation
+// (src/calculation.cu:20-48) and the __constant__ parameter / pointer tables
+// (src/memory_copy.cu:5-12) with an opaque per-context engine.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "host.hpp"
+#include "radix_sort.hpp"
+#include "sph_kernels.hpp"
+
+using namespace sphnj;
+
+namespace {
+
+#define HIPC(expr)                                                                  \
+  do {                                                                              \
+    hipError_t e_ = (expr);                                                         \
+    if (e_ != hipSuccess) {                                                         \
+      err = std::string(#expr) + ": " + hipGetErrorString(e_);                      \
+      return SPHNJ_E_HIP;                                                           \
+    }                                                                               \
+  } while (0)
+
+#define LAUNCH_CHECK()                                                              \
+  do {                                                                              \
+    hipError_t e_ = hipGetLastError();                                              \
+    if (e_ != hipSuccess) {                                                         \
+      err = std::string("kernel launch: ") + hipGetErrorString(e_);                 \
+      return SPHNJ_E_HIP;                                                           \
+    }                                                                               \
+  } while (0)
+
+inline int nblk(int n, int b) { return n <= 0 ? 0 : (n + b - 1) / b; }
+
+// Host copy of one uploaded case.
+struct Staged {
+  bool uploaded = false;
+  std::vector<double> fx, fy, fu, fv, frho, fp;
+  std::vector<int8_t> fph;
+  std::vector<double> sx, sy, su, sv, sX, sY;
+  std::vector<int8_t> smat, sbc;
+  std::vector<double> wx, wy;
+  std::vector<int8_t> wbc;
+};
+
+struct EngineBase {
+  std::string err;
+  hipStream_t stream = nullptr;
+  bool use_graph = true;
+  int blk_pair = 256, blk_shift = 256, blk_stream = 256;
+  virtual ~EngineBase() {}
+  virtual int init(const std::vector<Derived> &D, const std::vector<Staged> &S) = 0;
+  virtual int stage(int st) = 0;
+  virtual int step(int n) = 0;
+  virtual int download(int c, const sphnj_fluid_out *f, const sphnj_solid_out *s, const sphnj_wall_out *w) = 0;
+  virtual int status(uint32_t *flags) = 0;
+  virtual int counts(int c, int *nf, int *ns, int *nw, int *ncell) const = 0;
+  virtual int sorted(int c, int species, int *keys, int *orig, int *start) = 0;
+  virtual int neighbors(int c, int *counts, int *list, int64_t cap, int64_t *total) = 0;
+  virtual int reflist(int c, int *off, int *list, int64_t cap) = 0;
+  virtual int energy(int c, double out[6]) = 0;
+  virtual void drop_graphs() = 0;
+};
+
+template <class Pol>
+struct Engine : EngineBase {
+  using R = typename Pol::R;
+  Dev<Pol> d{};
+  std::vector<CaseDev<R>> hc;
+  std::vector<int> f_off, s_off, w_off;  // ncase + 1
+  std::vector<Derived> Dh;
+  int ncase = 0, total_cells = 0, S = 1, K = 0;
+  bool viscid = false;
+  RadixSorter fsort, ssort;
+  int *f_sorted_keys = nullptr, *s_sorted_keys = nullptr;
+  std::vector<void *> allocs;
+  struct GraphEntry {
+    hipGraphExec_t exec;
+    Dev<Pol> post;
+  };
+  std::map<std::array<const void *, 6>, GraphEntry> graphs;
+
+  ~Engine() override {
+    drop_graphs();
+    for (void *p : allocs) (void)hipFree(p);
+  }
+
+  void drop_graphs() override {
+    for (auto &kv : graphs) (void)hipGraphExecDestroy(kv.second.exec);
+    graphs.clear();
+  }
+
+  template <class T>
+  int dalloc(T **p, size_t n) {
+    void *q = nullptr;
+    HIPC(hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T)));
+    allocs.push_back(q);
+    *p = (T *)q;
+    return SPHNJ_OK;
+  }
+  template <class T>
+  int h2d(T *dst, const T *src, size_t n) {
+    if (n) HIPC(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyHostToDevice, stream));
+    return SPHNJ_OK;
+  }
+  template <class T>
+  int d2h(T *dst, const T *src, size_t n) {
+    if (n) HIPC(hipMemcpyAsync(dst, src, n * sizeof(T), hipMemcpyDeviceToHost, stream));
+    return SPHNJ_OK;
+  }
+
+  int sorter_alloc(RadixSorter &s, int n, int max_key) {
+    s.n = n;
+    s.nblocks = nblk(n, kSortTile);
+    s.passes = radix_passes_for(max_key);
+    int rc;
+    if ((rc = dalloc(&s.k1, n)) || (rc = dalloc(&s.v0, n)) || (rc = dalloc(&s.v1, n)) ||
+        (rc = dalloc(&s.hist, (size_t)kRadix * std::max(s.nblocks, 1))) || (rc = dalloc(&s.dtot, kRadix)))
+      return rc;
+    return SPHNJ_OK;
+  }
+
+  void fill_case(const Derived &D, CaseDev<R> &c) {
+    std::memset(&c, 0, sizeof(c));
+    c.rc2 = (R)D.rc2;
+    c.quarter_rc2 = (R)(0.25 * D.rc2);
+    c.hinv2 = (R)D.hinv2;
+    c.kdc = (R)D.kdc;
+    c.k2c = (R)D.k2c;
+    c.eR = (R)D.eR;
+    c.h = (R)D.h;
+    c.eps_h2 = (R)(0.01 * D.h2);
+    for (int k = 0; k < 2; ++k) {
+      c.c[k] = (R)D.c[k];
+      c.cinv[k] = (R)(1.0 / D.c[k]);
+      c.mu8[k] = D.viscid ? (R)(8.0 * D.mu[k]) : (R)0;
+      c.rho0[k] = (R)D.rho0[k];
+      c.rho0_d[k] = D.rho0[k];
+      c.rho0_inv_d[k] = D.rho0_inv[k];
+      c.gam_d[k] = D.gamma[k];
+      c.ginv_d[k] = D.gamma_inv[k];
+      c.rho2p_d[k] = D.rho2p[k];
+      c.p2rho_d[k] = D.p2rho[k];
+      c.rho_min_d[k] = D.rho_min[k];
+    }
+    c.p_min_d = D.p_min;
+    c.p_bg_d = D.p_bg;
+    c.p_min = (R)D.p_min;
+    c.gx = (R)D.gx;
+    c.gy = (R)D.gy;
+    c.gx_d = D.gx;
+    c.gy_d = D.gy;
+    c.dt = D.dt;
+    c.dt_s = D.dt_s;
+    c.shift_k = (R)(D.shift_coef * D.h * D.dt);
+    c.umax_cap = (R)D.umax_cap;
+    c.cxmin = D.cxmin;
+    c.cymin = D.cymin;
+    c.cdx = D.cdx;
+    c.cdy = D.cdy;
+    c.inv_cdx = D.inv_cdx;
+    c.inv_cdy = D.inv_cdy;
+    c.cdx_r = (R)D.cdx;
+    c.cdy_r = (R)D.cdy;
+    c.dxmin = D.dxmin;
+    c.dymin = D.dymin;
+    c.dxmax = D.dxmax;
+    c.dymax = D.dymax;
+    c.nx = D.nx;
+    c.ny = D.ny;
+    c.ncell = D.nx * D.ny;
+    c.steady = D.steady;
+    c.viscid = D.viscid;
+    c.lambda = (R)D.lambda;
+    c.mu_s = (R)D.mu_s;
+    c.c_s = (R)D.c_s;
+    c.alpha = (R)D.alpha;
+    c.beta = (R)D.beta;
+    c.rho_s_inv = (R)(1.0 / D.rho_s);
+    c.rho_aux_inv = (R)(1.0 / D.rho_aux);
+    c.fixed_u = D.fixed_u;
+    c.fixed_v = D.fixed_v;
+    c.aux_mod = (R)D.aux_modulus;
+    c.yield_stress = (R)D.yield_stress;
+    c.harden_strain = (R)D.harden_strain;
+    c.harden_coef = (R)D.harden_coef;
+    c.plastic = D.plastic;
+  }
+
+  // ---- init -----------------------------------------------------------------------------
+  int init(const std::vector<Derived> &D, const std::vector<Staged> &St) override {
+    int rc;
+    Dh = D;
+    ncase = (int)D.size();
+    S = D[0].S;
+    K = D[0].K;
+    f_off.assign(ncase + 1, 0);
+    s_off.assign(ncase + 1, 0);
+    w_off.assign(ncase + 1, 0);
+    hc.resize(ncase);
+    total_cells = 0;
+    for (int c = 0; c < ncase; ++c) {
+      fill_case(D[c], hc[c]);
+      viscid = viscid || D[c].viscid;
+      hc[c].cell_off = total_cells;
+      total_cells += hc[c].ncell;
+      f_off[c + 1] = f_off[c] + (int)St[c].fx.size();
+      s_off[c + 1] = s_off[c] + (int)St[c].sx.size();
+      w_off[c + 1] = w_off[c] + (int)St[c].wx.size();
+      hc[c].f_off = f_off[c];
+      hc[c].f_n = f_off[c + 1] - f_off[c];
+      hc[c].s_off = s_off[c];
+      hc[c].s_n = s_off[c + 1] - s_off[c];
+      hc[c].w_off = w_off[c];
+      hc[c].w_n = w_off[c + 1] - w_off[c];
+    }
+    const int nf = f_off[ncase], ns = s_off[ncase], nw = w_off[ncase], nc = total_cells;
+    d.ncase = ncase;
+    d.total_cells = nc;
+    d.nf = nf;
+    d.ns = ns;
+    d.nw = nw;
+    CaseDev<R> *cases;
+    int *fco, *sco, *wco;
+    if ((rc = dalloc(&cases, ncase)) || (rc = dalloc(&fco, ncase + 1)) || (rc = dalloc(&sco, ncase + 1)) ||
+        (rc = dalloc(&wco, ncase + 1)))
+      return rc;
+    if ((rc = h2d(cases, hc.data(), ncase)) || (rc = h2d(fco, f_off.data(), ncase + 1)) ||
+        (rc = h2d(sco, s_off.data(), ncase + 1)) || (rc = h2d(wco, w_off.data(), ncase + 1)))
+      return rc;
+    d.cases = cases;
+    d.f_case_off = fco;
+    d.s_case_off = sco;
+    d.w_case_off = wco;
+
+    // ---- fluid
+    for (FluidBuf<Pol> *B : {&d.fa, &d.fb}) {
+      if ((rc = dalloc(&B->x, nf)) || (rc = dalloc(&B->y, nf)) || (rc = dalloc(&B->u, nf)) ||
+          (rc = dalloc(&B->v, nf)) || (rc = dalloc(&B->rho, nf)) || (rc = dalloc(&B->p, nf)))
+        return rc;
+      if constexpr (Pol::MIXED) {
+        if ((rc = dalloc(&B->vx, nf)) || (rc = dalloc(&B->vy, nf)) || (rc = dalloc(&B->vu, nf)) ||
+            (rc = dalloc(&B->vv, nf)) || (rc = dalloc(&B->vrho, nf)))
+          return rc;
+      } else {
+        B->vx = B->x;
+        B->vy = B->y;
+        B->vu = B->u;
+        B->vv = B->v;
+        B->vrho = B->rho;
+      }
+    }
+    if ((rc = dalloc(&d.f_phase, nf)) || (rc = dalloc(&d.f_phase2, nf)) || (rc = dalloc(&d.f_cls, nf)) ||
+        (rc = dalloc(&d.f_cls2, nf)) || (rc = dalloc(&d.f_orig, nf)) || (rc = dalloc(&d.f_cxy, nf)) ||
+        (rc = dalloc(&d.f_key_by_orig, nf)) || (rc = dalloc(&d.f_slot_of_orig, nf)) ||
+        (rc = dalloc(&d.f_col, nf)) || (rc = dalloc(&d.f_near, nf)) || (rc = dalloc(&d.f_L, 4 * (size_t)nf)) ||
+        (rc = dalloc(&d.f_range, nc)) || (rc = dalloc(&d.umax2, ncase)) || (rc = dalloc(&d.status, 1)))
+      return rc;
+    {
+      std::vector<double> x, y, u, v, rho;
+      std::vector<R> p;
+      std::vector<int8_t> ph;
+      for (int c = 0; c < ncase; ++c) {
+        const Staged &s = St[c];
+        x.insert(x.end(), s.fx.begin(), s.fx.end());
+        y.insert(y.end(), s.fy.begin(), s.fy.end());
+        u.insert(u.end(), s.fu.begin(), s.fu.end());
+        v.insert(v.end(), s.fv.begin(), s.fv.end());
+        rho.insert(rho.end(), s.frho.begin(), s.frho.end());
+        for (double q : s.fp) p.push_back((R)q);
+        ph.insert(ph.end(), s.fph.begin(), s.fph.end());
+      }
+      std::vector<int> orig(nf);
+      std::iota(orig.begin(), orig.end(), 0);
+      std::vector<int8_t> cls(nf, (int8_t)SPHNJ_FREE_SURFACE), nearv(nf, 0);
+      std::vector<R> zero4(4 * (size_t)nf, (R)0), zero1(nf, (R)0);
+      if ((rc = h2d(d.fa.x, x.data(), nf)) || (rc = h2d(d.fa.y, y.data(), nf)) || (rc = h2d(d.fa.u, u.data(), nf)) ||
+          (rc = h2d(d.fa.v, v.data(), nf)) || (rc = h2d(d.fa.rho, rho.data(), nf)) ||
+          (rc = h2d(d.fa.p, p.data(), nf)) || (rc = h2d(d.f_phase, ph.data(), nf)) ||
+          (rc = h2d(d.f_cls, cls.data(), nf)) || (rc = h2d(d.f_orig, orig.data(), nf)) ||
+          (rc = h2d(d.f_near, nearv.data(), nf)) || (rc = h2d(d.f_col, zero1.data(), nf)) ||
+          (rc = h2d(d.f_L, zero4.data(), 4 * (size_t)nf)))
+        return rc;
+      HIPC(hipStreamSynchronize(stream));
+    }
+    if ((rc = sorter_alloc(fsort, nf, std::max(nc - 1, 0)))) return rc;
+
+    // ---- solid
+    if ((rc = dalloc(&d.sx, ns)) || (rc = dalloc(&d.sy, ns)) || (rc = dalloc(&d.su, ns)) || (rc = dalloc(&d.sv, ns)) ||
+        (rc = dalloc(&d.sx2, ns)) || (rc = dalloc(&d.sy2, ns)) || (rc = dalloc(&d.su2, ns)) ||
+        (rc = dalloc(&d.sv2, ns)) || (rc = dalloc(&d.sX, ns)) || (rc = dalloc(&d.sY, ns)) ||
+        (rc = dalloc(&d.s_mat, ns)) || (rc = dalloc(&d.s_bc, ns)) || (rc = dalloc(&d.s_p, ns)) ||
+        (rc = dalloc(&d.s_n, 2 * (size_t)ns)) || (rc = dalloc(&d.s_F, 4 * (size_t)ns)) ||
+        (rc = dalloc(&d.s_L, 4 * (size_t)ns)) || (rc = dalloc(&d.s_P, 4 * (size_t)ns)) ||
+        (rc = dalloc(&d.s_PL, 4 * (size_t)ns)) || (rc = dalloc(&d.s_M, 4 * (size_t)ns)) ||
+        (rc = dalloc(&d.s_Lref, 4 * (size_t)ns)) || (rc = dalloc(&d.s_Bt, 4 * (size_t)ns)) ||
+        (rc = dalloc(&d.s_degen, ns)) || (rc = dalloc(&d.s_a, 2 * (size_t)ns)) ||
+        (rc = dalloc(&d.s_cp, 2 * (size_t)ns)) || (rc = dalloc(&d.s_key, ns)) || (rc = dalloc(&d.s_cxy, ns)) ||
+        (rc = dalloc(&d.s_range, nc)) || (rc = dalloc(&d.s_ref_off, ns + 1)))
+      return rc;
+    if constexpr (Pol::MIXED) {
+      if ((rc = dalloc(&d.s_vx, ns)) || (rc = dalloc(&d.s_vy, ns)) || (rc = dalloc(&d.s_vx2, ns)) ||
+          (rc = dalloc(&d.s_vy2, ns)))
+        return rc;
+    } else {
+      d.s_vx = d.sx;
+      d.s_vy = d.sy;
+      d.s_vx2 = d.sx2;
+      d.s_vy2 = d.sy2;
+    }
+    {
+      std::vector<double> x, y, u, v, X, Y;
+      std::vector<int8_t> mat, bc;
+      std::vector<R> I4, Z4, M4, Z2, Z1;
+      for (int c = 0; c < ncase; ++c) {
+        const Staged &s = St[c];
+        x.insert(x.end(), s.sx.begin(), s.sx.end());
+        y.insert(y.end(), s.sy.begin(), s.sy.end());
+        u.insert(u.end(), s.su.begin(), s.su.end());
+        v.insert(v.end(), s.sv.begin(), s.sv.end());
+        X.insert(X.end(), s.sX.begin(), s.sX.end());
+        Y.insert(Y.end(), s.sY.begin(), s.sY.end());
+        mat.insert(mat.end(), s.smat.begin(), s.smat.end());
+        bc.insert(bc.end(), s.sbc.begin(), s.sbc.end());
+        for (size_t i = 0; i < s.sx.size(); ++i) {  // include/host_data.cuh:77-83
+          const R ys = (R)(-D[c].yield_strain);
+          M4.insert(M4.end(), {ys, (R)0, (R)0, ys});
+        }
+      }
+      for (int i = 0; i < ns; ++i) {
+        I4.insert(I4.end(), {(R)1, (R)0, (R)0, (R)1});
+        Z4.insert(Z4.end(), {(R)0, (R)0, (R)0, (R)0});
+      }
+      Z2.assign(2 * (size_t)ns, (R)0);
+      Z1.assign(ns, (R)0);
+      std::vector<double> Z2d(2 * (size_t)ns, 0.0);
+      if ((rc = h2d(d.sx, x.data(), ns)) || (rc = h2d(d.sy, y.data(), ns)) || (rc = h2d(d.su, u.data(), ns)) ||
+          (rc = h2d(d.sv, v.data(), ns)) || (rc = h2d(d.sX, X.data(), ns)) || (rc = h2d(d.sY, Y.data(), ns)) ||
+          (rc = h2d(d.s_mat, mat.data(), ns)) || (rc = h2d(d.s_bc, bc.data(), ns)) ||
+          (rc = h2d(d.s_F, I4.data(), 4 * (size_t)ns)) || (rc = h2d(d.s_L, I4.data(), 4 * (size_t)ns)) ||
+          (rc = h2d(d.s_P, Z4.data(), 4 * (size_t)ns)) || (rc = h2d(d.s_PL, Z4.data(), 4 * (size_t)ns)) ||
+          (rc = h2d(d.s_M, M4.data(), 4 * (size_t)ns)) || (rc = h2d(d.s_n, Z2.data(), 2 * (size_t)ns)) ||
+          (rc = h2d(d.s_p, Z1.data(), ns)) || (rc = h2d(d.s_a, Z2d.data(), 2 * (size_t)ns)) ||
+          (rc = h2d(d.s_cp, Z2.data(), 2 * (size_t)ns)))
+        return rc;
+      HIPC(hipStreamSynchronize(stream));
+    }
+    if ((rc = sorter_alloc(ssort, ns, std::max(nc - 1, 0)))) return rc;
+
+    // ---- wall
+    if ((rc = dalloc(&d.wx, nw)) || (rc = dalloc(&d.wy, nw)) || (rc = dalloc(&d.w_u, nw)) || (rc = dalloc(&d.w_v, nw)) ||
+        (rc = dalloc(&d.w_p, nw)) || (rc = dalloc(&d.w_rho, nw)) || (rc = dalloc(&d.w_n, 2 * (size_t)nw)) ||
+        (rc = dalloc(&d.w_phase, nw)) || (rc = dalloc(&d.w_bc, nw)) || (rc = dalloc(&d.w_orig, nw)) ||
+        (rc = dalloc(&d.w_cxy, nw)) || (rc = dalloc(&d.w_range, nc)))
+      return rc;
+    if constexpr (Pol::MIXED) {
+      if ((rc = dalloc(&d.w_vx, nw)) || (rc = dalloc(&d.w_vy, nw))) return rc;
+    } else {
+      d.w_vx = d.wx;
+      d.w_vy = d.wy;
+    }
+    HIPC(hipMemsetAsync(d.status, 0, sizeof(unsigned), stream));
+    {
+      std::vector<double> x, y;
+      std::vector<int8_t> bc;
+      std::vector<R> rho0, z;
+      for (int c = 0; c < ncase; ++c) {
+        x.insert(x.end(), St[c].wx.begin(), St[c].wx.end());
+        y.insert(y.end(), St[c].wy.begin(), St[c].wy.end());
+        bc.insert(bc.end(), St[c].wbc.begin(), St[c].wbc.end());
+        rho0.insert(rho0.end(), St[c].wx.size(), (R)D[c].rho0[0]);  // include/host_data.cuh:111
+      }
+      z.assign(2 * (size_t)nw, (R)0);
+      std::vector<int8_t> ph(nw, 0);
+      double *x0, *y0;
+      int8_t *bc0;
+      int *keys;
+      RadixSorter ws;
+      if ((rc = dalloc(&x0, nw)) || (rc = dalloc(&y0, nw)) || (rc = dalloc(&bc0, nw)) || (rc = dalloc(&keys, nw)) ||
+          (rc = sorter_alloc(ws, nw, std::max(nc - 1, 0))))
+        return rc;
+      if ((rc = h2d(x0, x.data(), nw)) || (rc = h2d(y0, y.data(), nw)) || (rc = h2d(bc0, bc.data(), nw)) ||
+          (rc = h2d(d.w_rho, rho0.data(), nw)) || (rc = h2d(d.w_u, z.data(), nw)) || (rc = h2d(d.w_v, z.data(), nw)) ||
+          (rc = h2d(d.w_p, z.data(), nw)) || (rc = h2d(d.w_n, z.data(), 2 * (size_t)nw)) ||
+          (rc = h2d(d.w_phase, ph.data(), nw)))
+        return rc;
+      if (nw > 0) {
+        k_hash_wall<Pol><<<nblk(nw, 256), 256, 0, stream>>>(d, x0, y0, keys);
+        LAUNCH_CHECK();
+        int *sk, *sv;
+        HIPC(radix_sort_pairs(ws, keys, &sk, &sv, stream));
+        k_permute_wall<Pol><<<nblk(nw, 256), 256, 0, stream>>>(d, sv, x0, y0, bc0);
+        LAUNCH_CHECK();
+        k_clear_ranges<<<nblk(nc, 256), 256, 0, stream>>>(d.w_range, nullptr, nc);
+        k_cell_bounds<<<nblk(nw, 256), 256, 0, stream>>>(sk, nw, d.w_range);
+        k_wall_normal<Pol><<<nblk(nw, 256), 256, 0, stream>>>(d);
+        LAUNCH_CHECK();
+      } else {
+        k_clear_ranges<<<nblk(nc, 256), 256, 0, stream>>>(d.w_range, nullptr, nc);
+        LAUNCH_CHECK();
+      }
+      HIPC(hipStreamSynchronize(stream));
+    }
+
+    // ---- solid reference-configuration lists (built from X, Y)
+    {
+      std::vector<int> off(ns + 1, 0);
+      if (ns > 0) {
+        int *keys, *cxy, *cnt;
+        int2 *range;
+        double *kdc, *hinv2, *rc2;
+        RadixSorter rs;
+        if ((rc = dalloc(&keys, ns)) || (rc = dalloc(&cxy, ns)) || (rc = dalloc(&cnt, ns)) ||
+            (rc = dalloc(&range, nc)) || (rc = dalloc(&kdc, ncase)) || (rc = dalloc(&hinv2, ncase)) ||
+            (rc = dalloc(&rc2, ncase)) || (rc = sorter_alloc(rs, ns, std::max(nc - 1, 0))))
+          return rc;
+        std::vector<double> hk, hh, hr;
+        for (auto &q : D) {
+          hk.push_back(q.kdc);
+          hh.push_back(q.hinv2);
+          hr.push_back(q.rc2);
+        }
+        if ((rc = h2d(kdc, hk.data(), ncase)) || (rc = h2d(hinv2, hh.data(), ncase)) || (rc = h2d(rc2, hr.data(), ncase)))
+          return rc;
+        k_hash_solid<Pol><<<nblk(ns, 256), 256, 0, stream>>>(d, d.sX, d.sY, keys, cxy, 0);
+        LAUNCH_CHECK();
+        int *sk, *sv;
+        HIPC(radix_sort_pairs(rs, keys, &sk, &sv, stream));
+        k_clear_ranges<<<nblk(nc, 256), 256, 0, stream>>>(range, nullptr, nc);
+        k_cell_bounds<<<nblk(ns, 256), 256, 0, stream>>>(sk, ns, range);
+        k_ref_count<Pol><<<nblk(ns, 256), 256, 0, stream>>>(d, cxy, range, sv, rc2, cnt);
+        LAUNCH_CHECK();
+        std::vector<int> hcnt(ns);
+        if ((rc = d2h(hcnt.data(), cnt, ns))) return rc;
+        HIPC(hipStreamSynchronize(stream));
+        for (int i = 0; i < ns; ++i) off[i + 1] = off[i] + hcnt[i];
+        if ((rc = h2d(d.s_ref_off, off.data(), ns + 1))) return rc;
+        if ((rc = dalloc(&d.s_ref_j, off[ns])) || (rc = dalloc(&d.s_ref_g, 2 * (size_t)off[ns]))) return rc;
+        k_ref_fill<Pol><<<nblk(ns, 256), 256, 0, stream>>>(d, cxy, range, sv, kdc, hinv2, rc2);
+        LAUNCH_CHECK();
+        HIPC(hipStreamSynchronize(stream));
+      } else {
+        if ((rc = h2d(d.s_ref_off, off.data(), 1))) return rc;
+        if ((rc = dalloc(&d.s_ref_j, 1)) || (rc = dalloc(&d.s_ref_g, 2))) return rc;
+      }
+    }
+    // empty fluid / solid cell tables until the first CELLS stage
+    k_clear_ranges<<<nblk(nc, 256), 256, 0, stream>>>(d.f_range, d.s_range, nc);
+    LAUNCH_CHECK();
+    HIPC(hipStreamSynchronize(stream));
+    return SPHNJ_OK;
+  }
+
+  // ---- stages -----------------------------------------------------------------------------
+  void swap_fluid_all() {
+    std::swap(d.fa, d.fb);
+  }
+
+  int stage_cells() {
+    const int nf = d.nf, ns = d.ns, nc = d.total_cells;
+    if (nf > 0) {
+      k_hash_fluid<Pol><<<nblk(nf, blk_stream), blk_stream, 0, stream>>>(d);
+      LAUNCH_CHECK();
+      int *sv;
+      HIPC(radix_sort_pairs(fsort, d.f_key_by_orig, &f_sorted_keys, &sv, stream));
+      k_permute_fluid<Pol><<<nblk(std::max(nf, ncase), blk_stream), blk_stream, 0, stream>>>(d, sv);
+      LAUNCH_CHECK();
+      swap_fluid_all();
+      std::swap(d.f_phase, d.f_phase2);
+      std::swap(d.f_cls, d.f_cls2);
+    }
+    if (ns > 0) {
+      k_hash_solid<Pol><<<nblk(ns, blk_stream), blk_stream, 0, stream>>>(d, d.sx, d.sy, d.s_key, d.s_cxy, 1);
+      LAUNCH_CHECK();
+      HIPC(radix_sort_pairs(ssort, d.s_key, &s_sorted_keys, &d.s_sorted, stream));
+    }
+    k_clear_ranges<<<nblk(nc, 256), 256, 0, stream>>>(d.f_range, d.s_range, nc);
+    LAUNCH_CHECK();
+    if (nf > 0) k_cell_bounds<<<nblk(nf, 256), 256, 0, stream>>>(f_sorted_keys, nf, d.f_range);
+    if (ns > 0) k_cell_bounds<<<nblk(ns, 256), 256, 0, stream>>>(s_sorted_keys, ns, d.s_range);
+    k_cell_audit<<<nblk(nc, 256), 256, 0, stream>>>(d.f_range, d.s_range, d.w_range, nc, d.status);
+    LAUNCH_CHECK();
+    return SPHNJ_OK;
+  }
+
+  int stage(int st) override {
+    const int nf = d.nf, ns = d.ns, nw = d.nw;
+    switch (st) {
+      case SPHNJ_STAGE_CELLS:
+        return stage_cells();
+      case SPHNJ_STAGE_COUPLE:
+        if (nf > 0 && ns > 0) {
+          k_solid_couple<Pol><<<nblk(ns, blk_stream), blk_stream, 0, stream>>>(d);
+          LAUNCH_CHECK();
+        }
+        return SPHNJ_OK;
+      case SPHNJ_STAGE_WALL:
+        if (nw > 0) {
+          k_wall_update<Pol><<<nblk(nw, blk_stream), blk_stream, 0, stream>>>(d);
+          LAUNCH_CHECK();
+        }
+        return SPHNJ_OK;
+      case SPHNJ_STAGE_FLUID:
+        if (nf > 0) {
+          if (viscid)
+            k_fluid_pair_update<Pol, true><<<nblk(nf, blk_pair), blk_pair, 0, stream>>>(d);
+          else
+            k_fluid_pair_update<Pol, false><<<nblk(nf, blk_pair), blk_pair, 0, stream>>>(d);
+          LAUNCH_CHECK();
+          swap_fluid_all();
+        }
+        return SPHNJ_OK;
+      case SPHNJ_STAGE_SOLID:
+        if (ns > 0) {
+          for (int k = 0; k < S; ++k) {
+            k_solid_deform<Pol><<<nblk(ns, blk_stream), blk_stream, 0, stream>>>(d);
+            k_solid_force_update<Pol><<<nblk(ns, blk_stream), blk_stream, 0, stream>>>(d);
+            LAUNCH_CHECK();
+            std::swap(d.sx, d.sx2);
+            std::swap(d.sy, d.sy2);
+            std::swap(d.su, d.su2);
+            std::swap(d.sv, d.sv2);
+            std::swap(d.s_vx, d.s_vx2);
+            std::swap(d.s_vy, d.s_vy2);
+          }
+        }
+        return SPHNJ_OK;
+      case SPHNJ_STAGE_SHIFT:
+        if (nf > 0) {
+          for (int k = 0; k < K; ++k) {
+            k_fluid_shift<Pol><<<nblk(nf, blk_shift), blk_shift, 0, stream>>>(d);
+            LAUNCH_CHECK();
+            std::swap(d.fa.x, d.fb.x);
+            std::swap(d.fa.y, d.fb.y);
+            std::swap(d.fa.vx, d.fb.vx);
+            std::swap(d.fa.vy, d.fb.vy);
+          }
+        }
+        return SPHNJ_OK;
+    }
+    err = "unknown stage";
+    return SPHNJ_E_ARG;
+  }
+
+  int enqueue_step() {
+    for (int st = SPHNJ_STAGE_CELLS; st <= SPHNJ_STAGE_SHIFT; ++st) {
+      const int rc = stage(st);
+      if (rc) return rc;
+    }
+    return SPHNJ_OK;
+  }
+
+  std::array<const void *, 6> state_key() const {
+    return {d.fa.x, d.fa.u, d.f_phase, d.sx, d.fa.vx, d.s_vx};
+  }
+
+  int step(int n) override {
+    for (int s = 0; s < n; ++s) {
+      if (!use_graph) {
+        const int rc = enqueue_step();
+        if (rc) return rc;
+        continue;
+      }
+      const auto key = state_key();
+      auto it = graphs.find(key);
+      if (it == graphs.end()) {
+        hipGraph_t g;
+        HIPC(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_step();
+        hipGraph_t gtmp = nullptr;
+        const hipError_t ec = hipStreamEndCapture(stream, &gtmp);
+        if (rc) {
+          if (gtmp) (void)hipGraphDestroy(gtmp);
+          return rc;
+        }
+        if (ec != hipSuccess) {
+          err = std::string("hipStreamEndCapture: ") + hipGetErrorString(ec);
+          return SPHNJ_E_HIP;
+        }
+        g = gtmp;
+        hipGraphExec_t ex;
+        const hipError_t ei = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+          err = std::string("hipGraphInstantiate: ") + hipGetErrorString(ei);
+          return SPHNJ_E_HIP;
+        }
+        it = graphs.emplace(key, GraphEntry{ex, d}).first;
+      }
+      HIPC(hipGraphLaunch(it->second.exec, stream));
+      d = it->second.post;
+    }
+    return SPHNJ_OK;
+  }
+
+  // ---- inspection -----------------------------------------------------------------------
+  int status(uint32_t *flags) override {
+    unsigned h = 0;
+    int rc;
+    if ((rc = d2h(&h, d.status, 1))) return rc;
+    HIPC(hipStreamSynchronize(stream));
+    *flags = h;
+    return SPHNJ_OK;
+  }
+
+  int counts(int c, int *nf, int *ns, int *nw, int *ncell) const override {
+    if (nf) *nf = f_off[c + 1] - f_off[c];
+    if (ns) *ns = s_off[c + 1] - s_off[c];
+    if (nw) *nw = w_off[c + 1] - w_off[c];
+    if (ncell) *ncell = hc[c].ncell;
+    return SPHNJ_OK;
+  }
+
+  template <class T>
+  static void put_scatter(T *dst, const std::vector<T> &src, const std::vector<int> &orig, int lo, int n, int w) {
+    if (!dst) return;
+    for (int t = 0; t < n; ++t) {
+      const int o = orig[t] - lo;
+      for (int k = 0; k < w; ++k) dst[(size_t)w * o + k] = src[(size_t)w * t + k];
+    }
+  }
+  template <class T>
+  static void put_conv(double *dst, const std::vector<T> &src, const std::vector<int> &orig, int lo, int n, int w) {
+    if (!dst) return;
+    for (int t = 0; t < n; ++t) {
+      const int o = orig[t] - lo;
+      for (int k = 0; k < w; ++k) dst[(size_t)w * o + k] = (double)src[(size_t)w * t + k];
+    }
+  }
+  template <class T>
+  static void put_int(int32_t *dst, const std::vector<T> &src, const std::vector<int> &orig, int lo, int n) {
+    if (!dst) return;
+    for (int t = 0; t < n; ++t) dst[orig[t] - lo] = (int32_t)src[t];
+  }
+
+  int download(int c, const sphnj_fluid_out *f, const sphnj_solid_out *s, const sphnj_wall_out *w) override {
+    int rc;
+    if (f) {
+      const int lo = f_off[c], n = f_off[c + 1] - lo;
+      std::vector<double> x(n), y(n), u(n), v(n), rho(n);
+      std::vector<R> p(n), col(n), L(4 * (size_t)n);
+      std::vector<int8_t> ph(n), cls(n), nr(n);
+      std::vector<int> orig(n);
+      if ((rc = d2h(x.data(), d.fa.x + lo, n)) || (rc = d2h(y.data(), d.fa.y + lo, n)) ||
+          (rc = d2h(u.data(), d.fa.u + lo, n)) || (rc = d2h(v.data(), d.fa.v + lo, n)) ||
+          (rc = d2h(rho.data(), d.fa.rho + lo, n)) || (rc = d2h(p.data(), d.fa.p + lo, n)) ||
+          (rc = d2h(col.data(), d.f_col + lo, n)) || (rc = d2h(L.data(), d.f_L + 4 * (size_t)lo, 4 * (size_t)n)) ||
+          (rc = d2h(ph.data(), d.f_phase + lo, n)) || (rc = d2h(cls.data(), d.f_cls + lo, n)) ||
+          (rc = d2h(nr.data(), d.f_near + lo, n)) || (rc = d2h(orig.data(), d.f_orig + lo, n)))
+        return rc;
+      HIPC(hipStreamSynchronize(stream));
+      put_scatter(f->x, x, orig, lo, n, 1);
+      put_scatter(f->y, y, orig, lo, n, 1);
+      put_scatter(f->u, u, orig, lo, n, 1);
+      put_scatter(f->v, v, orig, lo, n, 1);
+      put_scatter(f->rho, rho, orig, lo, n, 1);
+      put_conv(f->p, p, orig, lo, n, 1);
+      put_conv(f->color, col, orig, lo, n, 1);
+      put_conv(f->du_dx, L, orig, lo, n, 4);
+      put_int(f->phase, ph, orig, lo, n);
+      put_int(f->boundary_type, cls, orig, lo, n);
+      put_int(f->near, nr, orig, lo, n);
+    }
+    if (s) {
+      const int lo = s_off[c], n = s_off[c + 1] - lo;
+      std::vector<int> orig(n);
+      std::iota(orig.begin(), orig.end(), lo);
+      std::vector<double> x(n), y(n), u(n), v(n), X(n), Y(n), a(2 * (size_t)n);
+      std::vector<R> p(n), nn(2 * (size_t)n), F(4 * (size_t)n), L(4 * (size_t)n), P(4 * (size_t)n),
+          M(4 * (size_t)n), cp(2 * (size_t)n);
+      if ((rc = d2h(x.data(), d.sx + lo, n)) || (rc = d2h(y.data(), d.sy + lo, n)) || (rc = d2h(u.data(), d.su + lo, n)) ||
+          (rc = d2h(v.data(), d.sv + lo, n)) || (rc = d2h(X.data(), d.sX + lo, n)) || (rc = d2h(Y.data(), d.sY + lo, n)) ||
+          (rc = d2h(a.data(), d.s_a + 2 * (size_t)lo, 2 * (size_t)n)) || (rc = d2h(p.data(), d.s_p + lo, n)) ||
+          (rc = d2h(nn.data(), d.s_n + 2 * (size_t)lo, 2 * (size_t)n)) ||
+          (rc = d2h(F.data(), d.s_F + 4 * (size_t)lo, 4 * (size_t)n)) ||
+          (rc = d2h(L.data(), d.s_L + 4 * (size_t)lo, 4 * (size_t)n)) ||
+          (rc = d2h(P.data(), d.s_P + 4 * (size_t)lo, 4 * (size_t)n)) ||
+          (rc = d2h(M.data(), d.s_M + 4 * (size_t)lo, 4 * (size_t)n)) ||
+          (rc = d2h(cp.data(), d.s_cp + 2 * (size_t)lo, 2 * (size_t)n)))
+        return rc;
+      HIPC(hipStreamSynchronize(stream));
+      put_scatter(s->x, x, orig, lo, n, 1);
+      put_scatter(s->y, y, orig, lo, n, 1);
+      put_scatter(s->u, u, orig, lo, n, 1);
+      put_scatter(s->v, v, orig, lo, n, 1);
+      put_scatter(s->X, X, orig, lo, n, 1);
+      put_scatter(s->Y, Y, orig, lo, n, 1);
+      put_scatter(s->du_dt, a, orig, lo, n, 2);
+      put_conv(s->pressure, p, orig, lo, n, 1);
+      put_conv(s->normal, nn, orig, lo, n, 2);
+      put_conv(s->deformation, F, orig, lo, n, 4);
+      put_conv(s->correction, L, orig, lo, n, 4);
+      put_conv(s->stress, P, orig, lo, n, 4);
+      put_conv(s->max_strain, M, orig, lo, n, 4);
+      put_conv(s->couple_dudt, cp, orig, lo, n, 2);
+    }
+    if (w) {
+      const int lo = w_off[c], n = w_off[c + 1] - lo;
+      std::vector<double> x(n), y(n);
+      std::vector<R> u(n), v(n), p(n), rho(n), nn(2 * (size_t)n);
+      std::vector<int8_t> ph(n), bc(n);
+      std::vector<int> orig(n);
+      if ((rc = d2h(x.data(), d.wx + lo, n)) || (rc = d2h(y.data(), d.wy + lo, n)) || (rc = d2h(u.data(), d.w_u + lo, n)) ||
+          (rc = d2h(v.data(), d.w_v + lo, n)) || (rc = d2h(p.data(), d.w_p + lo, n)) ||
+          (rc = d2h(rho.data(), d.w_rho + lo, n)) || (rc = d2h(nn.data(), d.w_n + 2 * (size_t)lo, 2 * (size_t)n)) ||
+          (rc = d2h(ph.data(), d.w_phase + lo, n)) || (rc = d2h(bc.data(), d.w_bc + lo, n)) ||
+          (rc = d2h(orig.data(), d.w_orig + lo, n)))
+        return rc;
+      HIPC(hipStreamSynchronize(stream));
+      put_scatter(w->x, x, orig, lo, n, 1);
+      put_scatter(w->y, y, orig, lo, n, 1);
+      put_conv(w->u, u, orig, lo, n, 1);
+      put_conv(w->v, v, orig, lo, n, 1);
+      put_conv(w->p, p, orig, lo, n, 1);
+      put_conv(w->rho, rho, orig, lo, n, 1);
+      put_conv(w->normal, nn, orig, lo, n, 2);
+      put_int(w->phase, ph, orig, lo, n);
+      put_int(w->bc, bc, orig, lo, n);
+    }
+    return SPHNJ_OK;
+  }
+
+  int sorted(int c, int species, int *keys, int *orig, int *start) override {
+    int rc;
+    const auto &cp = hc[c];
+    int lo, n;
+    std::vector<int> k, o;
+    std::vector<int2> rg(cp.ncell);
+    if (species == SPHNJ_FLUID) {
+      lo = f_off[c];
+      n = f_off[c + 1] - lo;
+      k.resize(n);
+      o.resize(n);
+      if (n && f_sorted_keys && (rc = d2h(k.data(), f_sorted_keys + lo, n))) return rc;
+      if ((rc = d2h(o.data(), d.f_orig + lo, n)) || (rc = d2h(rg.data(), d.f_range + cp.cell_off, cp.ncell))) return rc;
+    } else if (species == SPHNJ_SOLID) {
+      lo = s_off[c];
+      n = s_off[c + 1] - lo;
+      k.resize(n);
+      o.resize(n);
+      if (n && s_sorted_keys && ((rc = d2h(k.data(), s_sorted_keys + lo, n)) || (rc = d2h(o.data(), d.s_sorted + lo, n))))
+        return rc;
+      if ((rc = d2h(rg.data(), d.s_range + cp.cell_off, cp.ncell))) return rc;
+    } else {
+      lo = w_off[c];
+      n = w_off[c + 1] - lo;
+      std::vector<int> cxy(n);
+      o.resize(n);
+      k.resize(n);
+      if ((rc = d2h(cxy.data(), d.w_cxy + lo, n)) || (rc = d2h(o.data(), d.w_orig + lo, n)) ||
+          (rc = d2h(rg.data(), d.w_range + cp.cell_off, cp.ncell)))
+        return rc;
+      HIPC(hipStreamSynchronize(stream));
+      for (int t = 0; t < n; ++t) k[t] = cp.cell_off + (cxy[t] >> 16) * cp.ny + (cxy[t] & 0xffff);
+    }
+    HIPC(hipStreamSynchronize(stream));
+    for (int t = 0; t < n; ++t) {
+      if (keys) keys[t] = k[t];
+      if (orig) orig[t] = o[t] - lo;
+    }
+    if (start) {
+      int next = n;
+      start[cp.ncell] = n;
+      for (int q = cp.ncell - 1; q >= 0; --q) {
+        if (rg[q].y > rg[q].x) next = rg[q].x - lo;
+        start[q] = next;
+      }
+    }
+    return SPHNJ_OK;
+  }
+
+  int neighbors(int c, int *counts_out, int *list, int64_t cap, int64_t *total) override {
+    int rc;
+    const int lo = f_off[c], n = f_off[c + 1] - lo;
+    if (n == 0) {
+      if (total) *total = 0;
+      return SPHNJ_OK;
+    }
+    int *dcnt, *doff, *dlist;
+    std::vector<void *> tmp;
+    auto talloc = [&](int **p, size_t m) -> int {
+      void *q = nullptr;
+      HIPC(hipMalloc(&q, std::max<size_t>(m, 1) * sizeof(int)));
+      tmp.push_back(q);
+      *p = (int *)q;
+      return SPHNJ_OK;
+    };
+    auto cleanup = [&]() {
+      for (void *q : tmp) (void)hipFree(q);
+    };
+    if ((rc = talloc(&dcnt, n)) || (rc = talloc(&doff, n))) {
+      cleanup();
+      return rc;
+    }
+    k_neighbors<Pol><<<nblk(n, 256), 256, 0, stream>>>(d, lo, lo + n, nullptr, dcnt, nullptr);
+    std::vector<int> cnt(n), orig(n);
+    if ((rc = d2h(cnt.data(), dcnt, n)) || (rc = d2h(orig.data(), d.f_orig + lo, n))) {
+      cleanup();
+      return rc;
+    }
+    hipError_t e = hipStreamSynchronize(stream);
+    if (e != hipSuccess) {
+      cleanup();
+      err = hipGetErrorString(e);
+      return SPHNJ_E_HIP;
+    }
+    // CSR in original order
+    std::vector<int64_t> ooff(n + 1, 0);
+    std::vector<int> cnt_o(n);
+    for (int t = 0; t < n; ++t) cnt_o[orig[t] - lo] = cnt[t];
+    for (int i = 0; i < n; ++i) ooff[i + 1] = ooff[i] + cnt_o[i];
+    const int64_t tot = ooff[n];
+    if (total) *total = tot;
+    if (counts_out) std::copy(cnt_o.begin(), cnt_o.end(), counts_out);
+    if (list && tot > 0) {
+      if (tot > 0x7fffffff) {
+        cleanup();
+        err = "neighbour list too large";
+        return SPHNJ_E_ARG;
+      }
+      std::vector<int> soff(n);
+      for (int t = 0; t < n; ++t) soff[t] = (int)ooff[orig[t] - lo];
+      if ((rc = talloc(&dlist, (size_t)tot)) || (rc = h2d(doff, soff.data(), n))) {
+        cleanup();
+        return rc;
+      }
+      k_neighbors<Pol><<<nblk(n, 256), 256, 0, stream>>>(d, lo, lo + n, doff, nullptr, dlist);
+      std::vector<int> hl((size_t)tot);
+      if ((rc = d2h(hl.data(), dlist, (size_t)tot))) {
+        cleanup();
+        return rc;
+      }
+      e = hipStreamSynchronize(stream);
+      if (e != hipSuccess) {
+        cleanup();
+        err = hipGetErrorString(e);
+        return SPHNJ_E_HIP;
+      }
+      std::copy(hl.begin(), hl.begin() + std::min<int64_t>(cap, tot), list);
+    }
+    cleanup();
+    return SPHNJ_OK;
+  }
+
+  int reflist(int c, int *off, int *list, int64_t cap) override {
+    int rc;
+    const int lo = s_off[c], n = s_off[c + 1] - lo;
+    std::vector<int> o(n + 1);
+    if ((rc = d2h(o.data(), d.s_ref_off + lo, n + 1))) return rc;
+    HIPC(hipStreamSynchronize(stream));
+    const int base = o[0], tot = o[n] - base;
+    if (off)
+      for (int i = 0; i <= n; ++i) off[i] = o[i] - base;
+    if (list && tot > 0) {
+      std::vector<int> l(tot);
+      if ((rc = d2h(l.data(), d.s_ref_j + base, tot))) return rc;
+      HIPC(hipStreamSynchronize(stream));
+      for (int e = 0; e < tot && e < cap; ++e) list[e] = l[e] - lo;
+    }
+    return SPHNJ_OK;
+  }
+
+  int energy(int c, double out[6]) override {
+    int rc;
+    const Derived &D = Dh[c];
+    const double consts[7] = {D.d * D.d, D.rho0[0], D.rho0[1], D.rho_s, D.rho_aux, D.lambda, D.mu_s};
+    double *dc, *part;
+    const int G = 256;
+    void *q1 = nullptr, *q2 = nullptr;
+    HIPC(hipMalloc(&q1, sizeof(consts)));
+    dc = (double *)q1;
+    if (hipMalloc(&q2, (size_t)G * 5 * sizeof(double)) != hipSuccess) {
+      (void)hipFree(q1);
+      err = "hipMalloc";
+      return SPHNJ_E_HIP;
+    }
+    part = (double *)q2;
+    std::vector<double> hp((size_t)G * 5);
+    rc = h2d(dc, consts, 7);
+    if (!rc) {
+      k_energy_partial<Pol><<<G, 256, 0, stream>>>(d, f_off[c], f_off[c + 1], s_off[c], s_off[c + 1], dc, part);
+      rc = d2h(hp.data(), part, (size_t)G * 5);
+    }
+    const hipError_t e = hipStreamSynchronize(stream);
+    (void)hipFree(q1);
+    (void)hipFree(q2);
+    if (rc) return rc;
+    if (e != hipSuccess) {
+      err = hipGetErrorString(e);
+      return SPHNJ_E_HIP;
+    }
+    double ke = 0, mass = 0, m2 = 0, kes = 0, ep = 0;
+    for (int b = 0; b < G; ++b) {
+      ke += hp[5 * b];
+      mass += hp[5 * b + 1];
+      m2 = std::max(m2, hp[5 * b + 2]);
+      kes += hp[5 * b + 3];
+      ep += hp[5 * b + 4];
+    }
+    out[0] = ke;
+    out[1] = mass;
+    out[2] = std::sqrt(m2);
+    out[3] = kes;
+    out[4] = ep;
+    out[5] = (double)(f_off[c + 1] - f_off[c]);
+    return SPHNJ_OK;
+  }
+};
+
+}  // namespace
+
+struct sphnj_ctx {
+  int precision = 0;
+  int ncase = 0;
+  std::vector<sphnj_params> P;
+  std::vector<Derived> D;
+  std::vector<Staged> staged;
+  std::unique_ptr<EngineBase> eng;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  double last_ms = 0.0;
+  bool inited = false;
+  bool use_graph = true;
+  std::string err;
+  int fail(int code, const std::string &m) {
+    err = m;
+    return code;
+  }
+  int from_engine(int code) {
+    if (code && eng) err = eng->err;
+    return code;
+  }
+};
+
+extern "C" {
+
+int sphnj_create(const sphnj_params *params, int precision, sphnj_ctx **out) {
+  return sphnj_batch_create(params, 1, precision, out);
+}
+
+int sphnj_batch_create(const sphnj_params *pc, int32_t ncase, int precision, sphnj_ctx **out) {
+  if (!pc || ncase < 1 || !out) {
+    set_global_error("sphnj_batch_create: bad argument");
+    return SPHNJ_E_ARG;
+  }
+  *out = nullptr;
+  if (precision != SPHNJ_PREC_P64 && precision != SPHNJ_PREC_M32) {
+    set_global_error("unknown precision");
+    return SPHNJ_E_ARG;
+  }
+  std::vector<Derived> D(ncase);
+  for (int c = 0; c < ncase; ++c) {
+    const int rc = sphnj_params_validate(&pc[c]);
+    if (rc) return rc;
+    std::string e;
+    if (derive(pc[c], &D[c], &e)) {
+      set_global_error(e);
+      return SPHNJ_E_PARAM;
+    }
+    if (D[c].S != D[0].S || D[c].K != D[0].K) {
+      set_global_error("all cases of a batch must share solid_sub_step and shift_per_step");
+      return SPHNJ_E_UNSUPPORTED;
+    }
+  }
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+    set_global_error("no GPU visible");
+    return SPHNJ_E_NODEVICE;
+  }
+  auto *c = new sphnj_ctx();
+  c->precision = precision;
+  c->ncase = ncase;
+  c->P.assign(pc, pc + ncase);
+  c->D = D;
+  c->staged.resize(ncase);
+  if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+    set_global_error("cannot create HIP stream/events");
+    delete c;
+    return SPHNJ_E_HIP;
+  }
+  *out = c;
+  return SPHNJ_OK;
+}
+
+int sphnj_upload(sphnj_ctx *c, int32_t ci, const sphnj_fluid_in *f, const sphnj_solid_in *s, const sphnj_wall_in *w) {
+  if (!c) return SPHNJ_E_ARG;
+  if (ci < 0 || ci >= c->ncase) return c->fail(SPHNJ_E_ARG, "case index out of range");
+  if (c->inited) return c->fail(SPHNJ_E_STATE, "upload after init");
+  Staged &S = c->staged[ci];
+  S = Staged();
+  if (f && f->n > 0) {
+    if (!f->x || !f->y || !f->u || !f->v || !f->rho || !f->p || !f->phase)
+      return c->fail(SPHNJ_E_ARG, "fluid input has a null array");
+    S.fx.assign(f->x, f->x + f->n);
+    S.fy.assign(f->y, f->y + f->n);
+    S.fu.assign(f->u, f->u + f->n);
+    S.fv.assign(f->v, f->v + f->n);
+    S.frho.assign(f->rho, f->rho + f->n);
+    S.fp.assign(f->p, f->p + f->n);
+    for (int i = 0; i < f->n; ++i) {
+      if (f->phase[i] != 0 && f->phase[i] != 1) return c->fail(SPHNJ_E_ARG, "fluid phase must be 0 or 1");
+      S.fph.push_back((int8_t)f->phase[i]);
+    }
+  }
+  if (s && s->n > 0) {
+    if (!s->x || !s->y || !s->u || !s->v || !s->X || !s->Y || !s->material)
+      return c->fail(SPHNJ_E_ARG, "solid input has a null array");
+    S.sx.assign(s->x, s->x + s->n);
+    S.sy.assign(s->y, s->y + s->n);
+    S.su.assign(s->u, s->u + s->n);
+    S.sv.assign(s->v, s->v + s->n);
+    S.sX.assign(s->X, s->X + s->n);
+    S.sY.assign(s->Y, s->Y + s->n);
+    for (int i = 0; i < s->n; ++i) {
+      if (s->material[i] < 0 || s->material[i] > 2) return c->fail(SPHNJ_E_ARG, "unknown solid material");
+      S.smat.push_back((int8_t)s->material[i]);
+      const int b = s->bc ? s->bc[i] : 0;
+      S.sbc.push_back((int8_t)(b == 1 ? 1 : 0));
+    }
+  }
+  if (w && w->n > 0) {
+    if (!w->x || !w->y || !w->bc) return c->fail(SPHNJ_E_ARG, "wall input has a null array");
+    S.wx.assign(w->x, w->x + w->n);
+    S.wy.assign(w->y, w->y + w->n);
+    for (int i = 0; i < w->n; ++i) S.wbc.push_back((int8_t)w->bc[i]);
+  }
+  S.uploaded = true;
+  return SPHNJ_OK;
+}
+
+int sphnj_init(sphnj_ctx *c) {
+  if (!c) return SPHNJ_E_ARG;
+  if (c->inited) return c->fail(SPHNJ_E_STATE, "already initialised");
+  for (int i = 0; i < c->ncase; ++i)
+    if (!c->staged[i].uploaded) return c->fail(SPHNJ_E_STATE, "case " + std::to_string(i) + " not uploaded");
+  if (c->precision == SPHNJ_PREC_P64)
+    c->eng.reset(new Engine<P64>());
+  else
+    c->eng.reset(new Engine<M32>());
+  c->eng->stream = c->stream;
+  c->eng->use_graph = c->use_graph;
+  const int rc = c->eng->init(c->D, c->staged);
+  if (rc) {
+    c->err = c->eng->err;
+    c->eng.reset();
+    return rc;
+  }
+  c->staged.clear();
+  c->staged.shrink_to_fit();
+  c->inited = true;
+  return SPHNJ_OK;
+}
+
+int sphnj_step(sphnj_ctx *c, int32_t n) {
+  if (!c || n < 0) return SPHNJ_E_ARG;
+  if (!c->inited) return c->fail(SPHNJ_E_STATE, "step before init");
+  if (hipEventRecord(c->ev0, c->stream) != hipSuccess) return c->fail(SPHNJ_E_HIP, "hipEventRecord");
+  const int rc = c->eng->step(n);
+  if (rc) return c->from_engine(rc);
+  if (hipEventRecord(c->ev1, c->stream) != hipSuccess) return c->fail(SPHNJ_E_HIP, "hipEventRecord");
+  return SPHNJ_OK;
+}
+
+int sphnj_sync(sphnj_ctx *c) {
+  if (!c) return SPHNJ_E_ARG;
+  const hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return c->fail(SPHNJ_E_HIP, hipGetErrorString(e));
+  return SPHNJ_OK;
+}
+
+int sphnj_last_step_ms(sphnj_ctx *c, double *ms) {
+  if (!c || !ms) return SPHNJ_E_ARG;
+  if (hipEventSynchronize(c->ev1) != hipSuccess) return c->fail(SPHNJ_E_HIP, "hipEventSynchronize");
+  float f = 0.f;
+  if (hipEventElapsedTime(&f, c->ev0, c->ev1) != hipSuccess) return c->fail(SPHNJ_E_HIP, "hipEventElapsedTime");
+  *ms = (double)f;
+  return SPHNJ_OK;
+}
+
+int sphnj_stage(sphnj_ctx *c, int32_t st) {
+  if (!c) return SPHNJ_E_ARG;
+  if (!c->inited) return c->fail(SPHNJ_E_STATE, "stage before init");
+  if (st < SPHNJ_STAGE_CELLS || st > SPHNJ_STAGE_SHIFT) return c->fail(SPHNJ_E_ARG, "unknown stage");
+  return c->from_engine(c->eng->stage(st));
+}
+
+int sphnj_download(sphnj_ctx *c, int32_t ci, const sphnj_fluid_out *f, const sphnj_solid_out *s,
+                   const sphnj_wall_out *w) {
+  if (!c) return SPHNJ_E_ARG;
+  if (!c->inited) return c->fail(SPHNJ_E_STATE, "download before init");
+  if (ci < 0 || ci >= c->ncase) return c->fail(SPHNJ_E_ARG, "case index out of range");
+  return c->from_engine(c->eng->download(ci, f, s, w));
+}
+
+int sphnj_status(sphnj_ctx *c, uint32_t *flags) {
+  if (!c || !flags) return SPHNJ_E_ARG;
+  if (!c->inited) {
+    *flags = 0;
+    return SPHNJ_OK;
+  }
+  return c->from_engine(c->eng->status(flags));
+}
+
+int sphnj_counts(const sphnj_ctx *c, int32_t ci, int32_t *nf, int32_t *ns, int32_t *nw, int32_t *ncell) {
+  if (!c || ci < 0 || ci >= c->ncase) return SPHNJ_E_ARG;
+  if (c->inited) return c->eng->counts(ci, nf, ns, nw, ncell);
+  const Staged &S = c->staged[ci];
+  if (nf) *nf = (int32_t)S.fx.size();
+  if (ns) *ns = (int32_t)S.sx.size();
+  if (nw) *nw = (int32_t)S.wx.size();
+  if (ncell) *ncell = c->D[ci].nx * c->D[ci].ny;
+  return SPHNJ_OK;
+}
+
+const char *sphnj_last_error(const sphnj_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+void sphnj_destroy(sphnj_ctx *c) {
+  if (!c) return;
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  c->eng.reset();
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+int sphnj_set_graph(sphnj_ctx *c, int enable) {
+  if (!c) return SPHNJ_E_ARG;
+  c->use_graph = enable != 0;
+  if (c->eng) {
+    c->eng->use_graph = c->use_graph;
+    if (!c->use_graph) c->eng->drop_graphs();
+  }
+  return SPHNJ_OK;
+}
+
+int sphnj_get_block(const sphnj_ctx *c, int32_t fam, int32_t *block) {
+  if (!c || !block || !c->eng) return SPHNJ_E_ARG;
+  *block = fam == 0 ? c->eng->blk_pair : fam == 1 ? c->eng->blk_shift : c->eng->blk_stream;
+  return SPHNJ_OK;
+}
+
+int sphnj_set_block(sphnj_ctx *c, int32_t fam, int32_t block) {
+  if (!c || !c->eng) return SPHNJ_E_ARG;
+  if (block != 64 && block != 128 && block != 256) return c->fail(SPHNJ_E_ARG, "block must be 64, 128 or 256");
+  if (fam == 0)
+    c->eng->blk_pair = block;
+  else if (fam == 1)
+    c->eng->blk_shift = block;
+  else
+    c->eng->blk_stream = block;
+  c->eng->drop_graphs();
+  return SPHNJ_OK;
+}
+
+int sphnj_autotune(sphnj_ctx *c) {
+  if (!c) return SPHNJ_E_ARG;
+  if (!c->inited) return c->fail(SPHNJ_E_STATE, "autotune before init");
+  return SPHNJ_OK;
+}
+
+int sphnj_stage_sorted(sphnj_ctx *c, int32_t ci, int32_t species, int32_t *keys, int32_t *orig, int32_t *start) {
+  if (!c || ci < 0 || ci >= c->ncase || species < 0 || species > 2) return SPHNJ_E_ARG;
+  if (!c->inited) return c->fail(SPHNJ_E_STATE, "before init");
+  return c->from_engine(c->eng->sorted(ci, species, keys, orig, start));
+}
+
+int sphnj_stage_neighbors(sphnj_ctx *c, int32_t ci, int32_t *counts, int32_t *list, int64_t cap, int64_t *total) {
+  if (!c || ci < 0 || ci >= c->ncase) return SPHNJ_E_ARG;
+  if (!c->inited) return c->fail(SPHNJ_E_STATE, "before init");
+  return c->from_engine(c->eng->neighbors(ci, counts, list, cap, total));
+}
+
+int sphnj_solid_reflist(sphnj_ctx *c, int32_t ci, int32_t *off, int32_t *list, int64_t cap) {
+  if (!c || ci < 0 || ci >= c->ncase) return SPHNJ_E_ARG;
+  if (!c->inited) return c->fail(SPHNJ_E_STATE, "before init");
+  return c->from_engine(c->eng->reflist(ci, off, list, cap));
+}
+
+int sphnj_energy(sphnj_ctx *c, int32_t ci, double out[6]) {
+  if (!c || !out || ci < 0 || ci >= c->ncase) return SPHNJ_E_ARG;
+  if (!c->inited) return c->fail(SPHNJ_E_STATE, "before init");
+  return c->from_engine(c->eng->energy(ci, out));
+}
+
+}  // extern "C"

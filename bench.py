@@ -1,0 +1,216 @@
+"""Benchmark of the SPHNJ per-step particle pipeline on one MI355X.
+
+Workload (default): the two-phase dam break at dx = 0.002 m -- 500,000 water + 2,500,000 air +
+28,064 dummy wall particles (tank 4.0 x 3.0 m, water column 1.0 x 2.0 m, h = 1.3 dx, K = 10
+shift passes per step).  A step is one full fluid time step of the pipeline (cell hash + key
+sort + cell ranges, wall update, fluid pair + update, K shift passes) over all particles,
+inputs resident in HBM.  The FP32-mixed form (M32) is timed, then the plain FP64 form (P64) on
+the same input for the same number of steps; the relative L2 error of M32 against P64 is
+reported per field.  Per-stage kernel durations come from HIP events recorded on the
+library's stream.  The CPU baseline is the FP64 oracle on a bounded sample of the same workload.
+
+    python bench.py [--gpus 1] [--steps 100] [--warmup 10] [--config two_phase|dam5k|gate|wedge|ensemble]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "M particle-updates/s, FP32-mixed+batched vs FP64 plain on 1 GPU at ≤1e-4 rel L2"
+HBM_PEAK_GBS = 8000.0
+
+# Algorithmic (compulsory) bytes per particle per launch: each kernel reads its per-particle
+# inputs once and writes its outputs once, neighbour reads counted as ideal reuse.
+BYTES = {
+    # stage: {precision: bytes per fluid particle per launch}
+    "fluid": {"m32": 70 + 85, "p64": 130 + 113},   # pair + update (fused)
+    "shift": {"m32": 111, "p64": 155},             # gradient pass + shift update (fused), per pass
+    "cells": {"m32": 16 + 142, "p64": 20 + 150},   # hash + sort/permute/cell ranges
+}
+WALL_BYTES = {"m32": 38, "p64": 66}  # wall Shepard, per wall particle
+
+
+def configs():
+    from sphnj_amd import Case
+
+    return {
+        "two_phase": ("two-phase dam break dx=0.002 m: 500k water + 2.5M air + 28k wall, K=10",
+                      lambda: Case.generate("dam_break", dx=0.002, air=1)),
+        "dam5k": ("liquid dam break dx=0.02 m: 5k water + 2.9k wall, K=10",
+                  lambda: Case.generate("dam_break", dx=0.02)),
+        "gate": ("elastic-gate FSI dx=2.5e-4 m: 224k water + 6.3k solid + 58k wall, K=1, S=10",
+                 lambda: Case.generate("gate_fsi")),
+        "wedge": ("wedge entry dx=1e-3 m: 3.0M water + 1.5M air + 8.3k solid + 36k wall, K=1, S=10",
+                  lambda: Case.generate("wedge_entry")),
+    }
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64).ravel()
+    b = np.asarray(b, np.float64).ravel()
+    nb = np.linalg.norm(b)
+    return float(np.linalg.norm(a - b) / nb) if nb else float(np.linalg.norm(a - b))
+
+
+def timed_steps(sim, steps):
+    sim.sync()
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.sync()
+    return time.perf_counter() - t0
+
+
+def stage_profile(sim, steps):
+    sim.set_stage_timing(True)
+    sim.step(steps)
+    out = {}
+    for st in ("cells", "couple", "wall", "fluid", "solid", "shift"):
+        n, ms = sim.stage_time(st)
+        out[st] = {"launches": n, "ms_per_step": ms / steps, "ms_per_launch": (ms / n) if n else 0.0}
+    sim.set_stage_timing(False)
+    return out
+
+
+def run_ensemble(args):
+    """Batched ensemble: all members in one context (M32, graph-captured step) against the
+    members run one by one in plain FP64."""
+    from sphnj_amd import Case, Simulation
+
+    members = [Case.generate("ensemble_member", index=i) for i in range(args.members)]
+    n_total = sum(sum(m.counts) for m in members)
+    batch = Simulation(members, "m32")
+    batch.step(args.warmup)
+    t = timed_steps(batch, args.steps)
+    value = n_total * args.steps / t / 1e6
+    serial = 0.0
+    errs = []
+    for i, m in enumerate(members[: args.serial_members]):
+        one = Simulation(m, "p64")
+        one.step(args.warmup)
+        serial += timed_steps(one, args.steps) * len(members) / args.serial_members
+        a = batch.download(i)["fluid"]
+        b = one.download()["fluid"]
+        errs.append(max(rel_l2(a[k], b[k]) for k in ("x", "y", "u", "v", "rho", "p")))
+        one.close()
+    p64_value = n_total * args.steps / serial / 1e6
+    return value, p64_value, n_total, t, errs
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--config", default="two_phase", choices=["two_phase", "dam5k", "gate", "wedge", "ensemble"])
+    ap.add_argument("--profile-steps", type=int, default=5, help="steps timed per stage with HIP events")
+    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--members", type=int, default=256)
+    ap.add_argument("--serial-members", type=int, default=16)
+    ap.add_argument("--no-p64", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU: this build has no multi-GPU path (asked for --gpus {args.gpus}).",
+              file=sys.stderr)
+        return 2
+
+    from sphnj_amd import Simulation
+
+    if args.config == "ensemble":
+        value, p64_value, n_total, t, errs = run_ensemble(args)
+        line = {"metric": METRIC, "value": round(value, 3), "unit": "M particle-updates/s", "n_gpus": 1,
+                "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * t / args.steps, 4),
+                "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+                "data": "synthetic",
+                "config": {"workload": f"ensemble of {args.members} seeded liquid dam breaks, batched M32",
+                           "particles": n_total},
+                "p64_serial_value": round(p64_value, 3), "speedup_vs_p64": round(value / p64_value, 3),
+                "max_rel_l2_vs_p64": max(errs) if errs else None}
+        print(json.dumps(line))
+        return 0
+
+    desc, make = configs()[args.config]
+    case = make()
+    nf, ns, nw = case.counts
+    n_total = nf + ns + nw
+    print(f"# {desc}: {nf} fluid, {ns} solid, {nw} wall", file=sys.stderr, flush=True)
+
+    sim = Simulation(case, "m32")
+    sim.step(args.warmup)
+    t = timed_steps(sim, args.steps)
+    ev_ms = sim.last_step_ms()
+    value = n_total * args.steps / t / 1e6
+    prof = stage_profile(sim, args.profile_steps)
+    print(f"# m32: {1e3 * t / args.steps:.3f} ms/step (events {ev_ms / args.steps:.3f}), "
+          f"{value:.1f} M updates/s", file=sys.stderr, flush=True)
+
+    extra = {}
+    if not args.no_p64:
+        ref = Simulation(case, "p64")
+        ref.step(args.warmup)
+        tp = timed_steps(ref, args.steps)
+        p64_value = n_total * args.steps / tp / 1e6
+        prof64 = stage_profile(ref, args.profile_steps)
+        a, b = sim.download(), ref.download()
+        errs = {f: rel_l2(a["fluid"][f], b["fluid"][f]) for f in ("x", "y", "u", "v", "rho", "p")}
+        if ns:
+            errs.update({"solid_" + f: rel_l2(a["solid"][f], b["solid"][f]) for f in ("x", "y", "stress")})
+        extra = {"p64_value": round(p64_value, 3), "p64_ms_per_step": round(1e3 * tp / args.steps, 4),
+                 "speedup_vs_p64": round(value / p64_value, 3),
+                 "rel_l2_vs_p64": {k: float(f"{v:.3e}") for k, v in errs.items()},
+                 "steps_compared": args.warmup + args.steps + args.profile_steps,
+                 "p64_stage_ms": {k: round(v["ms_per_step"], 4) for k, v in prof64.items()}}
+        print(f"# p64: {1e3 * tp / args.steps:.3f} ms/step, {p64_value:.1f} M updates/s; "
+              f"max rel L2 {max(errs.values()):.2e}", file=sys.stderr, flush=True)
+        ref.close()
+
+    # roofline of the dominant kernel (largest share of the step)
+    dom = max(("fluid", "shift", "cells"), key=lambda s: prof[s]["ms_per_step"])
+    per_launch_ms = prof[dom]["ms_per_launch"]
+    bytes_launch = BYTES[dom]["m32"] * nf
+    achieved = bytes_launch / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms else 0.0
+    kernel_name = {"fluid": "k_fluid_pair_update<M32>", "shift": "k_fluid_shift<M32>",
+                   "cells": "cells stage (hash + radix sort + permute + ranges)"}[dom]
+    roofline = {"bound": "hbm", "kernel": kernel_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "bytes_per_particle": BYTES[dom]["m32"], "ms_per_launch": round(per_launch_ms, 5)}
+
+    cpu = None
+    if args.cpu_steps > 0:
+        from oracle.oracle_py import Oracle, set_threads
+
+        cores = set_threads(0)
+        ora = Oracle(case)
+        t0 = time.perf_counter()
+        ora.step(args.cpu_steps)
+        tc = time.perf_counter() - t0
+        cpu = {"value": round(n_total * args.cpu_steps / tc / 1e6, 4), "unit": "M particle-updates/s",
+               "cores": cores, "kind": "port",
+               "sample": f"FP64 oracle (OpenMP), {args.cpu_steps} steps of the same workload, {tc:.1f} s"}
+        ora.close()
+
+    line = {"metric": METRIC, "value": round(value, 3), "unit": "M particle-updates/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * t / args.steps, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+            "data": "synthetic",
+            "config": {"workload": desc, "fluid": nf, "solid": ns, "wall": nw, "form": "M32 (FP32-mixed)",
+                       "dt": case.params.dt, "shift_per_step": case.params.shift_per_step,
+                       "solid_sub_step": case.params.solid_sub_step},
+            "roofline": roofline, "cpu_baseline": cpu,
+            "stage_ms": {k: round(v["ms_per_step"], 4) for k, v in prof.items()}}
+    line.update(extra)
+    print(json.dumps(line))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -199,8 +199,9 @@ int sphnj_case_write(const sphnj_params *params, const sphnj_particles *ps, cons
 
 /* ---- context life cycle -----------------------------------------------------------------
  * sphnj_create: one case (the drop-in).  sphnj_batch_create: ncase independent cases that
- * share one set of launches per phase (all cases must agree on solid_sub_step and
- * shift_per_step).  Device memory is allocated by sphnj_init, after every case is uploaded. */
+ * share one set of launches per phase; each case keeps its own parameters, dt, cell grid,
+ * shift_per_step and solid_sub_step.  Device memory is allocated by sphnj_init, after every
+ * case is uploaded. */
 int sphnj_create(const sphnj_params *params, int precision, sphnj_ctx **out);
 int sphnj_batch_create(const sphnj_params *per_case, int32_t ncase, int precision, sphnj_ctx **out);
 int sphnj_upload(sphnj_ctx *ctx, int32_t case_index, const sphnj_fluid_in *f,
@@ -267,8 +268,15 @@ int sphnj_solid_reflist(sphnj_ctx *ctx, int32_t case_index, int32_t *offsets, in
  * out[4] solid strain energy (src/output.cu:426-455 form), out[5] number of fluid particles. */
 int sphnj_energy(sphnj_ctx *ctx, int32_t case_index, double out[6]);
 
-/* ---- timing of the last sphnj_step call (HIP events on the context stream) ------------- */
+/* ---- timing (HIP events on the context stream) ------------------------------------------ */
+/* Elapsed time of the last sphnj_step call. */
 int sphnj_last_step_ms(sphnj_ctx *ctx, double *ms);
+/* While enabled, steps run without graphs and every stage is bracketed by events; resets
+ * the accumulated records. */
+int sphnj_set_stage_timing(sphnj_ctx *ctx, int enable);
+/* Accumulated time of one stage since timing was enabled, and the number of kernel launches
+ * of that stage's main kernel (K shift passes, 2 per solid sub-step, else 1 per step). */
+int sphnj_stage_time(sphnj_ctx *ctx, int32_t stage, int32_t *launches, double *total_ms);
 
 #ifdef __cplusplus
 }

@@ -147,6 +147,8 @@ LIB_PROTOS = {
     "sphnj_get_block": (C.c_int, [VP, C.c_int32, c_int32_p]),
     "sphnj_set_block": (C.c_int, [VP, C.c_int32, C.c_int32]),
     "sphnj_last_step_ms": (C.c_int, [VP, c_double_p]),
+    "sphnj_set_stage_timing": (C.c_int, [VP, C.c_int]),
+    "sphnj_stage_time": (C.c_int, [VP, C.c_int32, c_int32_p, c_double_p]),
 }
 for _k, _v in CTX_PROTOS.items():
     LIB_PROTOS["sphnj_" + _k] = _v

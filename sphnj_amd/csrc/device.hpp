@@ -41,6 +41,7 @@ struct CaseDev {
   double dxmin, dymin, dxmax, dymax;
   int nx, ny, cell_off, ncell;
   int steady, viscid;
+  int K, S;  // shift passes and solid sub-steps of this case
   // solid (SolidProperties, src/properties.cpp:69-89,118-161)
   R lambda, mu_s, c_s, alpha, beta;
   R rho_s_inv, rho_aux_inv;

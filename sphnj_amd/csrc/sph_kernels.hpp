@@ -485,13 +485,22 @@ __global__ void __launch_bounds__(256) k_fluid_pair_update(Dev<Pol> d) {
 // correctionCalculate (src/fluid.cu:316-367) + correctionUpdate / shiftDisplacementCalculate
 // (src/fluid.cu:369-427).  The gradient sums over all species with no phase filter.
 template <class Pol>
-__global__ void __launch_bounds__(256) k_fluid_shift(Dev<Pol> d) {
+__global__ void __launch_bounds__(256) k_fluid_shift(Dev<Pol> d, int pass) {
   using R = typename Pol::R;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.nf) return;
   const int c = case_of(i, d.f_case_off, d.ncase);
   const auto &cp = d.cases[c];
   const auto &A = d.fa;
+  if (pass >= cp.K) {  // this case has fewer passes than the batch: carry positions over
+    d.fb.x[i] = A.x[i];
+    d.fb.y[i] = A.y[i];
+    if constexpr (Pol::MIXED) {
+      d.fb.vx[i] = A.vx[i];
+      d.fb.vy[i] = A.vy[i];
+    }
+    return;
+  }
   const int pk = d.f_cxy[i], cx = cxy_x(pk), cy = cxy_y(pk);
   const R xi = A.vx[i], yi = A.vy[i];
   R gx = 0, gy = 0;
@@ -679,12 +688,13 @@ __device__ __forceinline__ R plastic_elastic(const CP &cp, R strain, R &max_stra
 // B^-1 depends only on reference positions and is precomputed at init.  Stress as
 // solidStressCalculation (src/solid.cu:131-167) plus the neo-Hookean extension.
 template <class Pol>
-__global__ void __launch_bounds__(256) k_solid_deform(Dev<Pol> d) {
+__global__ void __launch_bounds__(256) k_solid_deform(Dev<Pol> d, int sub) {
   using R = typename Pol::R;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.ns) return;
   const int c = case_of(i, d.s_case_off, d.ncase);
   const auto &cp = d.cases[c];
+  if (sub >= cp.S) return;
   const double dix = d.sx[i] - d.sX[i], diy = d.sy[i] - d.sY[i];
   R h00 = 0, h01 = 0, h10 = 0, h11 = 0;
   for (int e = d.s_ref_off[i]; e < d.s_ref_off[i + 1]; ++e) {
@@ -774,12 +784,23 @@ __global__ void __launch_bounds__(256) k_solid_deform(Dev<Pol> d) {
 // solidForceCalculate (src/solid.cu:307-353: solidFluidCouple :211-255, solidSolidForce
 // :257-305) fused with solidUpdate (:446-471); positions and velocities ping-pong.
 template <class Pol>
-__global__ void __launch_bounds__(256) k_solid_force_update(Dev<Pol> d) {
+__global__ void __launch_bounds__(256) k_solid_force_update(Dev<Pol> d, int sub) {
   using R = typename Pol::R;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.ns) return;
   const int c = case_of(i, d.s_case_off, d.ncase);
   const auto &cp = d.cases[c];
+  if (sub >= cp.S) {  // this case has fewer sub-steps than the batch: carry the state over
+    d.sx2[i] = d.sx[i];
+    d.sy2[i] = d.sy[i];
+    d.su2[i] = d.su[i];
+    d.sv2[i] = d.sv[i];
+    if constexpr (Pol::MIXED) {
+      d.s_vx2[i] = d.s_vx[i];
+      d.s_vy2[i] = d.s_vy[i];
+    }
+    return;
+  }
   const int pk = d.s_cxy[i], cx = cxy_x(pk), cy = cxy_y(pk);
   const double xi = d.sx[i], yi = d.sy[i], ui = d.su[i], vi = d.sv[i];
   const int mat = d.s_mat[i];
@@ -909,6 +930,7 @@ __global__ void __launch_bounds__(256) k_ref_fill(Dev<Pol> d, const int *cxy, co
   const int cx = cxy_x(cxy[i]), cy = cxy_y(cxy[i]);
   const double Xi = d.sX[i], Yi = d.sY[i];
   int e = d.s_ref_off[i];
+  const int e_end = d.s_ref_off[i + 1];  // from k_ref_count; never write past it
   double b00 = 0, b01 = 0, b10 = 0, b11 = 0;
   for (int a = -1; a <= 1; ++a) {
     const int jx = cx + a;
@@ -920,7 +942,7 @@ __global__ void __launch_bounds__(256) k_ref_fill(Dev<Pol> d, const int *cxy, co
       for (int k = r.x; k < r.y; ++k) {
         const int j = sorted[k];
         const double dX = d.sX[j] - Xi, dY = d.sY[j] - Yi, r2 = dX * dX + dY * dY;
-        if (r2 < rc2) {
+        if (r2 < rc2 && e < e_end) {
           const double w = exp(-(r2 * hinv2)) * kdc;
           d.s_ref_j[e] = j;
           d.s_ref_g[2 * e] = (R)(dX * w);
@@ -959,7 +981,7 @@ __global__ void __launch_bounds__(256) k_ref_fill(Dev<Pol> d, const int *cxy, co
 // ---- inspection: neighbour lists of the fluid particles ---------------------------------
 template <class Pol>
 __global__ void __launch_bounds__(256) k_neighbors(Dev<Pol> d, int lo, int hi, const int *offset, int *count,
-                                                   int *list) {
+                                                   int *list, const int *limit) {
   using R = typename Pol::R;
   const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= hi) return;
@@ -970,6 +992,7 @@ __global__ void __launch_bounds__(256) k_neighbors(Dev<Pol> d, int lo, int hi, c
   const R xi = A.vx[i], yi = A.vy[i];
   int n = 0;
   int *out = list ? list + offset[i - lo] : nullptr;
+  const int lim = limit ? limit[i - lo] : 0x7fffffff;  // count of the first pass
   for (int a = -1; a <= 1; ++a) {
     const int jx = cx + a;
     if (jx < 0 || jx >= cp.nx) continue;
@@ -983,7 +1006,7 @@ __global__ void __launch_bounds__(256) k_neighbors(Dev<Pol> d, int lo, int hi, c
       for (int s = rf.x; s < rf.y; ++s) {
         const R dx = A.vx[s] - bx, dy = A.vy[s] - by;
         if (dx * dx + dy * dy < cp.rc2) {
-          if (out) out[n] = (0 << 28) | (d.f_orig[s] - cp.f_off);
+          if (out && n < lim) out[n] = (0 << 28) | (d.f_orig[s] - cp.f_off);
           ++n;
         }
       }
@@ -992,7 +1015,7 @@ __global__ void __launch_bounds__(256) k_neighbors(Dev<Pol> d, int lo, int hi, c
         const int j = d.s_sorted[k];
         const R dx = d.s_vx[j] - bx, dy = d.s_vy[j] - by;
         if (dx * dx + dy * dy < cp.rc2) {
-          if (out) out[n] = (1 << 28) | (j - cp.s_off);
+          if (out && n < lim) out[n] = (1 << 28) | (j - cp.s_off);
           ++n;
         }
       }
@@ -1000,7 +1023,7 @@ __global__ void __launch_bounds__(256) k_neighbors(Dev<Pol> d, int lo, int hi, c
       for (int s = rw.x; s < rw.y; ++s) {
         const R dx = d.w_vx[s] - bx, dy = d.w_vy[s] - by;
         if (dx * dx + dy * dy < cp.rc2) {
-          if (out) out[n] = (2 << 28) | (d.w_orig[s] - cp.w_off);
+          if (out && n < lim) out[n] = (2 << 28) | (d.w_orig[s] - cp.w_off);
           ++n;
         }
       }

@@ -58,7 +58,35 @@ struct EngineBase {
   hipStream_t stream = nullptr;
   bool use_graph = true;
   int blk_pair = 256, blk_shift = 256, blk_stream = 256;
-  virtual ~EngineBase() {}
+  // per-stage timing: HIP events recorded on the context stream around each stage while on
+  bool timing = false;
+  struct StageRec {
+    int stage, launches;
+    hipEvent_t a, b;
+  };
+  std::vector<StageRec> recs;
+  std::vector<hipEvent_t> ev_pool;
+  hipEvent_t take_event() {
+    if (!ev_pool.empty()) {
+      hipEvent_t e = ev_pool.back();
+      ev_pool.pop_back();
+      return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+  }
+  void reset_times() {
+    for (auto &r : recs) {
+      ev_pool.push_back(r.a);
+      ev_pool.push_back(r.b);
+    }
+    recs.clear();
+  }
+  virtual ~EngineBase() {
+    reset_times();
+    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
+  }
   virtual int init(const std::vector<Derived> &D, const std::vector<Staged> &S) = 0;
   virtual int stage(int st) = 0;
   virtual int step(int n) = 0;
@@ -69,6 +97,7 @@ struct EngineBase {
   virtual int neighbors(int c, int *counts, int *list, int64_t cap, int64_t *total) = 0;
   virtual int reflist(int c, int *off, int *list, int64_t cap) = 0;
   virtual int energy(int c, double out[6]) = 0;
+  virtual int stage_time(int st, int *launches, double *ms) = 0;
   virtual void drop_graphs() = 0;
 };
 
@@ -181,6 +210,8 @@ struct Engine : EngineBase {
     c.ncell = D.nx * D.ny;
     c.steady = D.steady;
     c.viscid = D.viscid;
+    c.K = D.K;
+    c.S = D.S;
     c.lambda = (R)D.lambda;
     c.mu_s = (R)D.mu_s;
     c.c_s = (R)D.c_s;
@@ -202,8 +233,12 @@ struct Engine : EngineBase {
     int rc;
     Dh = D;
     ncase = (int)D.size();
-    S = D[0].S;
-    K = D[0].K;
+    S = 1;
+    K = 0;
+    for (int c = 0; c < ncase; ++c) {  // launches cover the largest count; kernels skip per case
+      K = std::max(K, D[c].K);
+      if (!St[c].sx.empty()) S = std::max(S, D[c].S);
+    }
     f_off.assign(ncase + 1, 0);
     s_off.assign(ncase + 1, 0);
     w_off.assign(ncase + 1, 0);
@@ -523,8 +558,8 @@ struct Engine : EngineBase {
       case SPHNJ_STAGE_SOLID:
         if (ns > 0) {
           for (int k = 0; k < S; ++k) {
-            k_solid_deform<Pol><<<nblk(ns, blk_stream), blk_stream, 0, stream>>>(d);
-            k_solid_force_update<Pol><<<nblk(ns, blk_stream), blk_stream, 0, stream>>>(d);
+            k_solid_deform<Pol><<<nblk(ns, blk_stream), blk_stream, 0, stream>>>(d, k);
+            k_solid_force_update<Pol><<<nblk(ns, blk_stream), blk_stream, 0, stream>>>(d, k);
             LAUNCH_CHECK();
             std::swap(d.sx, d.sx2);
             std::swap(d.sy, d.sy2);
@@ -538,7 +573,7 @@ struct Engine : EngineBase {
       case SPHNJ_STAGE_SHIFT:
         if (nf > 0) {
           for (int k = 0; k < K; ++k) {
-            k_fluid_shift<Pol><<<nblk(nf, blk_shift), blk_shift, 0, stream>>>(d);
+            k_fluid_shift<Pol><<<nblk(nf, blk_shift), blk_shift, 0, stream>>>(d, k);
             LAUNCH_CHECK();
             std::swap(d.fa.x, d.fb.x);
             std::swap(d.fa.y, d.fb.y);
@@ -552,11 +587,47 @@ struct Engine : EngineBase {
     return SPHNJ_E_ARG;
   }
 
+  int launches_of(int st) const {
+    if (st == SPHNJ_STAGE_SHIFT) return d.nf > 0 ? K : 0;
+    if (st == SPHNJ_STAGE_SOLID) return d.ns > 0 ? 2 * S : 0;
+    return 1;
+  }
+
   int enqueue_step() {
     for (int st = SPHNJ_STAGE_CELLS; st <= SPHNJ_STAGE_SHIFT; ++st) {
+      hipEvent_t a = nullptr, b = nullptr;
+      if (timing) {
+        a = take_event();
+        b = take_event();
+        if (!a || !b) {
+          err = "hipEventCreate";
+          return SPHNJ_E_HIP;
+        }
+        HIPC(hipEventRecord(a, stream));
+      }
       const int rc = stage(st);
       if (rc) return rc;
+      if (timing) {
+        HIPC(hipEventRecord(b, stream));
+        recs.push_back({st, launches_of(st), a, b});
+      }
     }
+    return SPHNJ_OK;
+  }
+
+  int stage_time(int st, int *launches, double *ms) override {
+    HIPC(hipStreamSynchronize(stream));
+    int n = 0;
+    double t = 0.0;
+    for (auto &r : recs) {
+      if (r.stage != st) continue;
+      float f = 0.f;
+      HIPC(hipEventElapsedTime(&f, r.a, r.b));
+      t += f;
+      n += r.launches;
+    }
+    *launches = n;
+    *ms = t;
     return SPHNJ_OK;
   }
 
@@ -566,7 +637,7 @@ struct Engine : EngineBase {
 
   int step(int n) override {
     for (int s = 0; s < n; ++s) {
-      if (!use_graph) {
+      if (!use_graph || timing) {
         const int rc = enqueue_step();
         if (rc) return rc;
         continue;
@@ -802,7 +873,7 @@ struct Engine : EngineBase {
       cleanup();
       return rc;
     }
-    k_neighbors<Pol><<<nblk(n, 256), 256, 0, stream>>>(d, lo, lo + n, nullptr, dcnt, nullptr);
+    k_neighbors<Pol><<<nblk(n, 256), 256, 0, stream>>>(d, lo, lo + n, nullptr, dcnt, nullptr, nullptr);
     std::vector<int> cnt(n), orig(n);
     if ((rc = d2h(cnt.data(), dcnt, n)) || (rc = d2h(orig.data(), d.f_orig + lo, n))) {
       cleanup();
@@ -834,7 +905,7 @@ struct Engine : EngineBase {
         cleanup();
         return rc;
       }
-      k_neighbors<Pol><<<nblk(n, 256), 256, 0, stream>>>(d, lo, lo + n, doff, nullptr, dlist);
+      k_neighbors<Pol><<<nblk(n, 256), 256, 0, stream>>>(d, lo, lo + n, doff, nullptr, dlist, dcnt);
       std::vector<int> hl((size_t)tot);
       if ((rc = d2h(hl.data(), dlist, (size_t)tot))) {
         cleanup();
@@ -966,10 +1037,6 @@ int sphnj_batch_create(const sphnj_params *pc, int32_t ncase, int precision, sph
     if (derive(pc[c], &D[c], &e)) {
       set_global_error(e);
       return SPHNJ_E_PARAM;
-    }
-    if (D[c].S != D[0].S || D[c].K != D[0].K) {
-      set_global_error("all cases of a batch must share solid_sub_step and shift_per_step");
-      return SPHNJ_E_UNSUPPORTED;
     }
   }
   int ndev = 0;
@@ -1161,6 +1228,27 @@ int sphnj_set_block(sphnj_ctx *c, int32_t fam, int32_t block) {
   else
     c->eng->blk_stream = block;
   c->eng->drop_graphs();
+  return SPHNJ_OK;
+}
+
+int sphnj_set_stage_timing(sphnj_ctx *c, int enable) {
+  if (!c || !c->eng) return SPHNJ_E_ARG;
+  const hipError_t e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) return c->fail(SPHNJ_E_HIP, hipGetErrorString(e));
+  c->eng->timing = enable != 0;
+  c->eng->reset_times();
+  return SPHNJ_OK;
+}
+
+int sphnj_stage_time(sphnj_ctx *c, int32_t stage, int32_t *launches, double *total_ms) {
+  if (!c || !c->eng || !launches || !total_ms) return SPHNJ_E_ARG;
+  if (stage < SPHNJ_STAGE_CELLS || stage > SPHNJ_STAGE_SHIFT) return c->fail(SPHNJ_E_ARG, "unknown stage");
+  int n = 0;
+  double t = 0;
+  const int rc = c->eng->stage_time(stage, &n, &t);
+  if (rc) return c->from_engine(rc);
+  *launches = n;
+  *total_ms = t;
   return SPHNJ_OK;
 }
 

@@ -285,6 +285,16 @@ class Simulation(ContextBase):
     def set_block(self, family: int, block: int) -> None:
         self._call("set_block", family, block)
 
+    def set_stage_timing(self, on: bool) -> None:
+        self._call("set_stage_timing", int(on))
+
+    def stage_time(self, name: str) -> tuple[int, float]:
+        """(kernel launches, total ms) of a stage since timing was enabled."""
+        n = C.c_int32()
+        ms = C.c_double()
+        self._call("stage_time", A.STAGES[name], C.byref(n), C.byref(ms))
+        return n.value, ms.value
+
     # ---- the reference's phase names (DeviceCalculation, include/calculation.cuh:9-23) ----
     def particle_cell_update(self) -> None:
         """particleCellUpdate + sortParticle + fillCell: hash, sort, cell ranges."""

@@ -24,6 +24,7 @@ template <class R>
 struct CaseDev {
   // kernel (Kernel::initiate, src/kernel.cpp:25-41)
   R rc2, hinv2, kdc, k2c, eR, h, eps_h2;  // eps_h2 = 0.01 h^2
+  R nhl2;  // -hinv2 * log2(e): exp(-r^2/h^2) = 2^(r^2 nhl2)
   R quarter_rc2;
   // fluid per phase: pair-side constants in R
   R c[2], cinv[2], mu8[2], rho0[2];

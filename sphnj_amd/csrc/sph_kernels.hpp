@@ -318,6 +318,133 @@ __global__ void __launch_bounds__(256) k_wall_update(Dev<Pol> d) {
 }
 
 // ======================================================================================
+// LDS neighbour tiles
+// ======================================================================================
+// A block owns kTileSlots consecutive sorted fluid slots, whose global keys lie in [k0, k1].
+// The 3x3 neighbour cells of any of them that sit in neighbour column a (-1, 0, +1) have keys
+// in [k0 + a ny - 1, k1 + a ny + 1]: a contiguous key range, hence a contiguous slot segment.
+// The block stages the three segments in LDS once (coalesced) and every lane reads its
+// candidates from there.  A tile that straddles two cases, spans more than kTileMaxCells
+// cells per segment or holds more than the kernel's capacity falls back to global loads.
+constexpr int kTileSlots = 256;
+constexpr int kTileMaxCells = 96;
+
+struct TileHdr {
+  int ok;
+  int k0[3], nc[3], lo[3], hi[3], base[3];
+};
+
+template <class R>
+struct V2;
+template <>
+struct V2<float> {
+  using T = float2;
+};
+template <>
+struct V2<double> {
+  using T = double2;
+};
+
+// Every thread of the block must call this (it synchronises).  On return T.ok says whether
+// the tile is staged; cr[a][k] holds the slot range of cell T.k0[a] + k.
+template <class Pol>
+__device__ __forceinline__ void tile_setup(const Dev<Pol> &d, int i0, int cap, TileHdr &T,
+                                           int2 (*cr)[kTileMaxCells]) {
+  const int t = threadIdx.x;
+  if (t == 0) {
+    const int il = min(i0 + (int)blockDim.x, d.nf) - 1;
+    const int c0 = case_of(i0, d.f_case_off, d.ncase), c1 = case_of(il, d.f_case_off, d.ncase);
+    const auto &cp = d.cases[c0];
+    const int k0 = d.f_key[i0], k1 = d.f_key[il];
+    const int cmin = cp.cell_off, cmax = cp.cell_off + cp.ncell - 1;
+    for (int a = 0; a < 3; ++a) {
+      const int ka = max(k0 + (a - 1) * cp.ny - 1, cmin), kb = min(k1 + (a - 1) * cp.ny + 1, cmax);
+      T.k0[a] = ka;
+      T.nc[a] = kb >= ka ? kb - ka + 1 : 0;
+      T.lo[a] = 0x7fffffff;
+      T.hi[a] = -1;
+    }
+    T.ok = (c0 == c1) && (k1 - k0 + 3 <= kTileMaxCells);
+  }
+  __syncthreads();
+  if (T.ok) {
+    for (int e = t; e < 3 * kTileMaxCells; e += blockDim.x) {
+      const int a = e / kTileMaxCells, k = e - a * kTileMaxCells;
+      if (k < T.nc[a]) {
+        const int2 r = d.f_range[T.k0[a] + k];
+        cr[a][k] = r;
+        if (r.y > r.x) {
+          atomicMin(&T.lo[a], r.x);
+          atomicMax(&T.hi[a], r.y);
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (t == 0 && T.ok) {
+    int base = 0;
+    for (int a = 0; a < 3; ++a) {
+      if (T.hi[a] < 0) {
+        T.lo[a] = 0;
+        T.hi[a] = 0;
+      }
+      T.base[a] = base;
+      base += T.hi[a] - T.lo[a];
+    }
+    T.ok = base <= cap;
+  }
+  __syncthreads();
+}
+
+// Copy field f of the three segments into LDS (dst[base[a] + k] = f(lo[a] + k)).
+template <class F>
+__device__ __forceinline__ void tile_stage(const TileHdr &T, F &&f) {
+  for (int a = 0; a < 3; ++a) {
+    const int n = T.hi[a] - T.lo[a];
+    for (int k = threadIdx.x; k < n; k += blockDim.x) f(T.base[a] + k, T.lo[a] + k);
+  }
+}
+
+// Gradient weight w'(r^2) = exp(-r^2 / h^2) kd c32 (Kernel::initiate, src/kernel.cpp:37-40).
+template <class R, class CP>
+__device__ __forceinline__ R grad_w(R r2, const CP &cp) {
+  if constexpr (sizeof(R) == 4)
+    return __builtin_amdgcn_exp2f(r2 * cp.nhl2) * cp.kdc;
+  else
+    return exp(-(r2 * cp.hinv2)) * cp.kdc;
+}
+
+// Calls f(s) for s in [lo, hi), U at a time so the loads of U candidates are in flight together.
+template <int U, class F>
+__device__ __forceinline__ void for_range(int lo, int hi, F &&f) {
+  int s = lo;
+  for (; s + U <= hi; s += U) {
+#pragma unroll
+    for (int k = 0; k < U; ++k) f(s + k);
+  }
+  for (; s < hi; ++s) f(s);
+}
+
+// Per-case constants a neighbour loop needs, held in registers.
+template <class R>
+struct LoopConsts {
+  R rc2, nhl2, hinv2, kdc;
+  template <class CP>
+  __device__ __forceinline__ explicit LoopConsts(const CP &cp)
+      : rc2(cp.rc2), nhl2(cp.nhl2), hinv2(cp.hinv2), kdc(cp.kdc) {}
+  // Gradient weight w'(r^2) = exp(-r^2/h^2) kd c32 inside the cut-off, exactly 0 outside, so a
+  // candidate outside the cut-off adds exact zeros and the loop needs no branch.
+  __device__ __forceinline__ R w(R r2) const {
+    R e;
+    if constexpr (sizeof(R) == 4)
+      e = __builtin_amdgcn_exp2f(r2 * nhl2);
+    else
+      e = exp(-(r2 * hinv2));
+    return r2 < rc2 ? e * kdc : R(0);
+  }
+};
+
+// ======================================================================================
 // FLUID: Riemann pair loop fused with the explicit update
 // ======================================================================================
 
@@ -359,27 +486,62 @@ __device__ __forceinline__ void riemann_pair(R dx, R dy, R r2, R w, R ui, R vi, 
   }
 }
 
+template <class Pol>
+struct PairCap {
+  static constexpr int value = Pol::MIXED ? 1536 : 1024;
+};
+
 // findFluidPair (src/fluid.cu:193-254) + fluidUpdate (src/fluid.cu:256-287).  Reads the
 // current buffers (fa) of every neighbour and writes particle i's updated state into the
 // alternate buffers (fb), so neighbours never see a half-updated state.  Also reduces the
 // per-case max |u|^2 of the updated velocities (getMaxSpeed, src/fluid.cu:429-440).
 template <class Pol, bool VISCID>
-__global__ void __launch_bounds__(256) k_fluid_pair_update(Dev<Pol> d) {
+__global__ void __launch_bounds__(kTileSlots) k_fluid_pair_update(Dev<Pol> d) {
   using R = typename Pol::R;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  using R2 = typename V2<R>::T;
+  constexpr int CAP = PairCap<Pol>::value;
+  __shared__ R2 s_pos[CAP], s_vel[CAP], s_rp[CAP];
+  __shared__ int s_fl[CAP];
+  __shared__ int2 s_cr[3][kTileMaxCells];
+  __shared__ TileHdr T;
+  const int i0 = blockIdx.x * blockDim.x;
+  const int i = i0 + threadIdx.x;
+  const auto &A = d.fa;
+  tile_setup(d, i0, CAP, T, s_cr);
+  const bool staged = T.ok;
+  if (staged) {
+    tile_stage(T, [&](int l, int g) {
+      s_pos[l] = R2{A.vx[g], A.vy[g]};
+      s_vel[l] = R2{A.vu[g], A.vv[g]};
+      s_rp[l] = R2{A.vrho[g], A.p[g]};
+      s_fl[l] = (int)d.f_phase[g] | ((int)d.f_cls[g] << 8);
+    });
+  }
+  __syncthreads();
   unsigned long long my_u2 = 0ull;
   int c = 0;
   if (i < d.nf) {
     c = case_of(i, d.f_case_off, d.ncase);
     const auto &cp = d.cases[c];
-    const auto &A = d.fa;
     const int pk = d.f_cxy[i], cx = cxy_x(pk), cy = cxy_y(pk);
     const R xi = A.vx[i], yi = A.vy[i], ui = A.vu[i], vi = A.vv[i], rhoi = A.vrho[i], pi = A.p[i];
     const int phi = d.f_phase[i];
     const bool liq = phi == 0;
     const R cs = cp.c[phi], csinv = cp.cinv[phi], mu8 = cp.mu8[phi];
     const R rhoi_inv = krcp<R>(rhoi);
+    const LoopConsts<R> K(cp);
+    const R eps_h2 = cp.eps_h2, qrc2 = cp.quarter_rc2;
     PairSums<R> S;
+    // fluidFluidCalculate on one candidate (w = 0 outside the cut-off: exact zero terms)
+    auto fluid_j = [&](int s, R dx, R dy, R r2, R uj, R vj, R rhoj, R pj, int fl) {
+      const R w = K.w(r2);
+      riemann_pair<R, VISCID>(dx, dy, r2, w, ui, vi, uj, vj, rhoi, rhoi_inv, rhoj, pi, pj, cs, csinv, mu8, eps_h2, S);
+      if (!(liq && (fl & 0xff) == 1)) S.col += r2 * w;
+      // FREE_SURFACE neighbour: HAS_NEAR (2) within half the cut-off, else HAS (1); the
+      // reference's sticky update (src/fluid.cu:67-73) ends at the maximum over neighbours.
+      const bool fs = r2 < K.rc2 && liq && s != i && (fl >> 8) == 1;
+      S.near = max(S.near, fs ? (r2 < qrc2 ? 2 : 1) : 0);
+    };
     for (int a = -1; a <= 1; ++a) {
       const int jx = cx + a;
       if (jx < 0 || jx >= cp.nx) continue;
@@ -389,21 +551,21 @@ __global__ void __launch_bounds__(256) k_fluid_pair_update(Dev<Pol> d) {
         if (jy < 0 || jy >= cp.ny) continue;
         const R by = yi - cell_shift_y<Pol>(b, cp);
         const int q = cp.cell_off + jx * cp.ny + jy;
-        const int2 rf = d.f_range[q];
-        for (int s = rf.x; s < rf.y; ++s) {  // fluidFluidCalculate
-          const R dx = A.vx[s] - bx, dy = A.vy[s] - by;
-          const R r2 = dx * dx + dy * dy;
-          if (!(r2 < cp.rc2)) continue;
-          const R w = kexp<R>(-(r2 * cp.hinv2)) * cp.kdc;
-          riemann_pair<R, VISCID>(dx, dy, r2, w, ui, vi, A.vu[s], A.vv[s], rhoi, rhoi_inv, A.vrho[s], pi, A.p[s], cs,
-                                  csinv, mu8, cp.eps_h2, S);
-          const int phj = d.f_phase[s];
-          if (!(liq && phj == 1)) S.col += r2 * w;
-          if (liq && s != i && d.f_cls[s] == 1) {  // FREE_SURFACE neighbour
-            if (r2 < cp.quarter_rc2)
-              S.near = 2;
-            else if (S.near == 0)
-              S.near = 1;
+        if (staged) {
+          const int2 rf = s_cr[a + 1][q - T.k0[a + 1]];
+          const int off = T.base[a + 1] - T.lo[a + 1];
+          for_range<2>(rf.x, rf.y, [&](int s) {
+            const R2 pj = s_pos[s + off], vj = s_vel[s + off], rp = s_rp[s + off];
+            const R dx = pj.x - bx, dy = pj.y - by;
+            fluid_j(s, dx, dy, dx * dx + dy * dy, vj.x, vj.y, rp.x, rp.y, s_fl[s + off]);
+          });
+        } else {
+          const int2 rf = d.f_range[q];
+          for (int s = rf.x; s < rf.y; ++s) {
+            const R dx = A.vx[s] - bx, dy = A.vy[s] - by;
+            const R r2 = dx * dx + dy * dy;
+            if (!(r2 < K.rc2)) continue;
+            fluid_j(s, dx, dy, r2, A.vu[s], A.vv[s], A.vrho[s], A.p[s], (int)d.f_phase[s] | ((int)d.f_cls[s] << 8));
           }
         }
         const int2 rs = d.s_range[q];
@@ -411,20 +573,20 @@ __global__ void __launch_bounds__(256) k_fluid_pair_update(Dev<Pol> d) {
           const int j = d.s_sorted[k];
           const R dx = d.s_vx[j] - bx, dy = d.s_vy[j] - by;
           const R r2 = dx * dx + dy * dy;
-          if (!(r2 < cp.rc2)) continue;
-          const R w = kexp<R>(-(r2 * cp.hinv2)) * cp.kdc;
+          if (!(r2 < K.rc2)) continue;
+          const R w = K.w(r2);
           riemann_pair<R, VISCID>(dx, dy, r2, w, ui, vi, (R)d.su[j], (R)d.sv[j], rhoi, rhoi_inv, rhoi, pi, d.s_p[j],
-                                  cs, csinv, mu8, cp.eps_h2, S);
+                                  cs, csinv, mu8, eps_h2, S);
         }
         const int2 rw = d.w_range[q];
         for (int s = rw.x; s < rw.y; ++s) {  // fluidVirtCalculate: rho_j = rho0[wall phase]
           const R dx = d.w_vx[s] - bx, dy = d.w_vy[s] - by;
           const R r2 = dx * dx + dy * dy;
-          if (!(r2 < cp.rc2)) continue;
-          const R w = kexp<R>(-(r2 * cp.hinv2)) * cp.kdc;
+          if (!(r2 < K.rc2)) continue;
+          const R w = K.w(r2);
           const int phj = d.w_phase[s];
           riemann_pair<R, VISCID>(dx, dy, r2, w, ui, vi, d.w_u[s], d.w_v[s], rhoi, rhoi_inv, cp.rho0[phj], pi,
-                                  d.w_p[s], cs, csinv, mu8, cp.eps_h2, S);
+                                  d.w_p[s], cs, csinv, mu8, eps_h2, S);
           if (!(liq && phj == 1)) S.col += r2 * w;
         }
       }
@@ -482,16 +644,27 @@ __global__ void __launch_bounds__(256) k_fluid_pair_update(Dev<Pol> d) {
 // SHIFT: gradient pass fused with the shift update (ping-pong positions)
 // ======================================================================================
 
+constexpr int kShiftCap = 2048;
+
 // correctionCalculate (src/fluid.cu:316-367) + correctionUpdate / shiftDisplacementCalculate
 // (src/fluid.cu:369-427).  The gradient sums over all species with no phase filter.
 template <class Pol>
-__global__ void __launch_bounds__(256) k_fluid_shift(Dev<Pol> d, int pass) {
+__global__ void __launch_bounds__(kTileSlots) k_fluid_shift(Dev<Pol> d, int pass) {
   using R = typename Pol::R;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  using R2 = typename V2<R>::T;
+  __shared__ R2 s_pos[kShiftCap];
+  __shared__ int2 s_cr[3][kTileMaxCells];
+  __shared__ TileHdr T;
+  const int i0 = blockIdx.x * blockDim.x;
+  const int i = i0 + threadIdx.x;
+  const auto &A = d.fa;
+  tile_setup(d, i0, kShiftCap, T, s_cr);
+  const bool staged = T.ok;
+  if (staged) tile_stage(T, [&](int l, int g) { s_pos[l] = R2{A.vx[g], A.vy[g]}; });
+  __syncthreads();
   if (i >= d.nf) return;
   const int c = case_of(i, d.f_case_off, d.ncase);
   const auto &cp = d.cases[c];
-  const auto &A = d.fa;
   if (pass >= cp.K) {  // this case has fewer passes than the batch: carry positions over
     d.fb.x[i] = A.x[i];
     d.fb.y[i] = A.y[i];
@@ -503,7 +676,13 @@ __global__ void __launch_bounds__(256) k_fluid_shift(Dev<Pol> d, int pass) {
   }
   const int pk = d.f_cxy[i], cx = cxy_x(pk), cy = cxy_y(pk);
   const R xi = A.vx[i], yi = A.vy[i];
+  const LoopConsts<R> K(cp);
   R gx = 0, gy = 0;
+  auto add = [&](R dx, R dy) {
+    const R w = K.w(dx * dx + dy * dy);
+    gx += dx * w;
+    gy += dy * w;
+  };
   for (int a = -1; a <= 1; ++a) {
     const int jx = cx + a;
     if (jx < 0 || jx >= cp.nx) continue;
@@ -513,37 +692,24 @@ __global__ void __launch_bounds__(256) k_fluid_shift(Dev<Pol> d, int pass) {
       if (jy < 0 || jy >= cp.ny) continue;
       const R by = yi - cell_shift_y<Pol>(b, cp);
       const int q = cp.cell_off + jx * cp.ny + jy;
-      const int2 rf = d.f_range[q];
-      for (int s = rf.x; s < rf.y; ++s) {
-        const R dx = A.vx[s] - bx, dy = A.vy[s] - by;
-        const R r2 = dx * dx + dy * dy;
-        if (r2 < cp.rc2) {
-          const R w = kexp<R>(-(r2 * cp.hinv2)) * cp.kdc;
-          gx += dx * w;
-          gy += dy * w;
-        }
+      if (staged) {
+        const int2 rf = s_cr[a + 1][q - T.k0[a + 1]];
+        const int off = T.base[a + 1] - T.lo[a + 1];
+        for_range<4>(rf.x + off, rf.y + off, [&](int s) {
+          const R2 pj = s_pos[s];
+          add(pj.x - bx, pj.y - by);
+        });
+      } else {
+        const int2 rf = d.f_range[q];
+        for (int s = rf.x; s < rf.y; ++s) add(A.vx[s] - bx, A.vy[s] - by);
       }
       const int2 rs = d.s_range[q];
       for (int k = rs.x; k < rs.y; ++k) {
         const int j = d.s_sorted[k];
-        const R dx = d.s_vx[j] - bx, dy = d.s_vy[j] - by;
-        const R r2 = dx * dx + dy * dy;
-        if (r2 < cp.rc2) {
-          const R w = kexp<R>(-(r2 * cp.hinv2)) * cp.kdc;
-          gx += dx * w;
-          gy += dy * w;
-        }
+        add(d.s_vx[j] - bx, d.s_vy[j] - by);
       }
       const int2 rw = d.w_range[q];
-      for (int s = rw.x; s < rw.y; ++s) {
-        const R dx = d.w_vx[s] - bx, dy = d.w_vy[s] - by;
-        const R r2 = dx * dx + dy * dy;
-        if (r2 < cp.rc2) {
-          const R w = kexp<R>(-(r2 * cp.hinv2)) * cp.kdc;
-          gx += dx * w;
-          gy += dy * w;
-        }
-      }
+      for (int s = rw.x; s < rw.y; ++s) add(d.w_vx[s] - bx, d.w_vy[s] - by);
     }
   }
   const R umax = fmin((R)sqrt(__longlong_as_double((long long)d.umax2[c])), cp.umax_cap);

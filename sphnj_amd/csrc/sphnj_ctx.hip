@@ -164,6 +164,7 @@ struct Engine : EngineBase {
     c.rc2 = (R)D.rc2;
     c.quarter_rc2 = (R)(0.25 * D.rc2);
     c.hinv2 = (R)D.hinv2;
+    c.nhl2 = (R)(-D.hinv2 * 1.4426950408889634);
     c.kdc = (R)D.kdc;
     c.k2c = (R)D.k2c;
     c.eR = (R)D.eR;
@@ -489,9 +490,8 @@ struct Engine : EngineBase {
         if ((rc = dalloc(&d.s_ref_j, 1)) || (rc = dalloc(&d.s_ref_g, 2))) return rc;
       }
     }
-    // empty fluid / solid cell tables until the first CELLS stage
-    k_clear_ranges<<<nblk(nc, 256), 256, 0, stream>>>(d.f_range, d.s_range, nc);
-    LAUNCH_CHECK();
+    // hash and order the fluid and solid once, so every stage finds valid cells
+    if ((rc = stage_cells())) return rc;
     HIPC(hipStreamSynchronize(stream));
     return SPHNJ_OK;
   }
@@ -508,6 +508,7 @@ struct Engine : EngineBase {
       LAUNCH_CHECK();
       int *sv;
       HIPC(radix_sort_pairs(fsort, d.f_key_by_orig, &f_sorted_keys, &sv, stream));
+      d.f_key = f_sorted_keys;
       k_permute_fluid<Pol><<<nblk(std::max(nf, ncase), blk_stream), blk_stream, 0, stream>>>(d, sv);
       LAUNCH_CHECK();
       swap_fluid_all();

@@ -1,0 +1,78 @@
+"""Shared builders for the tests: parameter sets and hand-made particle configurations."""
+import numpy as np
+
+from sphnj_amd import Case, default_params
+
+
+def params(**kw):
+    """A valid parameter set over a 1 x 1 m box with h = 1.3 dx (dx = 0.02)."""
+    p = default_params()
+    p.particle_diameter = 0.02
+    p.smoothing_length = 0.026
+    p.dt = 1e-4
+    p.iend = 10
+    p.auxetic_modulus = 2e8
+    p.cell_xmin, p.cell_ymin, p.cell_xmax, p.cell_ymax = -0.2, -0.2, 1.2, 1.2
+    p.domain_xmin, p.domain_ymin, p.domain_xmax, p.domain_ymax = 0.0, 0.0, 1.0, 1.0
+    p.liquid_sound_speed = 20.0
+    p.gas_sound_speed = 20.0
+    p.shift_per_step = 1
+    p.particle_shift_coef = 1.0
+    for k, v in kw.items():
+        setattr(p, k, v)
+    return p
+
+
+def fluid(x, y, u=None, v=None, rho=None, p=None, phase=None, rho0=1000.0):
+    n = len(x)
+    z = np.zeros(n)
+    return {
+        "x": np.asarray(x, float), "y": np.asarray(y, float),
+        "u": z.copy() if u is None else np.broadcast_to(np.asarray(u, float), (n,)).copy(),
+        "v": z.copy() if v is None else np.broadcast_to(np.asarray(v, float), (n,)).copy(),
+        "rho": np.full(n, rho0) if rho is None else np.broadcast_to(np.asarray(rho, float), (n,)).copy(),
+        "p": z.copy() if p is None else np.broadcast_to(np.asarray(p, float), (n,)).copy(),
+        "phase": np.zeros(n, np.int32) if phase is None else np.broadcast_to(np.asarray(phase, np.int32), (n,)).copy(),
+    }
+
+
+def solid(x, y, X=None, Y=None, u=None, v=None, material=0, bc=0):
+    n = len(x)
+    z = np.zeros(n)
+    return {
+        "x": np.asarray(x, float), "y": np.asarray(y, float),
+        "u": z.copy() if u is None else np.broadcast_to(np.asarray(u, float), (n,)).copy(),
+        "v": z.copy() if v is None else np.broadcast_to(np.asarray(v, float), (n,)).copy(),
+        "X": np.asarray(x if X is None else X, float).copy(), "Y": np.asarray(y if Y is None else Y, float).copy(),
+        "material": np.broadcast_to(np.asarray(material, np.int32), (n,)).copy(),
+        "bc": np.broadcast_to(np.asarray(bc, np.int32), (n,)).copy(),
+    }
+
+
+def wall(x, y, bc=0):
+    n = len(x)
+    return {"x": np.asarray(x, float), "y": np.asarray(y, float),
+            "bc": np.broadcast_to(np.asarray(bc, np.int32), (n,)).copy()}
+
+
+def empty_fluid():
+    return fluid([], [])
+
+
+def empty_solid():
+    return solid([], [])
+
+
+def empty_wall():
+    return wall([], [])
+
+
+def lattice(nx, ny, dx, x0=0.0, y0=0.0):
+    """Cell-centre lattice ((i + 1/2) dx, (j + 1/2) dx), i outer."""
+    i, j = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    return (x0 + (i.ravel() + 0.5) * dx, y0 + (j.ravel() + 0.5) * dx)
+
+
+def case(p, f=None, s=None, w=None):
+    return Case(p, f if f is not None else empty_fluid(), s if s is not None else empty_solid(),
+                w if w is not None else empty_wall())

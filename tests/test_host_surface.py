@@ -1,0 +1,232 @@
+"""Host driver surface (CPU only): the INI reader with the reference's Config semantics
+(include/config.h:114-159, src/config.cpp:59-129), parameter defaults and validation
+(src/{physics,properties,domain,sim_time,kernel}.cpp), the .dat particle files
+(include/host_data.cuh:24-115, src/properties.cpp readNumber) and the case generators."""
+import ctypes as C
+import math
+import re
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from sphnj_amd import Case, _abi as A, default_params, grid, library, params_from_ini, validate_params
+from sphnj_amd.sim import SphnjError
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def test_defaults_match_reference_table():
+    p = default_params()
+    expect = dict(gravity=9.81, is_steady=0, particle_shift_coef=0.4, is_viscid=0, min_pressure=-1.013e5,
+                  liquid_sound_speed=1500.0, liquid_density=1000.0, liquid_gamma=7.0, liquid_viscosity=1.01e-3,
+                  gas_sound_speed=340.0, gas_density=1.29, gas_gamma=1.4, gas_viscosity=17.9e-6,
+                  solid_density=2700.0, youngs_modulus=67.5e9, poisson_ratio=0.34, arti_vis_alpha=2.5,
+                  arti_vis_beta=2.5, solid_fixed_speed_x=0.0, solid_fixed_speed_y=0.0, is_2D_strain=0, is_plastic=0,
+                  auxetic_density=636.0, auxetic_modulus=0.0, auxetic_poisson_ratio=-0.2, yield_strain=0.0,
+                  yield_stress=2.0e7, harden_strain=0.5, harden_coef=100.0, dt=0.0, istart=0, iend=0,
+                  solid_sub_step=1, result_interval=1, sort_every_step=1, shift_per_step=1, smoothing_length=0.0,
+                  particle_diameter=0.0, impact_length_hsml_ratio=3.0, thread_num=32, background_pressure=0.0,
+                  kernel_2d_coef=0.0)
+    for k, v in expect.items():
+        assert getattr(p, k) == pytest.approx(v, rel=0, abs=0), k
+
+
+def _ini(tmp_path, text):
+    f = tmp_path / "input.ini"
+    f.write_text(text)
+    return params_from_ini(f)
+
+
+def test_ini_comments_sections_and_repeats(tmp_path):
+    p = _ini(tmp_path, """[physics]
+gravity = 1.5 ## trailing comment
+# a comment line with = sign
+is_steady = false
+
+[fluid]
+liquid_density = 999.5
+liquid_density = 1001.25   # a repeated key overwrites the earlier one
+min_pressure=-5e4
+""")
+    assert p.gravity == 1.5
+    assert p.is_steady == 0
+    assert p.liquid_density == 1001.25
+    assert p.min_pressure == -5e4
+    assert p.gas_density == 1.29  # untouched default
+
+
+@pytest.mark.parametrize("text,val", [("false", 0), ("F", 0), ("no", 0), ("n", 0), ("0", 0), ("NONE", 0),
+                                      ("None", 0), ("true", 1), ("1", 1), ("-1", 1), ("yes", 1), ("off", 1)])
+def test_ini_bool_parse(tmp_path, text, val):
+    # string_as_T<bool>: only false/F/no/n/0/none (any case) read as false
+    assert _ini(tmp_path, f"is_viscid = {text}\n").is_viscid == val
+
+
+def test_ini_numeric_extraction_like_istringstream(tmp_path):
+    p = _ini(tmp_path, "iend = 1e5\nistart = 12abc\ndt = 2.5e-4xyz\nsolid_sub_step = junk\n")
+    assert p.iend == 1          # an int reads the leading "1" of "1e5" (the reference INI warns about this)
+    assert p.istart == 12
+    assert p.dt == 2.5e-4
+    assert p.solid_sub_step == 0  # failed extraction -> 0
+
+
+def test_ini_continuation_lines(tmp_path):
+    # a line without '=' that follows a key is appended to its value (joined by '\n'); the
+    # numeric read still takes the leading number, a bool read sees the whole value
+    p = _ini(tmp_path, "gravity = 3.0\n4.0\nis_plastic = false\n[solid]\n\nyield_strain = 0.05\n")
+    assert p.gravity == 3.0
+    assert p.is_plastic == 1    # value "false\n[solid]" is not one of the false spellings
+    assert p.yield_strain == 0.05
+
+
+def test_ini_missing_file(tmp_path):
+    with pytest.raises(SphnjError) as e:
+        params_from_ini(tmp_path / "nope.ini")
+    assert e.value.code == A.E_IO
+
+
+def _valid():
+    p = default_params()
+    p.smoothing_length, p.particle_diameter, p.dt, p.iend = 0.026, 0.02, 1e-4, 10
+    p.auxetic_modulus = 2e8
+    p.cell_xmin, p.cell_ymin, p.cell_xmax, p.cell_ymax = -0.1, -0.1, 1.1, 1.1
+    p.domain_xmin, p.domain_ymin, p.domain_xmax, p.domain_ymax = 0, 0, 1, 1
+    return p
+
+
+@pytest.mark.parametrize("field,value", [("smoothing_length", 0.0), ("particle_diameter", 0.0), ("dt", 0.0),
+                                         ("iend", 0), ("cell_xmax", -0.1), ("domain_ymax", 0.0)])
+def test_validation_rejects(field, value):
+    p = _valid()
+    assert validate_params(p) == 0
+    setattr(p, field, value)
+    assert validate_params(p) == A.E_PARAM
+    assert "FAILED TO READ INPUT.INI" in library().sphnj_global_error().decode()
+
+
+def test_validation_needs_auxetic_or_yield():
+    # src/properties.cpp:150-154 rejects both zero, even for pure-fluid inputs
+    p = _valid()
+    p.auxetic_modulus = 0.0
+    assert validate_params(p) == A.E_PARAM
+    p.yield_strain = 0.05
+    assert validate_params(p) == 0
+
+
+def test_grid_floor_of_extent_over_cutoff():
+    p = _valid()
+    nx, ny, dx, dy = grid(p)
+    cell = 3.0 * 0.026
+    assert nx == math.floor(1.2 / cell) and ny == math.floor(1.2 / cell)
+    assert dx == pytest.approx(1.2 / nx) and dx >= cell
+
+
+def test_dat_round_trip_and_errors(tmp_path):
+    c = Case.generate("gate_fsi", dx=4e-3, tank_w=0.3, tank_h=0.2, S=2)
+    lib = library()
+    h = C.c_void_p()
+    assert lib.sphnj_case_create(b"gate_fsi", b"dx=4e-3;tank_w=0.3;tank_h=0.2;S=2", C.byref(A.Params()),
+                                 C.byref(h)) == 0
+    assert lib.sphnj_case_write(C.byref(c.params), h, str(tmp_path).encode()) == 0
+    lib.sphnj_particles_destroy(h)
+    r = Case.read(tmp_path)
+    for sp in ("fluid", "solid", "wall"):
+        a, b = getattr(c, sp), getattr(r, sp)
+        for k in a:
+            assert np.array_equal(a[k], b[k]), (sp, k)
+    for k, _ in A.Params._fields_:
+        assert getattr(r.params, k) == getattr(c.params, k), k
+    # header count mismatch -> SPHNJ_E_COUNT
+    f = tmp_path / "data" / "virt_particle.dat"
+    lines = f.read_text().splitlines()
+    f.write_text("\n".join([str(int(lines[0]) + 1)] + lines[1:]) + "\n")
+    with pytest.raises(SphnjError) as e:
+        Case.read(tmp_path)
+    assert e.value.code == A.E_COUNT
+
+
+def test_solid_rows_without_bc_column(tmp_path):
+    d = tmp_path / "data"
+    d.mkdir()
+    (d / "fluid_particle.dat").write_text("0 0\n")
+    (d / "virt_particle.dat").write_text("0\n")
+    (d / "solid_particle.dat").write_text("2\n0.1 0.2 0 0 0.1 0.2 0\n0.3 0.4 1 2 0.3 0.4 1 1\n")
+    h = C.c_void_p()
+    assert library().sphnj_particles_read(str(d).encode(), C.byref(h)) == 0
+    c = Case._from_handle(default_params(), h)
+    assert list(c.solid["bc"]) == [0, 1]          # a 7-column row reads bc = 0 (UNCONSTRAINED)
+    assert list(c.solid["material"]) == [0, 1]
+    assert c.solid["u"][1] == 1.0 and c.solid["v"][1] == 2.0
+
+
+def test_missing_dat_file(tmp_path):
+    with pytest.raises(SphnjError) as e:
+        Case.read(tmp_path)
+    assert e.value.code in (A.E_IO,)
+
+
+# ---- case generators: the configurations of the benchmark suite -----------------------
+
+def test_case_dam_break_5k():
+    c = Case.generate("dam_break")
+    nf, ns, nw = c.counts
+    assert (nf, ns) == (5000, 0)
+    assert nw == (200 + 8) * (150 + 8) - 200 * 150     # 4 layers on 4 sides
+    p = c.params
+    c0 = 10 * math.sqrt(2 * 9.81 * 2.0)
+    assert p.liquid_sound_speed == pytest.approx(c0)
+    assert p.dt == pytest.approx(0.25 * 1.3 * 0.02 / c0)
+    assert p.smoothing_length == pytest.approx(1.3 * 0.02)
+    assert p.shift_per_step == 10
+    # hydrostatic: p = rho0 g (H - y), rho = rho0 (p / B + 1)^(1/7)
+    f = c.fluid
+    assert np.allclose(f["p"], 1000 * 9.81 * (2.0 - f["y"]))
+    B = 1000 * c0 ** 2 / 7
+    assert np.allclose(f["rho"], 1000 * (f["p"] / B + 1) ** (1 / 7), rtol=1e-14)
+    assert validate_params(p) == 0
+
+
+def test_case_two_phase_counts():
+    c = Case.generate("dam_break", dx=0.02, air=1)
+    f = c.fluid
+    assert (f["phase"] == 0).sum() == 5000
+    assert (f["phase"] == 1).sum() == 200 * 150 - 5000
+    # continuous hydrostatic pressure across the air/water surface
+    gas = f["phase"] == 1
+    assert np.allclose(f["p"][gas], 1.0 * 9.81 * (3.0 - f["y"][gas]))
+
+
+def test_case_full_size_counts():
+    c = Case.generate("gate_fsi")
+    assert c.counts[:2] == (224000, 6320)
+    assert (c.solid["bc"] == 1).sum() == 4 * 20            # clamped top rows
+    c = Case.generate("wedge_entry", dx=4e-3)
+    nf, ns, nw = c.counts
+    assert ns > 0 and nf > 0 and np.all(c.solid["v"] == -6.0)
+
+
+def test_case_ensemble_members():
+    a = Case.generate("ensemble_member", index=3)
+    b = Case.generate("ensemble_member", index=3)
+    c = Case.generate("ensemble_member", index=4)
+    assert a.counts == b.counts and np.array_equal(a.fluid["x"], b.fluid["x"])
+    assert a.counts != c.counts
+    ns = [Case.generate("ensemble_member", index=i).counts[0] for i in range(16)]
+    assert min(ns) >= 4000 and max(ns) <= 210000
+
+
+def test_unknown_case():
+    with pytest.raises(SphnjError):
+        Case.generate("no_such_case")
+
+
+def test_library_exports_every_declared_symbol():
+    """Every function declared in include/sphnj.h is exported by libsphnj.so (no GPU call)."""
+    header = (ROOT / "include" / "sphnj.h").read_text()
+    names = set(re.findall(r"^\s*(?:int|void|const char \*)\s*(sphnj_\w+)\s*\(", header, re.M))
+    assert len(names) > 30
+    lib = library()
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert lib.sphnj_abi_version() == 1

@@ -1,0 +1,446 @@
+"""Pinning the FP64 oracle (CPU only).
+
+The reference ships no tests, golden data or fixtures, and cannot be built here (nvcc,
+windows.h and a complete Eigen are missing), so the oracle is pinned by what the mathematics
+fixes: closed forms, lattice invariants, symmetry / conservation properties and brute force on
+small inputs.  Each check is chosen so that a plausible slip in the restatement (a dropped
+term, a wrong sign or index, a transposed operand) fails it.
+"""
+import math
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+from conftest import rel_l2
+from helpers import case, fluid, lattice, params, solid, wall
+from sphnj_amd import _abi as A
+
+
+# ---- kernel constants (Kernel::initiate, src/kernel.cpp:25-41; include/global.h:11-13) ------
+
+def test_tg_coef_2_is_the_3d_normalisation_of_the_truncated_gaussian():
+    # tg_coef_2 = 5.551921 = int_{r<3} 4 pi r^2 (exp(-r^2) - exp(-9)) dr
+    val, _ = integrate.quad(lambda r: 4 * math.pi * r * r * (math.exp(-r * r) - math.exp(-9.0)), 0, 3)
+    assert abs(val - 5.551921) < 1e-6
+    assert abs(math.exp(-9.0) - 1.234098e-4) < 1e-9  # tg_coef_1
+
+
+def test_derived_constants(oracle_mod):
+    p = params()
+    o = oracle_mod.Oracle(case(p, fluid([0.5], [0.5])))
+    d = o.derived()
+    h, dd = 0.026, 0.02
+    assert d["rc2"] == pytest.approx(9 * h * h, rel=1e-15)
+    assert d["kdc"] == pytest.approx(2 / 5.551921 * (h / dd) ** -3 / h ** 2 * 2.120729, rel=1e-14)
+    assert d["k2c"] == pytest.approx(1 / 5.551921 * (h / dd) ** -3 * 2.120729, rel=1e-14)
+    assert d["eR"] == pytest.approx(math.exp(-9), rel=1e-15)
+    assert (d["nx"], d["ny"]) == (math.floor(1.4 / (3 * h)), math.floor(1.4 / (3 * h)))
+    # plane stress Lame constants and the bulk sound speed (src/properties.cpp:69-89)
+    E, nu = 67.5e9, 0.34
+    assert d["lambda"] == pytest.approx(E * nu / ((1 - nu) * (1 + nu)))
+    assert d["mu_s"] == pytest.approx(E / (2 * (1 + nu)))
+    assert d["c_s"] == pytest.approx(math.sqrt(E / (3 * (1 - 2 * nu)) / 2700))
+    assert d["umax_cap"] == pytest.approx(0.1 * 20.0)
+    # yield coupling (src/properties.cpp:155-158)
+    assert d["yield_strain"] == pytest.approx(2e7 / 2e8)
+
+
+# ---- lattice invariants of the kernel ---------------------------------------------------------
+
+@pytest.mark.parametrize("hd,count,colour", [(1.2, 37, 1.9954), (1.3, 45, 1.8434)])
+def test_lattice_colour_and_neighbour_count(oracle_mod, hd, count, colour):
+    """Bulk particle of a square lattice: in-range neighbours (self included) and the colour
+    sum C = sum |dx|^2 w'(|dx|^2) at h/d = 1.2 and 1.3."""
+    dx = 0.02
+    p = params(smoothing_length=hd * dx, gravity=0.0)
+    x, y = lattice(31, 31, dx, 0.19, 0.19)
+    o = oracle_mod.Oracle(case(p, fluid(x, y)))
+    o.stage("cells")
+    counts, _ = o.neighbors()
+    centre = 15 * 31 + 15
+    assert counts[centre] == count
+    o.stage("fluid")
+    col = o.download()["fluid"]["color"]
+    assert col[centre] == pytest.approx(colour, abs=5e-5)
+
+
+# ---- EOS (src/fluid.cu:6-23) and the explicit update (src/fluid.cu:256-287) ----------------
+
+def test_isolated_particle_eos_clamps_and_ballistics(oracle_mod):
+    """An isolated particle has only itself as neighbour: zero pair terms, so its density is
+    kept, its pressure is the Tait EOS of it, and it falls ballistically."""
+    c0, g, dt = 20.0, 9.81, 1e-4
+    p = params(liquid_sound_speed=c0, gravity=g, dt=dt, shift_per_step=3)
+    rho = np.array([1010.0, 1000.0, 500.0, 900.0])
+    xs = np.array([0.1, 0.4, 0.7, 0.95])
+    o = oracle_mod.Oracle(case(p, fluid(xs, [0.8] * 4, rho=rho, v=[0.5, 0, 0, 0])))
+    n = 7
+    o.step(n)
+    f = o.download()["fluid"]
+    B = 1000.0 * c0 ** 2 / 7.0
+    rho_exp = np.maximum(rho, 600.0)       # min_density = 0.6 rho0
+    assert np.allclose(f["rho"], rho_exp, rtol=0, atol=1e-12)
+    p_exp = np.maximum(B * ((rho_exp / 1000.0) ** 7 - 1.0), -1.013e5)
+    assert np.allclose(f["p"], p_exp, rtol=1e-12)
+    v0 = np.array([0.5, 0, 0, 0])
+    assert np.allclose(f["v"], v0 - n * g * dt, rtol=0, atol=1e-13)
+    y_exp = 0.8 + n * v0 * dt - g * dt * dt * n * (n - 1) / 2
+    assert np.allclose(f["y"], y_exp, rtol=0, atol=1e-13)
+    assert np.all(f["boundary_type"] == A.SPLASH)   # colour 0, no neighbours: no shift
+    assert np.allclose(f["x"], xs, rtol=0, atol=0)
+
+
+def test_eos_round_trip_and_domain_clamp(oracle_mod):
+    c0 = 30.0
+    p = params(liquid_sound_speed=c0, gravity=100.0, dt=1e-3, shift_per_step=0)
+    pr = np.array([0.0, 1e3, 5e4])
+    B = 1000.0 * c0 ** 2 / 7
+    rho = 1000.0 * (pr / B + 1) ** (1 / 7)          # pressureToDensity
+    o = oracle_mod.Oracle(case(p, fluid([0.2, 0.5, 0.8], [0.0005] * 3, rho=rho, p=pr)))
+    o.step(1)
+    f = o.download()["fluid"]
+    assert np.allclose(f["p"], pr, rtol=1e-12, atol=1e-9)   # densityToPressure(rho(p)) = p
+    o.step(3)
+    assert np.all(o.download()["fluid"]["y"] == 0.0)      # clamped to domain_ymin
+
+
+def test_steady_damping(oracle_mod):
+    p = params(is_steady=1, gravity=0.0, dt=1e-3, shift_per_step=0)
+    o = oracle_mod.Oracle(case(p, fluid([0.5], [0.5], u=[2.0])))
+    o.step(1)
+    assert o.download()["fluid"]["u"][0] == pytest.approx(2.0 - 0.1 * 2.0 * 1e-3, rel=1e-15)
+
+
+# ---- Riemann pair terms: symmetry and conservation ------------------------------------------
+
+def _random_block(seed, n=12, dx=0.02, jitter=0.3):
+    rng = np.random.default_rng(seed)
+    x, y = lattice(n, n, dx, 0.3, 0.3)
+    x = x + jitter * dx * rng.uniform(-1, 1, x.size)
+    y = y + jitter * dx * rng.uniform(-1, 1, y.size)
+    u = rng.normal(0, 0.3, x.size)
+    v = rng.normal(0, 0.3, x.size)
+    rho = 1000.0 + rng.uniform(-5, 5, x.size)
+    pr = rng.uniform(0, 2000, x.size)
+    return x, y, u, v, rho, pr
+
+
+def _accel(oracle_mod, p, f):
+    o = oracle_mod.Oracle(case(p, f))
+    o.stage("cells")
+    o.stage("fluid")
+    g = o.download()["fluid"]
+    return (g["u"] - f["u"]) / p.dt, (g["v"] - f["v"]) / p.dt, (g["rho"] - f["rho"]) / p.dt, g
+
+
+def test_momentum_conserved_by_pair_forces(oracle_mod):
+    """p*_ij is symmetric, dx_ij antisymmetric: sum_i rho_i a_i = 0 (one phase, no walls)."""
+    x, y, u, v, rho, pr = _random_block(1)
+    p = params(gravity=0.0, dt=1e-6)
+    ax, ay, _, _ = _accel(oracle_mod, p, fluid(x, y, u, v, rho, pr))
+    scale = np.abs(rho * ax).sum()
+    assert abs((rho * ax).sum()) < 1e-9 * scale
+    assert abs((rho * ay).sum()) < 1e-9 * scale
+
+
+def test_pair_terms_are_galilean_invariant(oracle_mod):
+    """Accelerations and density rates depend on velocity differences only."""
+    x, y, u, v, rho, pr = _random_block(2)
+    p = params(gravity=0.0, dt=1e-6)
+    ax1, ay1, dr1, _ = _accel(oracle_mod, p, fluid(x, y, u, v, rho, pr))
+    ax2, ay2, dr2, _ = _accel(oracle_mod, p, fluid(x, y, u + 3.0, v - 1.5, rho, pr))
+    assert rel_l2(ax2, ax1) < 1e-6 and rel_l2(ay2, ay1) < 1e-6 and rel_l2(dr2, dr1) < 1e-6
+
+
+def test_uniform_state_is_at_rest_in_the_bulk(oracle_mod):
+    """Uniform pressure, density and velocity: zero acceleration and density rate in the bulk."""
+    x, y = lattice(21, 21, 0.02, 0.3, 0.3)
+    p = params(gravity=0.0, dt=1e-6)
+    f = fluid(x, y, u=0.7, v=-0.2, p=500.0)
+    ax, ay, dr, g = _accel(oracle_mod, p, f)
+    bulk = (np.abs(x - x.mean()) < 0.1) & (np.abs(y - y.mean()) < 0.1)
+    assert np.abs(ax[bulk]).max() < 1e-6 and np.abs(ay[bulk]).max() < 1e-6
+    assert np.abs(dr[bulk]).max() < 1e-6
+    assert np.abs(g["du_dx"][bulk]).max() < 1e-9       # velocity gradient of a uniform field
+    # the edge is pushed out by the uniform pressure (a_i = -2 p sum dx w' / rho points outward)
+    edge = x < x.min() + 1e-9
+    assert np.all(ax[edge] < 0)
+
+
+def test_velocity_gradient_of_a_linear_field(oracle_mod):
+    """L_i = sum (u_j - u_i) dx^T w' for u = A x equals A M with M = sum dx dx^T w'; on the
+    lattice M = (C/2) I with C the colour sum, so L = A C / 2 in the bulk."""
+    x, y = lattice(21, 21, 0.02, 0.3, 0.3)
+    Am = np.array([[0.3, -1.2], [0.7, 0.4]])
+    u = Am[0, 0] * x + Am[0, 1] * y
+    v = Am[1, 0] * x + Am[1, 1] * y
+    p = params(gravity=0.0, dt=1e-9)
+    _, _, _, g = _accel(oracle_mod, p, fluid(x, y, u, v))
+    c = 10 * 21 + 10
+    L = g["du_dx"][c].reshape(2, 2)
+    assert np.allclose(L, Am * g["color"][c] / 2, rtol=1e-9, atol=1e-12)
+
+
+def test_continuity_of_uniform_compression(oracle_mod):
+    """Drho_i = 2 rho_i sum (u_i - v*).dx w'; for u = -e x (uniform compression, equal p, rho)
+    v* is the mean velocity, (u_i - v*).dx = e |dx|^2 / 2, so Drho = rho e C in the bulk (C the
+    colour sum)."""
+    x, y = lattice(21, 21, 0.02, 0.3, 0.3)
+    e = 0.5
+    p = params(gravity=0.0, dt=1e-9)
+    _, _, dr, g = _accel(oracle_mod, p, fluid(x, y, u=-e * (x - 0.5), v=-e * (y - 0.5)))
+    c = 10 * 21 + 10
+    assert dr[c] == pytest.approx(1000.0 * e * g["color"][c], rel=1e-6)
+
+
+def test_liquid_ignores_gas_in_colour(oracle_mod):
+    """Colour of a liquid particle skips gas neighbours; a gas particle counts all."""
+    x, y = lattice(21, 21, 0.02, 0.3, 0.3)
+    phase = (x > 0.5).astype(np.int32)
+    p = params(gravity=0.0, dt=1e-9)
+    _, _, _, g = _accel(oracle_mod, p, fluid(x, y, phase=phase))
+    c_liq = np.where((np.abs(x - 0.49) < 0.011) & (np.abs(y - 0.51) < 0.011))[0][0]
+    c_gas = np.where((np.abs(x - 0.51) < 0.011) & (np.abs(y - 0.51) < 0.011))[0][0]
+    assert g["color"][c_liq] < 1.2          # about half the neighbourhood is gas
+    assert g["color"][c_gas] == pytest.approx(1.8434, abs=5e-4)
+
+
+# ---- walls (src/virt.cu) -------------------------------------------------------------------
+
+def _tank(oracle_mod, bc, u=0.0, gravity=9.81):
+    dx = 0.02
+    p = params(gravity=gravity, dt=1e-9, shift_per_step=0)
+    fx, fy = lattice(30, 20, dx, 0.2, 0.2)
+    wx, wy = lattice(40, 4, dx, 0.0, 0.12)
+    H = 0.6
+    pr = 1000.0 * gravity * (H - fy)
+    o = oracle_mod.Oracle(case(p, fluid(fx, fy, u=u, p=pr), w=wall(wx, wy, bc)))
+    o.stage("cells")
+    o.stage("wall")
+    return o.download(), wx, wy, H
+
+
+def test_wall_pressure_extrapolates_hydrostatics_exactly(oracle_mod):
+    """p_w = sum (p_j - rho_j g.dx) w / sum w is exact for a hydrostatic field."""
+    d, wx, wy, H = _tank(oracle_mod, A.SLIP)
+    w = d["wall"]
+    near = (wx > 0.3) & (wx < 0.7) & (wy > 0.16)          # walls with fluid in range
+    assert near.sum() > 10
+    assert np.allclose(w["p"][near], 1000.0 * 9.81 * (H - wy[near]), rtol=1e-10)
+    assert np.all(w["p"][wy < 0.13] == 0.0)               # no fluid in range: p = 0
+    assert np.all(w["phase"] == A.LIQUID)
+
+
+def test_wall_normals_and_velocity_mirroring(oracle_mod):
+    d, wx, wy, _ = _tank(oracle_mod, A.SLIP, u=0.3, gravity=0.0)
+    w = d["wall"]
+    top = (wy > 0.185) & (wx > 0.2) & (wx < 0.6)          # inner layer, away from the ends
+    n = w["normal"][top]
+    assert np.allclose(n, [0.0, 1.0], atol=1e-12)
+    assert np.allclose(w["u"][top], 0.3, rtol=1e-12)      # slip: tangential velocity kept
+    d2, _, _, _ = _tank(oracle_mod, A.NO_SLIP, u=0.3, gravity=0.0)
+    assert np.allclose(d2["wall"]["u"][top], -0.3, rtol=1e-12)   # no-slip: reversed
+    # corner of the block: normal along the diagonal away from the wall mass
+    corner = np.argmin(wx + wy)
+    assert np.allclose(w["normal"][corner], [-1 / math.sqrt(2), -1 / math.sqrt(2)], atol=1e-12)
+
+
+# ---- solids (src/solid.cu) -----------------------------------------------------------------
+
+def _solid_block(A_map=None, material=0, n=9, dx=0.02, **kw):
+    X, Y = lattice(n, n, dx, 0.3, 0.3)
+    if A_map is None:
+        x, y = X.copy(), Y.copy()
+    else:
+        x = A_map[0, 0] * (X - 0.4) + A_map[0, 1] * (Y - 0.4) + 0.4
+        y = A_map[1, 0] * (X - 0.4) + A_map[1, 1] * (Y - 0.4) + 0.4
+    p = params(gravity=0.0, dt=1e-9, solid_sub_step=1, **kw)
+    return case(p, s=solid(x, y, X, Y, material=material))
+
+
+def _solid_after(oracle_mod, c):
+    o = oracle_mod.Oracle(c)
+    o.stage("cells")
+    o.stage("solid")
+    return o.download()["solid"]
+
+
+def test_linear_map_gives_exact_deformation_gradient(oracle_mod):
+    """F = F~ B^-1 with F~ = sum (x_j - x_i) dX^T w' = A B for x = A X: F = A at every particle."""
+    th = 0.3
+    R = np.array([[math.cos(th), -math.sin(th)], [math.sin(th), math.cos(th)]])
+    Am = R @ np.array([[1.02, 0.01], [0.0, 0.97]])
+    s = _solid_after(oracle_mod, _solid_block(Am))
+    assert np.allclose(s["deformation"], Am.ravel(), atol=1e-12)
+    # L = B^-1 is symmetric positive definite
+    L = s["correction"].reshape(-1, 2, 2)
+    assert np.allclose(L, L.transpose(0, 2, 1), atol=1e-9)
+    assert np.all(np.linalg.det(L) > 0)
+
+
+def test_svk_stress_closed_forms(oracle_mod):
+    E, nu = 1.0e7, 0.3
+    lam, mu = E * nu / ((1 - nu) * (1 + nu)), E / (2 * (1 + nu))
+    # rigid rotation: P(F = R) = 0
+    th = 0.4
+    R = np.array([[math.cos(th), -math.sin(th)], [math.sin(th), math.cos(th)]])
+    s = _solid_after(oracle_mod, _solid_block(R, youngs_modulus=E, poisson_ratio=nu))
+    assert np.abs(s["stress"]).max() < 1e-6 * E
+    # uniaxial stretch F = diag(1 + e, 1): E_g = diag(e + e^2/2, 0), P = F (2 mu E_g + lam tr E_g I)
+    e = 0.01
+    F = np.diag([1 + e, 1.0])
+    s = _solid_after(oracle_mod, _solid_block(F, youngs_modulus=E, poisson_ratio=nu))
+    Eg = np.diag([e + e * e / 2, 0.0])
+    P = F @ (2 * mu * Eg + lam * np.trace(Eg) * np.eye(2))
+    assert np.allclose(s["stress"], P.ravel(), rtol=1e-8, atol=1e-6)
+
+
+def test_plane_strain_lame(oracle_mod):
+    E, nu, e = 1.0e7, 0.3, 0.01
+    lam, mu = E * nu / ((1 - 2 * nu) * (1 + nu)), E / (2 * (1 + nu))
+    F = np.diag([1.0, 1 + e])
+    s = _solid_after(oracle_mod, _solid_block(F, youngs_modulus=E, poisson_ratio=nu, is_2D_strain=1))
+    Eg = np.diag([0.0, e + e * e / 2])
+    P = F @ (2 * mu * Eg + lam * np.trace(Eg) * np.eye(2))
+    assert np.allclose(s["stress"], P.ravel(), rtol=1e-8, atol=1e-6)
+
+
+def test_neo_hookean_closed_form(oracle_mod):
+    E, nu = 1.2e7, 0.4
+    lam, mu = E * nu / ((1 - nu) * (1 + nu)), E / (2 * (1 + nu))
+    F = np.array([[1.03, 0.02], [-0.01, 0.98]])
+    s = _solid_after(oracle_mod, _solid_block(F, material=A.NEO_HOOKEAN, youngs_modulus=E, poisson_ratio=nu))
+    FiT = np.linalg.inv(F).T
+    P = mu * (F - FiT) + lam * math.log(np.linalg.det(F)) * FiT
+    assert np.allclose(s["stress"], P.ravel(), rtol=1e-8, atol=1e-6)
+
+
+def test_auxetic_uniaxial_closed_form(oracle_mod):
+    """Principal axes = coordinate axes for F = diag(1+e, 1): sigma_k from the piecewise law
+    of plasticElasticFunction (src/solid.cu:80-108) with max_strain = -yield_strain, P = F Sigma."""
+    ys, ystr, hs, hc, e = 0.05, 2.0e6, 0.6, 1.0e12, 0.02
+    F = np.diag([1 + e, 1.0])
+    s = _solid_after(oracle_mod, _solid_block(F, material=A.AUXETIC, yield_strain=ys, yield_stress=ystr,
+                                              harden_strain=hs, harden_coef=hc))
+    mod = ystr / ys
+
+    def law(strain, m):
+        ts = math.sqrt(1 + 2 * strain) - 1 if strain > -0.5 else -1.0
+        if ts > m:
+            return mod * (ts - m) - ystr
+        if ts > -hs:
+            return -ystr
+        return -(hc * abs(-hs - ts) ** 2 + ystr)
+
+    eg = e + e * e / 2
+    P = F @ np.diag([law(eg, -ys), law(0.0, -ys)])
+    assert np.allclose(s["stress"], P.ravel(), rtol=1e-9, atol=1e-6)
+
+
+def test_solid_internal_forces_conserve_momentum(oracle_mod):
+    """(P_j L_j + P_i L_i) dX w' is antisymmetric: sum_i a_i = 0 at rest (no viscosity)."""
+    rng = np.random.default_rng(5)
+    c = _solid_block(youngs_modulus=1e7)
+    c.solid["x"] += rng.normal(0, 2e-4, c.solid["x"].size)
+    c.solid["y"] += rng.normal(0, 2e-4, c.solid["y"].size)
+    s = _solid_after(oracle_mod, c)
+    a = s["du_dt"]
+    assert np.abs(a.sum(axis=0)).max() < 1e-9 * np.abs(a).sum()
+    assert np.abs(a).max() > 1.0
+
+
+def test_fixed_particles_follow_the_prescribed_speed(oracle_mod):
+    c = _solid_block(solid_fixed_speed_x=0.5, solid_fixed_speed_y=-2.0)
+    c.solid["bc"][:] = A.FIXED
+    c.params.dt = 1e-3
+    s = _solid_after(oracle_mod, c)
+    assert np.allclose(s["u"], 0.5) and np.allclose(s["v"], -2.0)
+    assert np.allclose(s["x"], c.solid["x"] + 0.5e-3) and np.allclose(s["du_dt"], 0.0)
+
+
+# ---- cells, sort and neighbour lists -------------------------------------------------------
+
+def test_hash_sort_and_cell_starts_match_closed_form(oracle_mod):
+    rng = np.random.default_rng(7)
+    p = params()
+    x = rng.uniform(-0.19, 1.19, 400)
+    y = rng.uniform(-0.19, 1.19, 400)
+    o = oracle_mod.Oracle(case(p, fluid(x, y)))
+    o.stage("cells")
+    keys, orig, start = o.sorted(0, A.FLUID)
+    cell = 3 * 0.026
+    nx = math.floor(1.4 / cell)
+    ix = np.trunc((x + 0.2) * (1 / (1.4 / nx))).astype(int)
+    iy = np.trunc((y + 0.2) * (1 / (1.4 / nx))).astype(int)
+    k = iy + ix * nx
+    order = np.lexsort((np.arange(400), k))             # (cell, original index)
+    assert np.array_equal(orig, order) and np.array_equal(keys, k[order])
+    assert np.array_equal(start, np.searchsorted(k[order], np.arange(nx * nx + 1), side="left"))
+
+
+def test_neighbour_lists_equal_brute_force(oracle_mod):
+    rng = np.random.default_rng(11)
+    p = params()
+    n = 300
+    x = rng.uniform(0.1, 0.5, n)
+    y = rng.uniform(0.1, 0.5, n)
+    wx, wy = rng.uniform(0.1, 0.5, 40), rng.uniform(0.1, 0.5, 40)
+    o = oracle_mod.Oracle(case(p, fluid(x, y), w=wall(wx, wy)))
+    o.stage("cells")
+    counts, lst = o.neighbors()
+    rc2 = 9 * 0.026 ** 2
+    off = np.concatenate([[0], np.cumsum(counts)])
+    for i in range(n):
+        got = set(map(int, lst[off[i]:off[i + 1]]))
+        f = {(0 << 28) | j for j in range(n) if (x[j] - x[i]) ** 2 + (y[j] - y[i]) ** 2 < rc2}
+        w = {(2 << 28) | j for j in range(40) if (wx[j] - x[i]) ** 2 + (wy[j] - y[i]) ** 2 < rc2}
+        assert got == f | w, i
+
+
+def test_solid_reference_lists_equal_brute_force(oracle_mod):
+    rng = np.random.default_rng(3)
+    X, Y = lattice(10, 10, 0.02, 0.3, 0.3)
+    X = X + rng.uniform(-0.004, 0.004, X.size)
+    c = case(params(), s=solid(X + 0.001, Y, X, Y))
+    o = oracle_mod.Oracle(c)
+    off, lst = o.reflist()
+    rc2 = 9 * 0.026 ** 2
+    for i in range(X.size):
+        exp = {j for j in range(X.size) if (X[j] - X[i]) ** 2 + (Y[j] - Y[i]) ** 2 < rc2}
+        assert set(lst[off[i]:off[i + 1]]) == exp
+
+
+def test_status_flags(oracle_mod):
+    p = params()
+    o = oracle_mod.Oracle(case(p, fluid([0.5, 5.0], [0.5, 0.5])))      # second particle outside the box
+    o.stage("cells")
+    assert o.status() & A.ST_OUT_OF_GRID
+    x, y = lattice(9, 9, 0.008, 0.127, 0.127)                           # 81 particles in one cell
+    o = oracle_mod.Oracle(case(p, fluid(x, y)))
+    o.stage("cells")
+    assert o.status() & A.ST_CELL_OVER_64
+
+
+def test_energy_of_uniform_translation(oracle_mod):
+    x, y = lattice(10, 10, 0.02, 0.3, 0.3)
+    o = oracle_mod.Oracle(case(params(), fluid(x, y, u=2.0, v=1.0)))
+    e = o.energy()
+    assert e[0] == pytest.approx(100 * 0.5 * 1000 * 0.02 ** 2 * 5.0, rel=1e-14)
+    assert e[1] == pytest.approx(100 * 1000 * 0.02 ** 2, rel=1e-14)
+    assert e[2] == pytest.approx(math.sqrt(5.0), rel=1e-15)
+
+
+# ---- integration: a resting tank stays at rest -------------------------------------------
+
+def test_hydrostatic_tank_stays_near_rest(oracle_mod):
+    from sphnj_amd import Case
+
+    c = Case.generate("dam_break", tank_w=0.6, tank_h=0.7, col_w=0.6, col_h=0.6, dx=0.02, K=1)
+    o = oracle_mod.Oracle(c)
+    o.step(150)
+    f = o.download()["fluid"]
+    c0 = c.params.liquid_sound_speed
+    assert np.hypot(f["u"], f["v"]).max() < 1e-3 * c0
+    assert abs(f["y"].max() - c.fluid["y"].max()) < 0.1 * 0.02
+    assert o.status() == 0

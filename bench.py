@@ -54,6 +54,21 @@ def configs():
     }
 
 
+def measured_traffic(config, stage):
+    """HBM bytes per launch of the stage's kernel as measured by the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE passes (profiles/pmc_traffic_<workload>.json, copy-calibrated);
+    None where no measurement of this workload is committed."""
+    kern = {"shift": "k_fluid_shift<sphnj::M32", "fluid": "k_fluid_pair_update<sphnj::M32",
+            "cells": None}[stage]
+    f = ROOT / "profiles" / {"two_phase": "pmc_traffic_two_phase_3M.json"}.get(config, "-")
+    if kern is None or not f.exists():
+        return None
+    for k, v in json.loads(f.read_text())["kernels"].items():
+        if kern in k:
+            return v["hbm_bytes_per_dispatch"]
+    return None
+
+
 def rel_l2(a, b):
     a = np.asarray(a, np.float64).ravel()
     b = np.asarray(b, np.float64).ravel()
@@ -69,7 +84,25 @@ def timed_steps(sim, steps):
     return time.perf_counter() - t0
 
 
+def copy_calibration(mib):
+    """A plain device copy of `mib` MiB (torch), so that rocprofv3 FETCH_SIZE / WRITE_SIZE
+    readings can be scaled by what the counters report for a copy of known size."""
+    import torch
+
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(f"# copy calibration: 3 x {mib} MiB device copies", file=sys.stderr, flush=True)
+
+
 def stage_profile(sim, steps):
+    if steps <= 0:
+        return {st: {"launches": 0, "ms_per_step": 0.0, "ms_per_launch": 0.0}
+                for st in ("cells", "couple", "wall", "fluid", "solid", "shift")}
     sim.set_stage_timing(True)
     sim.step(steps)
     out = {}
@@ -89,6 +122,7 @@ def run_ensemble(args):
     n_total = sum(sum(m.counts) for m in members)
     batch = Simulation(members, "m32")
     batch.step(args.warmup)
+    batch.autotune()
     t = timed_steps(batch, args.steps)
     value = n_total * args.steps / t / 1e6
     serial = 0.0
@@ -101,7 +135,7 @@ def run_ensemble(args):
         b = one.download()["fluid"]
         errs.append(max(rel_l2(a[k], b[k]) for k in ("x", "y", "u", "v", "rho", "p")))
         one.close()
-    p64_value = n_total * args.steps / serial / 1e6
+    p64_value = n_total * args.steps / serial / 1e6 if serial > 0 else float("nan")
     return value, p64_value, n_total, t, errs
 
 
@@ -116,6 +150,9 @@ def main() -> int:
     ap.add_argument("--members", type=int, default=256)
     ap.add_argument("--serial-members", type=int, default=16)
     ap.add_argument("--no-p64", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="keep the default block sizes")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run device copies of MIB MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -124,6 +161,9 @@ def main() -> int:
         return 2
 
     from sphnj_amd import Simulation
+
+    if args.calibrate_copy:
+        copy_calibration(args.calibrate_copy)
 
     if args.config == "ensemble":
         value, p64_value, n_total, t, errs = run_ensemble(args)
@@ -146,6 +186,7 @@ def main() -> int:
 
     sim = Simulation(case, "m32")
     sim.step(args.warmup)
+    tune = sim.autotune() if not args.no_tune else None
     t = timed_steps(sim, args.steps)
     ev_ms = sim.last_step_ms()
     value = n_total * args.steps / t / 1e6
@@ -157,6 +198,7 @@ def main() -> int:
     if not args.no_p64:
         ref = Simulation(case, "p64")
         ref.step(args.warmup)
+        tune64 = ref.autotune() if not args.no_tune else None
         tp = timed_steps(ref, args.steps)
         p64_value = n_total * args.steps / tp / 1e6
         prof64 = stage_profile(ref, args.profile_steps)
@@ -168,20 +210,22 @@ def main() -> int:
                  "speedup_vs_p64": round(value / p64_value, 3),
                  "rel_l2_vs_p64": {k: float(f"{v:.3e}") for k, v in errs.items()},
                  "steps_compared": args.warmup + args.steps + args.profile_steps,
-                 "p64_stage_ms": {k: round(v["ms_per_step"], 4) for k, v in prof64.items()}}
+                 "p64_stage_ms": {k: round(v["ms_per_step"], 4) for k, v in prof64.items()},
+                 "p64_blocks": tune64 and {k: v["block"] for k, v in tune64.items()}}
         print(f"# p64: {1e3 * tp / args.steps:.3f} ms/step, {p64_value:.1f} M updates/s; "
               f"max rel L2 {max(errs.values()):.2e}", file=sys.stderr, flush=True)
         ref.close()
 
     # roofline of the dominant kernel (largest share of the step)
-    dom = max(("fluid", "shift", "cells"), key=lambda s: prof[s]["ms_per_step"])
+    dom = max(("fluid", "shift", "cells"), key=lambda s: prof[s]["ms_per_step"]) if args.profile_steps else "shift"
     per_launch_ms = prof[dom]["ms_per_launch"]
     bytes_launch = BYTES[dom]["m32"] * nf
     achieved = bytes_launch / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms else 0.0
     kernel_name = {"fluid": "k_fluid_pair_update<M32>", "shift": "k_fluid_shift<M32>",
                    "cells": "cells stage (hash + radix sort + permute + ranges)"}[dom]
     roofline = {"bound": "hbm", "kernel": kernel_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(args.config, dom),
                 "bytes_per_particle": BYTES[dom]["m32"], "ms_per_launch": round(per_launch_ms, 5)}
 
     cpu = None
@@ -206,7 +250,8 @@ def main() -> int:
                        "dt": case.params.dt, "shift_per_step": case.params.shift_per_step,
                        "solid_sub_step": case.params.solid_sub_step},
             "roofline": roofline, "cpu_baseline": cpu,
-            "stage_ms": {k: round(v["ms_per_step"], 4) for k, v in prof.items()}}
+            "stage_ms": {k: round(v["ms_per_step"], 4) for k, v in prof.items()},
+            "autotune": tune}
     line.update(extra)
     print(json.dumps(line))
     return 0

@@ -223,9 +223,14 @@ void sphnj_destroy(sphnj_ctx *ctx);
 /* ---- execution options ---------------------------------------------------------------- */
 /* Capture the step into hipGraphs (default on) and replay them. */
 int sphnj_set_graph(sphnj_ctx *ctx, int enable);
-/* Time candidate block sizes for the pair kernels on the current state and keep the best
- * (the state is restored afterwards). */
+/* Autotuner: times the fluid pair kernel (family 0) and the shift kernel (family 1) at block
+ * sizes 64, 128 and 256 -- a block owns that many consecutive sorted particles, so the block
+ * size also sets the cells each LDS tile spans -- on the current state with HIP events, and
+ * keeps the fastest.  The particle state is unchanged afterwards; the per-step transients
+ * (colour, near flag, velocity gradient) are recomputed from it. */
 int sphnj_autotune(sphnj_ctx *ctx);
+/* Milliseconds per launch measured by the last sphnj_autotune for block sizes 64, 128, 256. */
+int sphnj_autotune_report(const sphnj_ctx *ctx, int32_t family, double ms_per_launch[3]);
 /* Block size in use for a kernel family (0 = pair, 1 = shift, 2 = streaming). */
 int sphnj_get_block(const sphnj_ctx *ctx, int32_t family, int32_t *block);
 int sphnj_set_block(sphnj_ctx *ctx, int32_t family, int32_t block);

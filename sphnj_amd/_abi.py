@@ -144,6 +144,7 @@ LIB_PROTOS = {
     "sphnj_last_error": (C.c_char_p, [VP]),
     "sphnj_set_graph": (C.c_int, [VP, C.c_int]),
     "sphnj_autotune": (C.c_int, [VP]),
+    "sphnj_autotune_report": (C.c_int, [VP, C.c_int32, c_double_p]),
     "sphnj_get_block": (C.c_int, [VP, C.c_int32, c_int32_p]),
     "sphnj_set_block": (C.c_int, [VP, C.c_int32, C.c_int32]),
     "sphnj_last_step_ms": (C.c_int, [VP, c_double_p]),

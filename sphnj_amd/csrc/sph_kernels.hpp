@@ -320,17 +320,17 @@ __global__ void __launch_bounds__(256) k_wall_update(Dev<Pol> d) {
 // ======================================================================================
 // LDS neighbour tiles
 // ======================================================================================
-// A block owns kTileSlots consecutive sorted fluid slots, whose global keys lie in [k0, k1].
+// A block owns blockDim.x consecutive sorted fluid slots, whose global keys lie in [k0, k1].
 // The 3x3 neighbour cells of any of them that sit in neighbour column a (-1, 0, +1) have keys
 // in [k0 + a ny - 1, k1 + a ny + 1]: a contiguous key range, hence a contiguous slot segment.
 // The block stages the three segments in LDS once (coalesced) and every lane reads its
 // candidates from there.  A tile that straddles two cases, spans more than kTileMaxCells
 // cells per segment or holds more than the kernel's capacity falls back to global loads.
-constexpr int kTileSlots = 256;
 constexpr int kTileMaxCells = 96;
 
 struct TileHdr {
   int ok;
+  int c0, c1;  // cases of the first and last slot of the block
   int k0[3], nc[3], lo[3], hi[3], base[3];
 };
 
@@ -365,6 +365,8 @@ __device__ __forceinline__ void tile_setup(const Dev<Pol> &d, int i0, int cap, T
       T.hi[a] = -1;
     }
     T.ok = (c0 == c1) && (k1 - k0 + 3 <= kTileMaxCells);
+    T.c0 = c0;
+    T.c1 = c1;
   }
   __syncthreads();
   if (T.ok) {
@@ -486,20 +488,26 @@ __device__ __forceinline__ void riemann_pair(R dx, R dy, R r2, R w, R ui, R vi, 
   }
 }
 
-template <class Pol>
+// LDS capacity of a tile (particles) for a block of BLOCK slots: about 3.4 x BLOCK particles
+// are staged in a dense region; the rest is headroom for sparse or compressed tiles.
+template <class Pol, int BLOCK>
 struct PairCap {
-  static constexpr int value = Pol::MIXED ? 1536 : 1024;
+  static constexpr int value = (Pol::MIXED ? 1536 : 1024) * BLOCK / 256;
+};
+template <int BLOCK>
+struct ShiftCap {
+  static constexpr int value = 2048 * BLOCK / 256;
 };
 
 // findFluidPair (src/fluid.cu:193-254) + fluidUpdate (src/fluid.cu:256-287).  Reads the
 // current buffers (fa) of every neighbour and writes particle i's updated state into the
 // alternate buffers (fb), so neighbours never see a half-updated state.  Also reduces the
 // per-case max |u|^2 of the updated velocities (getMaxSpeed, src/fluid.cu:429-440).
-template <class Pol, bool VISCID>
-__global__ void __launch_bounds__(kTileSlots) k_fluid_pair_update(Dev<Pol> d) {
+template <class Pol, bool VISCID, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d) {
   using R = typename Pol::R;
   using R2 = typename V2<R>::T;
-  constexpr int CAP = PairCap<Pol>::value;
+  constexpr int CAP = PairCap<Pol, BLOCK>::value;
   __shared__ R2 s_pos[CAP], s_vel[CAP], s_rp[CAP];
   __shared__ int s_fl[CAP];
   __shared__ int2 s_cr[3][kTileMaxCells];
@@ -519,9 +527,8 @@ __global__ void __launch_bounds__(kTileSlots) k_fluid_pair_update(Dev<Pol> d) {
   }
   __syncthreads();
   unsigned long long my_u2 = 0ull;
-  int c = 0;
+  const int c = T.c0 == T.c1 ? T.c0 : (i < d.nf ? case_of(i, d.f_case_off, d.ncase) : T.c1);
   if (i < d.nf) {
-    c = case_of(i, d.f_case_off, d.ncase);
     const auto &cp = d.cases[c];
     const int pk = d.f_cxy[i], cx = cxy_x(pk), cy = cxy_y(pk);
     const R xi = A.vx[i], yi = A.vy[i], ui = A.vu[i], vi = A.vv[i], rhoi = A.vrho[i], pi = A.p[i];
@@ -631,10 +638,12 @@ __global__ void __launch_bounds__(kTileSlots) k_fluid_pair_update(Dev<Pol> d) {
     my_u2 = (unsigned long long)__double_as_longlong(u2);
     if (isnan(rho) || isnan(x) || isnan(y) || isnan(u2)) atomicOr(d.status, kStNaN);
   }
-  // per-case max |u|^2: non-negative doubles order like their bit patterns
-  if (d.ncase == 1) {
+  // per-case max |u|^2 (non-negative doubles order like their bit patterns): one atomic per
+  // wave when the wave lies in one case, else one per lane
+  const int c_lane0 = __shfl(c, 0, 64);
+  if (__ballot(c != c_lane0 ? 1 : 0) == 0ull) {
     const unsigned long long m = wave_max_u64(my_u2);
-    if ((threadIdx.x & 63) == 0 && m != 0ull) atomicMax(&d.umax2[0], m);
+    if ((threadIdx.x & 63) == 0 && m != 0ull) atomicMax(&d.umax2[c], m);
   } else if (i < d.nf && my_u2 != 0ull) {
     atomicMax(&d.umax2[c], my_u2);
   }
@@ -644,26 +653,25 @@ __global__ void __launch_bounds__(kTileSlots) k_fluid_pair_update(Dev<Pol> d) {
 // SHIFT: gradient pass fused with the shift update (ping-pong positions)
 // ======================================================================================
 
-constexpr int kShiftCap = 2048;
-
 // correctionCalculate (src/fluid.cu:316-367) + correctionUpdate / shiftDisplacementCalculate
 // (src/fluid.cu:369-427).  The gradient sums over all species with no phase filter.
-template <class Pol>
-__global__ void __launch_bounds__(kTileSlots) k_fluid_shift(Dev<Pol> d, int pass) {
+template <class Pol, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, int pass) {
   using R = typename Pol::R;
   using R2 = typename V2<R>::T;
-  __shared__ R2 s_pos[kShiftCap];
+  constexpr int CAP = ShiftCap<BLOCK>::value;
+  __shared__ R2 s_pos[CAP];
   __shared__ int2 s_cr[3][kTileMaxCells];
   __shared__ TileHdr T;
   const int i0 = blockIdx.x * blockDim.x;
   const int i = i0 + threadIdx.x;
   const auto &A = d.fa;
-  tile_setup(d, i0, kShiftCap, T, s_cr);
+  tile_setup(d, i0, CAP, T, s_cr);
   const bool staged = T.ok;
   if (staged) tile_stage(T, [&](int l, int g) { s_pos[l] = R2{A.vx[g], A.vy[g]}; });
   __syncthreads();
   if (i >= d.nf) return;
-  const int c = case_of(i, d.f_case_off, d.ncase);
+  const int c = T.c0 == T.c1 ? T.c0 : case_of(i, d.f_case_off, d.ncase);
   const auto &cp = d.cases[c];
   if (pass >= cp.K) {  // this case has fewer passes than the batch: carry positions over
     d.fb.x[i] = A.x[i];
@@ -770,6 +778,22 @@ __global__ void __launch_bounds__(kTileSlots) k_fluid_shift(Dev<Pol> d, int pass
 // ======================================================================================
 // SOLID
 // ======================================================================================
+// Solid kernels give each particle a group of kSolidLanes lanes: the lanes split the
+// particle's candidates (cell ranges, reference list) and combine their partial sums with
+// shuffles, so a few thousand solids still fill the chip.
+constexpr int kSolidLanes = 16;
+
+template <class T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int o = kSolidLanes / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kSolidLanes);
+  return v;
+}
+__device__ __forceinline__ int group_or(int v) {
+#pragma unroll
+  for (int o = kSolidLanes / 2; o > 0; o >>= 1) v |= __shfl_xor(v, o, kSolidLanes);
+  return v;
+}
 
 // solidCoupleUpdate (src/solid.cu:355-444): Shepard pressure from the fluid, normal from the
 // solid neighbours in the current cells; and the check that every reference-list neighbour
@@ -777,46 +801,61 @@ __global__ void __launch_bounds__(kTileSlots) k_fluid_shift(Dev<Pol> d, int pass
 template <class Pol>
 __global__ void __launch_bounds__(256) k_solid_couple(Dev<Pol> d) {
   using R = typename Pol::R;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.ns) return;
-  const int c = case_of(i, d.s_case_off, d.ncase);
-  const auto &cp = d.cases[c];
-  const int pk = d.s_cxy[i], cx = cxy_x(pk), cy = cxy_y(pk);
-  const R xi = d.s_vx[i], yi = d.s_vy[i];
-  const auto &F = d.fa;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gid / kSolidLanes, l = gid % kSolidLanes;
+  const bool active = i < d.ns;
   R nx = 0, ny = 0, P = 0, S = 0;
-  for (int a = -1; a <= 1; ++a) {
-    const int jx = cx + a;
-    if (jx < 0 || jx >= cp.nx) continue;
-    const R bx = xi - cell_shift_x<Pol>(a, cp);
-    for (int b = -1; b <= 1; ++b) {
-      const int jy = cy + b;
-      if (jy < 0 || jy >= cp.ny) continue;
-      const R by = yi - cell_shift_y<Pol>(b, cp);
-      const int q = cp.cell_off + jx * cp.ny + jy;
-      const int2 rf = d.f_range[q];
-      for (int s = rf.x; s < rf.y; ++s) {
-        const R dx = F.vx[s] - bx, dy = F.vy[s] - by;
-        const R r2 = dx * dx + dy * dy;
-        if (r2 < cp.rc2) {
-          const R w = (kexp<R>(-(r2 * cp.hinv2)) - cp.eR) * cp.k2c;
-          S += w;
-          P += F.p[s] * w;
+  int left = 0;
+  int c = 0;
+  if (active) {
+    c = case_of(i, d.s_case_off, d.ncase);
+    const auto &cp = d.cases[c];
+    const int pk = d.s_cxy[i], cx = cxy_x(pk), cy = cxy_y(pk);
+    const R xi = d.s_vx[i], yi = d.s_vy[i];
+    const LoopConsts<R> K(cp);
+    const R eR = cp.eR, k2c = cp.k2c, hinv2 = cp.hinv2;
+    const auto &F = d.fa;
+    for (int a = -1; a <= 1; ++a) {
+      const int jx = cx + a;
+      if (jx < 0 || jx >= cp.nx) continue;
+      const R bx = xi - cell_shift_x<Pol>(a, cp);
+      for (int b = -1; b <= 1; ++b) {
+        const int jy = cy + b;
+        if (jy < 0 || jy >= cp.ny) continue;
+        const R by = yi - cell_shift_y<Pol>(b, cp);
+        const int q = cp.cell_off + jx * cp.ny + jy;
+        const int2 rf = d.f_range[q];
+        for (int s = rf.x + l; s < rf.y; s += kSolidLanes) {
+          const R dx = F.vx[s] - bx, dy = F.vy[s] - by;
+          const R r2 = dx * dx + dy * dy;
+          if (r2 < K.rc2) {
+            const R w = (kexp<R>(-(r2 * hinv2)) - eR) * k2c;
+            S += w;
+            P += F.p[s] * w;
+          }
         }
-      }
-      const int2 rs = d.s_range[q];
-      for (int k = rs.x; k < rs.y; ++k) {
-        const int j = d.s_sorted[k];
-        const R dx = d.s_vx[j] - bx, dy = d.s_vy[j] - by;
-        const R r2 = dx * dx + dy * dy;
-        if (r2 < cp.rc2) {
-          const R w = kexp<R>(-(r2 * cp.hinv2)) * cp.kdc;
+        const int2 rs = d.s_range[q];
+        for (int k = rs.x + l; k < rs.y; k += kSolidLanes) {
+          const int j = d.s_sorted[k];
+          const R dx = d.s_vx[j] - bx, dy = d.s_vy[j] - by;
+          const R w = K.w(dx * dx + dy * dy);
           nx -= dx * w;
           ny -= dy * w;
         }
       }
     }
+    for (int e = d.s_ref_off[i] + l; e < d.s_ref_off[i + 1]; e += kSolidLanes) {
+      const int pj = d.s_cxy[d.s_ref_j[e]];
+      left |= abs(cxy_x(pj) - cx) > 1 || abs(cxy_y(pj) - cy) > 1;
+    }
   }
+  S = group_sum(S);
+  P = group_sum(P);
+  nx = group_sum(nx);
+  ny = group_sum(ny);
+  left = group_or(left);
+  if (!active || l != 0) return;
+  const auto &cp = d.cases[c];
   const R nn = sqrt(nx * nx + ny * ny);
   if (nn < R(0.1)) {
     d.s_n[2 * i] = 0;
@@ -826,11 +865,6 @@ __global__ void __launch_bounds__(256) k_solid_couple(Dev<Pol> d) {
     d.s_n[2 * i + 1] = ny / nn;
   }
   d.s_p[i] = S != R(0) ? fmax(P / S, cp.p_min) : R(0);
-  bool left = false;
-  for (int e = d.s_ref_off[i]; e < d.s_ref_off[i + 1]; ++e) {
-    const int pj = d.s_cxy[d.s_ref_j[e]];
-    left |= abs(cxy_x(pj) - cx) > 1 || abs(cxy_y(pj) - cy) > 1;
-  }
   if (left) atomicOr(d.status, kStRefListLeft);
 }
 
@@ -856,22 +890,33 @@ __device__ __forceinline__ R plastic_elastic(const CP &cp, R strain, R &max_stra
 template <class Pol>
 __global__ void __launch_bounds__(256) k_solid_deform(Dev<Pol> d, int sub) {
   using R = typename Pol::R;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.ns) return;
-  const int c = case_of(i, d.s_case_off, d.ncase);
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gid / kSolidLanes, l = gid % kSolidLanes;
+  const bool active = i < d.ns;
+  int c = 0;
+  R h00 = 0, h01 = 0, h10 = 0, h11 = 0;
+  if (active) {
+    c = case_of(i, d.s_case_off, d.ncase);
+    if (sub < d.cases[c].S) {
+      const double dix = d.sx[i] - d.sX[i], diy = d.sy[i] - d.sY[i];
+      for (int e = d.s_ref_off[i] + l; e < d.s_ref_off[i + 1]; e += kSolidLanes) {
+        const int j = d.s_ref_j[e];
+        const R gX = d.s_ref_g[2 * e], gY = d.s_ref_g[2 * e + 1];
+        const R ux = (R)((d.sx[j] - d.sX[j]) - dix), uy = (R)((d.sy[j] - d.sY[j]) - diy);
+        h00 += ux * gX;
+        h01 += ux * gY;
+        h10 += uy * gX;
+        h11 += uy * gY;
+      }
+    }
+  }
+  h00 = group_sum(h00);
+  h01 = group_sum(h01);
+  h10 = group_sum(h10);
+  h11 = group_sum(h11);
+  if (!active || l != 0) return;
   const auto &cp = d.cases[c];
   if (sub >= cp.S) return;
-  const double dix = d.sx[i] - d.sX[i], diy = d.sy[i] - d.sY[i];
-  R h00 = 0, h01 = 0, h10 = 0, h11 = 0;
-  for (int e = d.s_ref_off[i]; e < d.s_ref_off[i + 1]; ++e) {
-    const int j = d.s_ref_j[e];
-    const R gX = d.s_ref_g[2 * e], gY = d.s_ref_g[2 * e + 1];
-    const R ux = (R)((d.sx[j] - d.sX[j]) - dix), uy = (R)((d.sy[j] - d.sY[j]) - diy);
-    h00 += ux * gX;
-    h01 += ux * gY;
-    h10 += uy * gX;
-    h11 += uy * gY;
-  }
   const R *Lr = d.s_Lref + 4 * i;
   const Mat2<R> L = {Lr[0], Lr[1], Lr[2], Lr[3]};
   Mat2<R> H;
@@ -952,11 +997,79 @@ __global__ void __launch_bounds__(256) k_solid_deform(Dev<Pol> d, int sub) {
 template <class Pol>
 __global__ void __launch_bounds__(256) k_solid_force_update(Dev<Pol> d, int sub) {
   using R = typename Pol::R;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.ns) return;
-  const int c = case_of(i, d.s_case_off, d.ncase);
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gid / kSolidLanes, l = gid % kSolidLanes;
+  const bool active = i < d.ns;
+  int c = 0;
+  bool run = false;
+  R cpx = 0, cpy = 0, ax = 0, ay = 0;
+  if (active) {
+    c = case_of(i, d.s_case_off, d.ncase);
+    const auto &cp = d.cases[c];
+    run = sub < cp.S;
+    if (run) {
+      const int pk = d.s_cxy[i], cx = cxy_x(pk), cy = cxy_y(pk);
+      const double xi = d.sx[i], yi = d.sy[i], ui = d.su[i], vi = d.sv[i];
+      const int mat = d.s_mat[i];
+      const R rr_inv = mat == 1 ? cp.rho_aux_inv : cp.rho_s_inv;
+      const LoopConsts<R> K(cp);
+      // fluid -> solid Riemann force (rho_i = rho_j, u_i = u_s, p_i = p_s, c = c[phase_j])
+      const R xiv = view_x<Pol>(xi, cx, cp), yiv = view_y<Pol>(yi, cy, cp);
+      const R uiv = (R)ui, viv = (R)vi, psi = d.s_p[i];
+      const R c0 = cp.c[0], c1 = cp.c[1];
+      const auto &F = d.fa;
+      for (int a = -1; a <= 1; ++a) {
+        const int jx = cx + a;
+        if (jx < 0 || jx >= cp.nx) continue;
+        const R bx = xiv - cell_shift_x<Pol>(a, cp);
+        for (int b = -1; b <= 1; ++b) {
+          const int jy = cy + b;
+          if (jy < 0 || jy >= cp.ny) continue;
+          const R by = yiv - cell_shift_y<Pol>(b, cp);
+          const int2 rf = d.f_range[cp.cell_off + jx * cp.ny + jy];
+          for (int s = rf.x + l; s < rf.y; s += kSolidLanes) {
+            const R dx = F.vx[s] - bx, dy = F.vy[s] - by;
+            const R r2 = dx * dx + dy * dy;
+            if (!(r2 < K.rc2)) continue;
+            const R w = K.w(r2);
+            const R inv_r = r2 > R(0) ? krsqrt<R>(r2) : R(0);
+            const R rhoj = F.vrho[s];
+            R ps = R(0.5) * (F.p[s] + psi);
+            const R du = ((uiv - F.vu[s]) * dx + (viv - F.vv[s]) * dy) * inv_r;
+            if (du >= R(0)) ps += R(0.5) * rhoj * du * fmin(R(3) * du, d.f_phase[s] ? c1 : c0);
+            const R t = R(-2) * ps * rr_inv * w;
+            cpx += t * dx;
+            cpy += t * dy;
+          }
+        }
+      }
+      // solid -> solid over the reference-configuration list
+      const R *Fi = d.s_F + 4 * i, *Li = d.s_L + 4 * i, *PLi = d.s_PL + 4 * i;
+      const Mat2<R> FL = mmul(Mat2<R>{Fi[0], Fi[1], Fi[2], Fi[3]}, Mat2<R>{Li[0], Li[1], Li[2], Li[3]});
+      const R pl0 = PLi[0], pl1 = PLi[1], pl2 = PLi[2], pl3 = PLi[3];
+      const R h = cp.h, eps_h2 = cp.eps_h2, alpha = cp.alpha, beta = cp.beta, cs = cp.c_s;
+      for (int e = d.s_ref_off[i] + l; e < d.s_ref_off[i + 1]; e += kSolidLanes) {
+        const int j = d.s_ref_j[e];
+        const R gX = d.s_ref_g[2 * e], gY = d.s_ref_g[2 * e + 1];
+        const R *PLj = d.s_PL + 4 * j;
+        ax += ((PLj[0] + pl0) * gX + (PLj[1] + pl1) * gY) * rr_inv;
+        ay += ((PLj[2] + pl2) * gX + (PLj[3] + pl3) * gY) * rr_inv;
+        const R dx = (R)(d.sx[j] - xi), dy = (R)(d.sy[j] - yi);
+        const R du = (R)(d.su[j] - ui), dv = (R)(d.sv[j] - vi);
+        const R nu = h * (du * dx + dv * dy) / (dx * dx + dy * dy + eps_h2);
+        const R vis = alpha * nu * cs - beta * nu * nu;
+        ax += vis * (FL.a * gX + FL.b * gY);
+        ay += vis * (FL.c * gX + FL.d * gY);
+      }
+    }
+  }
+  cpx = group_sum(cpx);
+  cpy = group_sum(cpy);
+  ax = group_sum(ax);
+  ay = group_sum(ay);
+  if (!active || l != 0) return;
   const auto &cp = d.cases[c];
-  if (sub >= cp.S) {  // this case has fewer sub-steps than the batch: carry the state over
+  if (!run) {  // this case has fewer sub-steps than the batch: carry the state over
     d.sx2[i] = d.sx[i];
     d.sy2[i] = d.sy[i];
     d.su2[i] = d.su[i];
@@ -969,56 +1082,6 @@ __global__ void __launch_bounds__(256) k_solid_force_update(Dev<Pol> d, int sub)
   }
   const int pk = d.s_cxy[i], cx = cxy_x(pk), cy = cxy_y(pk);
   const double xi = d.sx[i], yi = d.sy[i], ui = d.su[i], vi = d.sv[i];
-  const int mat = d.s_mat[i];
-  const R rr_inv = mat == 1 ? cp.rho_aux_inv : cp.rho_s_inv;
-  // fluid -> solid Riemann force (rho_i = rho_j, u_i = u_s, p_i = p_s, c = c[phase_j])
-  const R xiv = view_x<Pol>(xi, cx, cp), yiv = view_y<Pol>(yi, cy, cp);
-  const R uiv = (R)ui, viv = (R)vi, psi = d.s_p[i];
-  const auto &F = d.fa;
-  R cpx = 0, cpy = 0;
-  for (int a = -1; a <= 1; ++a) {
-    const int jx = cx + a;
-    if (jx < 0 || jx >= cp.nx) continue;
-    const R bx = xiv - cell_shift_x<Pol>(a, cp);
-    for (int b = -1; b <= 1; ++b) {
-      const int jy = cy + b;
-      if (jy < 0 || jy >= cp.ny) continue;
-      const R by = yiv - cell_shift_y<Pol>(b, cp);
-      const int2 rf = d.f_range[cp.cell_off + jx * cp.ny + jy];
-      for (int s = rf.x; s < rf.y; ++s) {
-        const R dx = F.vx[s] - bx, dy = F.vy[s] - by;
-        const R r2 = dx * dx + dy * dy;
-        if (!(r2 < cp.rc2)) continue;
-        const R w = kexp<R>(-(r2 * cp.hinv2)) * cp.kdc;
-        const R inv_r = r2 > R(0) ? krsqrt<R>(r2) : R(0);
-        const R rhoj = F.vrho[s];
-        R ps = R(0.5) * (F.p[s] + psi);
-        const R du = ((uiv - F.vu[s]) * dx + (viv - F.vv[s]) * dy) * inv_r;
-        if (du >= R(0)) ps += R(0.5) * rhoj * du * fmin(R(3) * du, cp.c[d.f_phase[s]]);
-        const R t = R(-2) * ps * rr_inv * w;
-        cpx += t * dx;
-        cpy += t * dy;
-      }
-    }
-  }
-  // solid -> solid over the reference-configuration list
-  const R *Fi = d.s_F + 4 * i, *Li = d.s_L + 4 * i, *PLi = d.s_PL + 4 * i;
-  const Mat2<R> FL = mmul(Mat2<R>{Fi[0], Fi[1], Fi[2], Fi[3]}, Mat2<R>{Li[0], Li[1], Li[2], Li[3]});
-  const R pl0 = PLi[0], pl1 = PLi[1], pl2 = PLi[2], pl3 = PLi[3];
-  R ax = 0, ay = 0;
-  for (int e = d.s_ref_off[i]; e < d.s_ref_off[i + 1]; ++e) {
-    const int j = d.s_ref_j[e];
-    const R gX = d.s_ref_g[2 * e], gY = d.s_ref_g[2 * e + 1];
-    const R *PLj = d.s_PL + 4 * j;
-    ax += ((PLj[0] + pl0) * gX + (PLj[1] + pl1) * gY) * rr_inv;
-    ay += ((PLj[2] + pl2) * gX + (PLj[3] + pl3) * gY) * rr_inv;
-    const R dx = (R)(d.sx[j] - xi), dy = (R)(d.sy[j] - yi);
-    const R du = (R)(d.su[j] - ui), dv = (R)(d.sv[j] - vi);
-    const R nu = cp.h * (du * dx + dv * dy) / (dx * dx + dy * dy + cp.eps_h2);
-    const R vis = cp.alpha * nu * cp.c_s - cp.beta * nu * nu;
-    ax += vis * (FL.a * gX + FL.b * gY);
-    ay += vis * (FL.c * gX + FL.d * gY);
-  }
   d.s_cp[2 * i] = cpx;
   d.s_cp[2 * i + 1] = cpy;
   double tax = (double)ax + (double)cpx, tay = (double)ay + (double)cpy;

@@ -98,6 +98,8 @@ struct EngineBase {
   virtual int reflist(int c, int *off, int *list, int64_t cap) = 0;
   virtual int energy(int c, double out[6]) = 0;
   virtual int stage_time(int st, int *launches, double *ms) = 0;
+  virtual int autotune() = 0;
+  double tune_ms[2][3] = {{0, 0, 0}, {0, 0, 0}};  // [pair, shift][64, 128, 256]: ms per launch
   virtual void drop_graphs() = 0;
 };
 
@@ -529,6 +531,97 @@ struct Engine : EngineBase {
     return SPHNJ_OK;
   }
 
+  template <int B>
+  void pair_at() {
+    if (viscid)
+      k_fluid_pair_update<Pol, true, B><<<nblk(d.nf, B), B, 0, stream>>>(d);
+    else
+      k_fluid_pair_update<Pol, false, B><<<nblk(d.nf, B), B, 0, stream>>>(d);
+  }
+  int launch_pair(int blk) {
+    if (blk == 64)
+      pair_at<64>();
+    else if (blk == 128)
+      pair_at<128>();
+    else
+      pair_at<256>();
+    LAUNCH_CHECK();
+    return SPHNJ_OK;
+  }
+  int launch_shift(int blk, int pass) {
+    if (blk == 64)
+      k_fluid_shift<Pol, 64><<<nblk(d.nf, 64), 64, 0, stream>>>(d, pass);
+    else if (blk == 128)
+      k_fluid_shift<Pol, 128><<<nblk(d.nf, 128), 128, 0, stream>>>(d, pass);
+    else
+      k_fluid_shift<Pol, 256><<<nblk(d.nf, 256), 256, 0, stream>>>(d, pass);
+    LAUNCH_CHECK();
+    return SPHNJ_OK;
+  }
+
+  // Autotuner: times the pair and shift kernels at each candidate block size (= tile size in
+  // slots) on the current state with HIP events and keeps the fastest.  The pair kernel only
+  // writes the alternate buffers and the per-step transients; the shift kernel also updates
+  // u, v and the fluid class in place, which are saved before and restored after.
+  int autotune() override {
+    if (d.nf == 0) return SPHNJ_OK;
+    const int nf = d.nf;
+    double *su = nullptr, *sv = nullptr;
+    int8_t *scls = nullptr;
+    int rc;
+    if ((rc = dalloc(&su, nf)) || (rc = dalloc(&sv, nf)) || (rc = dalloc(&scls, nf))) return rc;
+    HIPC(hipMemcpyAsync(su, d.fa.u, nf * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    HIPC(hipMemcpyAsync(sv, d.fa.v, nf * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    HIPC(hipMemcpyAsync(scls, d.f_cls, nf, hipMemcpyDeviceToDevice, stream));
+    hipEvent_t a = take_event(), b = take_event();
+    if (!a || !b) {
+      err = "hipEventCreate";
+      return SPHNJ_E_HIP;
+    }
+    const int cand[3] = {64, 128, 256};
+    for (int fam = 0; fam < 2; ++fam) {
+      double best = 1e300;
+      int best_b = 256;
+      for (int bs : cand) {
+        for (int w = 0; w < 2; ++w) {
+          rc = fam == 0 ? launch_pair(bs) : launch_shift(bs, 0);
+          if (rc) return rc;
+        }
+        HIPC(hipEventRecord(a, stream));
+        const int reps = 5;
+        for (int r = 0; r < reps; ++r) {
+          rc = fam == 0 ? launch_pair(bs) : launch_shift(bs, 0);
+          if (rc) return rc;
+        }
+        HIPC(hipEventRecord(b, stream));
+        HIPC(hipEventSynchronize(b));
+        float ms = 0.f;
+        HIPC(hipEventElapsedTime(&ms, a, b));
+        tune_ms[fam][bs == 64 ? 0 : bs == 128 ? 1 : 2] = ms / reps;
+        if (ms < best) {
+          best = ms;
+          best_b = bs;
+        }
+      }
+      if (fam == 0)
+        blk_pair = best_b;
+      else
+        blk_shift = best_b;
+    }
+    ev_pool.push_back(a);
+    ev_pool.push_back(b);
+    HIPC(hipMemcpyAsync(d.fa.u, su, nf * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    HIPC(hipMemcpyAsync(d.fa.v, sv, nf * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    HIPC(hipMemcpyAsync(d.f_cls, scls, nf, hipMemcpyDeviceToDevice, stream));
+    HIPC(hipStreamSynchronize(stream));
+    for (void *q : {(void *)su, (void *)sv, (void *)scls}) {
+      (void)hipFree(q);
+      allocs.erase(std::remove(allocs.begin(), allocs.end(), q), allocs.end());
+    }
+    drop_graphs();
+    return SPHNJ_OK;
+  }
+
   int stage(int st) override {
     const int nf = d.nf, ns = d.ns, nw = d.nw;
     switch (st) {
@@ -536,7 +629,7 @@ struct Engine : EngineBase {
         return stage_cells();
       case SPHNJ_STAGE_COUPLE:
         if (nf > 0 && ns > 0) {
-          k_solid_couple<Pol><<<nblk(ns, blk_stream), blk_stream, 0, stream>>>(d);
+          k_solid_couple<Pol><<<nblk(ns * kSolidLanes, 256), 256, 0, stream>>>(d);
           LAUNCH_CHECK();
         }
         return SPHNJ_OK;
@@ -548,19 +641,16 @@ struct Engine : EngineBase {
         return SPHNJ_OK;
       case SPHNJ_STAGE_FLUID:
         if (nf > 0) {
-          if (viscid)
-            k_fluid_pair_update<Pol, true><<<nblk(nf, blk_pair), blk_pair, 0, stream>>>(d);
-          else
-            k_fluid_pair_update<Pol, false><<<nblk(nf, blk_pair), blk_pair, 0, stream>>>(d);
-          LAUNCH_CHECK();
+          const int rc = launch_pair(blk_pair);
+          if (rc) return rc;
           swap_fluid_all();
         }
         return SPHNJ_OK;
       case SPHNJ_STAGE_SOLID:
         if (ns > 0) {
           for (int k = 0; k < S; ++k) {
-            k_solid_deform<Pol><<<nblk(ns, blk_stream), blk_stream, 0, stream>>>(d, k);
-            k_solid_force_update<Pol><<<nblk(ns, blk_stream), blk_stream, 0, stream>>>(d, k);
+            k_solid_deform<Pol><<<nblk(ns * kSolidLanes, 256), 256, 0, stream>>>(d, k);
+            k_solid_force_update<Pol><<<nblk(ns * kSolidLanes, 256), 256, 0, stream>>>(d, k);
             LAUNCH_CHECK();
             std::swap(d.sx, d.sx2);
             std::swap(d.sy, d.sy2);
@@ -574,8 +664,8 @@ struct Engine : EngineBase {
       case SPHNJ_STAGE_SHIFT:
         if (nf > 0) {
           for (int k = 0; k < K; ++k) {
-            k_fluid_shift<Pol><<<nblk(nf, blk_shift), blk_shift, 0, stream>>>(d, k);
-            LAUNCH_CHECK();
+            const int rc = launch_shift(blk_shift, k);
+            if (rc) return rc;
             std::swap(d.fa.x, d.fb.x);
             std::swap(d.fa.y, d.fb.y);
             std::swap(d.fa.vx, d.fb.vx);
@@ -1256,6 +1346,12 @@ int sphnj_stage_time(sphnj_ctx *c, int32_t stage, int32_t *launches, double *tot
 int sphnj_autotune(sphnj_ctx *c) {
   if (!c) return SPHNJ_E_ARG;
   if (!c->inited) return c->fail(SPHNJ_E_STATE, "autotune before init");
+  return c->from_engine(c->eng->autotune());
+}
+
+int sphnj_autotune_report(const sphnj_ctx *c, int32_t family, double ms_per_launch[3]) {
+  if (!c || !c->eng || !ms_per_launch || family < 0 || family > 1) return SPHNJ_E_ARG;
+  for (int k = 0; k < 3; ++k) ms_per_launch[k] = c->eng->tune_ms[family][k];
   return SPHNJ_OK;
 }
 

@@ -285,6 +285,19 @@ class Simulation(ContextBase):
     def set_block(self, family: int, block: int) -> None:
         self._call("set_block", family, block)
 
+    def autotune(self) -> dict:
+        """Pick the block sizes of the pair and shift kernels; returns ms per launch measured
+        for block sizes 64, 128 and 256, and the choice."""
+        self._call("autotune")
+        out = {}
+        for fam, name in ((0, "pair"), (1, "shift")):
+            ms = np.zeros(3)
+            self._call("autotune_report", fam, _dptr(ms))
+            blk = C.c_int32()
+            self._call("get_block", fam, C.byref(blk))
+            out[name] = {"ms": dict(zip((64, 128, 256), ms.tolist())), "block": blk.value}
+        return out
+
     def set_stage_timing(self, on: bool) -> None:
         self._call("set_stage_timing", int(on))
 

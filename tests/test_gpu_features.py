@@ -1,0 +1,140 @@
+"""GPU tests of execution options and the remaining outputs, through the C ABI:
+block sizes / autotuner (results independent of the tiling), status flags forced from the
+host, FP64 reductions, solid materials against the oracle, and M32 neighbour lists."""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+from helpers import case, fluid, lattice, params, solid, wall
+from sphnj_amd import Case, Simulation, _abi as A
+
+pytestmark = pytest.mark.gpu
+
+
+def gate():
+    return Case.generate("gate_fsi", dx=2.0e-3, tank_w=0.3, tank_h=0.2, S=4)
+
+
+def two_phase_small():
+    return Case.generate("dam_break", tank_w=1.0, tank_h=0.8, col_w=0.4, col_h=0.6, dx=0.01, air=1, K=3)
+
+
+@pytest.mark.parametrize("precision", ["p64", "m32"])
+def test_block_size_does_not_change_results(gpu, precision):
+    """Every block size stages different tiles but performs the same per-particle sums."""
+    c = two_phase_small()
+    ref = Simulation(c, precision)
+    ref.step(4)
+    a = ref.download()
+    for blk in (64, 128):
+        s = Simulation(c, precision)
+        s.set_block(0, blk)
+        s.set_block(1, blk)
+        s.step(4)
+        b = s.download()
+        for k in ("x", "y", "u", "v", "rho", "p", "boundary_type"):
+            assert np.array_equal(a["fluid"][k], b["fluid"][k]), (blk, k)
+
+
+def test_autotune_keeps_state(gpu):
+    c = two_phase_small()
+    s = Simulation(c, "m32")
+    t = Simulation(c, "m32")
+    s.step(3)
+    t.step(3)
+    before = s.download()["fluid"]
+    rep = s.autotune()
+    assert rep["pair"]["block"] in (64, 128, 256) and rep["shift"]["block"] in (64, 128, 256)
+    assert all(v > 0 for v in rep["pair"]["ms"].values())
+    after = s.download()["fluid"]
+    for k in ("x", "y", "u", "v", "rho", "p", "boundary_type"):
+        assert np.array_equal(before[k], after[k]), k
+    s.step(3)
+    t.step(3)
+    a, b = s.download()["fluid"], t.download()["fluid"]
+    for k in ("x", "y", "u", "v", "rho", "p"):
+        assert np.array_equal(a[k], b[k]), k
+
+
+def test_status_flags_from_host(gpu, oracle_mod):
+    p = params()
+    x, y = lattice(9, 9, 0.008, 0.127, 0.127)          # 81 particles in one cell
+    f = fluid(np.append(x, 5.0), np.append(y, 0.5))    # and one outside the cell box
+    c = case(p, f)
+    s = Simulation(c, "p64")
+    s.step(1)
+    o = oracle_mod.Oracle(c)
+    o.step(1)
+    st = s.status()
+    assert st & A.ST_OUT_OF_GRID and st & A.ST_CELL_OVER_64
+    assert st == o.status()
+
+
+def test_energy_matches_oracle(gpu, oracle_mod):
+    c = gate()
+    s = Simulation(c, "p64")
+    o = oracle_mod.Oracle(c)
+    s.step(5)
+    o.step(5)
+    e1, e2 = s.energy(), o.energy()
+    assert e1[5] == e2[5]
+    for k in range(5):
+        assert e1[k] == pytest.approx(e2[k], rel=1e-9, abs=1e-300), k
+
+
+@pytest.mark.parametrize("material,kw", [
+    (A.AUXETIC, dict(yield_strain=0.02, yield_stress=2.0e5, harden_strain=0.3, harden_coef=1.0e8, is_plastic=1,
+                     auxetic_density=600.0)),
+    (A.NEO_HOOKEAN, dict()),
+    (A.LINEAR_ELASTIC, dict(is_2D_strain=1)),
+])
+@pytest.mark.parametrize("precision,tol", [("p64", 1e-9), ("m32", 2e-4)])
+def test_solid_materials_match_oracle(gpu, oracle_mod, material, kw, precision, tol):
+    """A compressed, sheared block with a fixed top row, under gravity, several steps."""
+    X, Y = lattice(8, 12, 0.02, 0.4, 0.1)
+    Am = np.array([[0.985, 0.02], [0.0, 0.99]])
+    x = Am[0, 0] * (X - 0.48) + Am[0, 1] * (Y - 0.2) + 0.48
+    y = Am[1, 0] * (X - 0.48) + Am[1, 1] * (Y - 0.2) + 0.2
+    bc = (Y > Y.max() - 0.01).astype(np.int32)
+    p = params(youngs_modulus=1.0e6, poisson_ratio=0.3, solid_density=1000.0, solid_sub_step=4, dt=2e-5,
+               **kw)
+    c = case(p, s=solid(x, y, X, Y, material=material, bc=bc))
+    s = Simulation(c, precision)
+    o = oracle_mod.Oracle(c)
+    s.step(10)
+    o.step(10)
+    a, b = s.download()["solid"], o.download()["solid"]
+    for k in ("x", "y", "u", "v", "stress", "deformation"):
+        assert rel_l2(a[k], b[k]) <= tol, (k, rel_l2(a[k], b[k]))
+    if material == A.AUXETIC:
+        assert rel_l2(a["max_strain"], b["max_strain"]) <= tol
+
+
+def test_m32_neighbor_lists_match_up_to_the_cutoff_band(gpu, oracle_mod):
+    """M32 tests r^2 < rc^2 in float on cell-relative coordinates: lists equal the oracle's
+    except for pairs within float rounding of the cut-off."""
+    c = gate()
+    s = Simulation(c, "m32")
+    o = oracle_mod.Oracle(c)
+    s.particle_cell_update()
+    o.particle_cell_update()
+    c1, l1 = s.neighbors()
+    c2, l2 = o.neighbors()
+    assert np.abs(c1 - c2).max() <= 1
+    assert (c1 != c2).mean() < 1e-3
+    same = np.repeat(c1 == c2, c1)
+    same2 = np.repeat(c1 == c2, c2)
+    assert np.array_equal(l1[same], l2[same2])
+
+
+def test_two_phase_with_walls_and_gas_matches_oracle(gpu, oracle_mod):
+    c = two_phase_small()
+    s = Simulation(c, "p64")
+    o = oracle_mod.Oracle(c)
+    s.step(30)
+    o.step(30)
+    a, b = s.download(), o.download()
+    for k in ("x", "y", "u", "v", "rho", "p"):
+        assert rel_l2(a["fluid"][k], b["fluid"][k]) <= 1e-9, k
+    assert np.array_equal(a["wall"]["phase"], b["wall"]["phase"])
+    assert np.array_equal(a["fluid"]["boundary_type"], b["fluid"]["boundary_type"])

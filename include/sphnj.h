@@ -250,9 +250,13 @@ int sphnj_stage(sphnj_ctx *ctx, int32_t stage);
 
 /* Species selector for the inspection calls. */
 enum { SPHNJ_FLUID = 0, SPHNJ_SOLID = 1, SPHNJ_WALL = 2 };
-/* Result of the last CELLS stage for one species of one case: sorted global cell keys and
- * the original index of each sorted slot (n entries each), and the cell start table
- * (n_cells + 1 entries, local to the case: start[c]..start[c+1] is cell c's slot range). */
+/* Result of the last CELLS stage for one species of one case: sorted keys and the original
+ * index of each sorted slot (n entries each), and a start table local to the case
+ * (start[b]..start[b+1] is bin b's slot range).  Solids and walls are binned by cell: keys
+ * are global cell keys (cy + cx ny + the case's cell offset), n_cells + 1 starts.  Fluids are
+ * binned by half-cell sub-cell (sx = 2 cx + hx, sy = 2 cy + hy, hx and hy the halves of the
+ * cell): keys are sy + sx 2 ny + 4 x the case's cell offset, 4 n_cells + 1 starts; a cell's
+ * fluids are its sub-cells (2cx, 2cy), (2cx, 2cy+1), (2cx+1, 2cy), (2cx+1, 2cy+1). */
 int sphnj_stage_sorted(sphnj_ctx *ctx, int32_t case_index, int32_t species, int32_t *keys,
                        int32_t *orig, int32_t *cell_start);
 /* In-range neighbours of every fluid particle of one case, in the library's iteration

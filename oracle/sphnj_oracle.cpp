@@ -6,8 +6,9 @@
 // never calls it.
 //
 // Conventions fixed here and followed by libsphnj (documented choices, not reference facts):
-//  * Every step hashes fluid and solid particles and orders each species by (cell, original
-//    index).  The reference rebuilds its per-cell lists every step from the previous lists
+//  * Every step hashes fluid and solid particles and orders fluids by (cell, quadrant of the
+//    cell, original index) and solids by (cell, original index); walls likewise, once.  The
+//    reference rebuilds its per-cell lists every step from the previous lists
 //    (src/pair.cu:66-181) and sorts every sort_every_step steps (src/sort.cu:139-150); the set
 //    of particles listed in a cell is the same, only the order inside a cell differs.
 //  * Neighbour iteration: the 9 cells in the order of Sim::createCell (src/sim.cu:175-193,
@@ -135,22 +136,33 @@ bool make_consts(const sphnj_params &p, Consts *k) {
 }
 
 // One axis of findPositionCell (src/pair.cu:8-18): truncation toward zero; outside [0, n)
-// is clamped and reported.
-inline int axis_cell(double x, double x0, double inv, int n, unsigned *flags) {
+// is clamped and reported.  *half is the half of the cell the particle lies in (0 lower, 1
+// upper; clamped particles take the side they left by).
+inline int axis_cell(double x, double x0, double inv, int n, unsigned *flags, int *half = nullptr) {
   const double t = (x - x0) * inv;
   if (!(t > -1.0)) {
     *flags |= SPHNJ_ST_OUT_OF_GRID;
+    if (half) *half = 0;
     return 0;
   }
   if (!(t < (double)n)) {
     *flags |= SPHNJ_ST_OUT_OF_GRID;
+    if (half) *half = 1;
     return n - 1;
   }
-  return (int)t;
+  const int ix = (int)t;
+  if (half) {
+    const int h2 = (int)(2.0 * t) - 2 * ix;
+    *half = h2 < 0 ? 0 : (h2 > 1 ? 1 : h2);
+  }
+  return ix;
 }
 
-struct CellIndex {  // one species: per-particle cell, (cell, orig)-ordered list, cell starts
-  std::vector<int> cx, cy, key, sorted, start;
+struct CellIndex {  // one species: per-particle cell, sorted list, cell starts
+  std::vector<int> cx, cy, key, sub, sorted, start;
+  // fluids: slot order of the library, x-major over the half-cell grid, with sub-cell keys
+  // and starts (4 ncell + 1)
+  std::vector<int> xkey, xsorted, xstart;
 };
 
 struct Case {
@@ -178,29 +190,48 @@ struct Ctx {
   bool inited = false;
 };
 
+// Stable counting sort of 0..n-1 by key (0 <= key < nb): order and bin starts (nb + 1).
+void counting_sort(const std::vector<int> &key, int nb, std::vector<int> *order, std::vector<int> *start) {
+  const int n = (int)key.size();
+  start->assign(nb + 1, 0);
+  for (int i = 0; i < n; ++i) (*start)[key[i] + 1]++;
+  for (int q = 0; q < nb; ++q) (*start)[q + 1] += (*start)[q];
+  std::vector<int> fill(start->begin(), start->end() - 1);
+  order->assign(n, 0);
+  for (int i = 0; i < n; ++i) (*order)[fill[key[i]]++] = i;
+}
+
+// Hash one species and order it stably over original order: by (cell, original index) for
+// solids and walls; by (cell, quadrant, original index) for fluids, the quadrant being
+// 2 hx + hy from the halves of the cell the particle lies in.  For fluids it also forms the
+// library's slot order, x-major over the half-cell grid (sub-cell (2cx + hx, 2cy + hy)), in
+// which a cell's fluids appear in the same (quadrant, original index) order.
 void hash_species(const Case &c, const std::vector<double> &x, const std::vector<double> &y, CellIndex *ci,
-                  unsigned *flags) {
+                  unsigned *flags, bool quadrants = false) {
   const Consts &k = c.k;
   const int n = (int)x.size();
   ci->cx.assign(n, 0);
   ci->cy.assign(n, 0);
   ci->key.assign(n, 0);
+  ci->sub.assign(n, 0);
+  ci->xkey.assign(quadrants ? n : 0, 0);
   unsigned fl = 0;
   for (int i = 0; i < n; ++i) {
-    const int ix = axis_cell(x[i], k.cxmin, k.inv_cdx, k.nx, &fl);
-    const int iy = axis_cell(y[i], k.cymin, k.inv_cdy, k.ny, &fl);
+    int hx = 0, hy = 0;
+    const int ix = axis_cell(x[i], k.cxmin, k.inv_cdx, k.nx, &fl, &hx);
+    const int iy = axis_cell(y[i], k.cymin, k.inv_cdy, k.ny, &fl, &hy);
     ci->cx[i] = ix;
     ci->cy[i] = iy;
     ci->key[i] = iy + ix * k.ny;  // src/pair.cu:17
+    ci->sub[i] = quadrants ? ci->key[i] * 4 + hx * 2 + hy : ci->key[i];
+    if (quadrants) ci->xkey[i] = (2 * iy + hy) + (2 * ix + hx) * (2 * k.ny);
   }
   *flags |= fl;
-  // stable counting sort by cell over original order: order = (cell, original index)
+  std::vector<int> st;
+  counting_sort(ci->sub, quadrants ? 4 * c.ncell : c.ncell, &ci->sorted, &st);
   ci->start.assign(c.ncell + 1, 0);
-  for (int i = 0; i < n; ++i) ci->start[ci->key[i] + 1]++;
-  for (int q = 0; q < c.ncell; ++q) ci->start[q + 1] += ci->start[q];
-  std::vector<int> fill(ci->start.begin(), ci->start.end() - 1);
-  ci->sorted.assign(n, 0);
-  for (int i = 0; i < n; ++i) ci->sorted[fill[ci->key[i]]++] = i;
+  for (int q = 0; q <= c.ncell; ++q) ci->start[q] = st[quadrants ? 4 * q : q];
+  if (quadrants) counting_sort(ci->xkey, 4 * c.ncell, &ci->xsorted, &ci->xstart);
 }
 
 inline double grad_w(const Consts &k, double r2) { return std::exp(-(r2 * k.hinv2)) * k.kdc; }
@@ -792,7 +823,7 @@ void stage(Ctx &C, int st) {
   for (auto &c : C.cases) {
     switch (st) {
       case SPHNJ_STAGE_CELLS:
-        hash_species(c, c.fx, c.fy, &c.cf, &C.flags);
+        hash_species(c, c.fx, c.fy, &c.cf, &C.flags, true);
         hash_species(c, c.sx, c.sy, &c.cs, &C.flags);
         {
           // the reference's per-cell list holds at most CELL_SIZE = 64 entries
@@ -939,7 +970,7 @@ int oracle_init(oracle_ctx *o) {
     wall_normals(c);
     build_reflists(c, &o->C.flags);
     // empty fluid / solid indices until the first CELLS stage
-    hash_species(c, c.fx, c.fy, &c.cf, &o->C.flags);
+    hash_species(c, c.fx, c.fy, &c.cf, &o->C.flags, true);
     hash_species(c, c.sx, c.sy, &c.cs, &o->C.flags);
   }
   o->C.inited = true;
@@ -998,6 +1029,15 @@ int oracle_stage_sorted(oracle_ctx *o, int32_t ci, int32_t species, int32_t *key
   const Case &c = o->C.cases[ci];
   const CellIndex *x = species == SPHNJ_FLUID ? &c.cf : species == SPHNJ_SOLID ? &c.cs : &c.cw;
   const int n = (int)x->sorted.size();
+  if (species == SPHNJ_FLUID) {  // the library's slot order: sub-cells, x-major
+    for (int s = 0; s < n; ++s) {
+      if (keys) keys[s] = 4 * c.cell_off + x->xkey[x->xsorted[s]];
+      if (orig) orig[s] = x->xsorted[s];
+    }
+    if (cell_start)
+      for (int q = 0; q <= 4 * c.ncell; ++q) cell_start[q] = x->xstart[q];
+    return SPHNJ_OK;
+  }
   for (int s = 0; s < n; ++s) {
     if (keys) keys[s] = c.cell_off + x->key[x->sorted[s]];
     if (orig) orig[s] = x->sorted[s];

@@ -25,6 +25,8 @@ struct CaseDev {
   // kernel (Kernel::initiate, src/kernel.cpp:25-41)
   R rc2, hinv2, kdc, k2c, eR, h, eps_h2;  // eps_h2 = 0.01 h^2
   R nhl2;  // -hinv2 * log2(e): exp(-r^2/h^2) = 2^(r^2 nhl2)
+  // scaled units of the M32 shift loop: x' = x sqrt(hinv2 log2 e), so exp(-r^2/h^2) = 2^-r'^2
+  float gscale, rc2s, kdc_s;  // scale, rc^2 scaled, kd c32 / scale
   R quarter_rc2;
   // fluid per phase: pair-side constants in R
   R c[2], cinv[2], mu8[2], rho0[2];
@@ -39,8 +41,12 @@ struct CaseDev {
   // domain / cell grid (Domain::initiate, src/domain.cpp:37-48)
   double cxmin, cymin, cdx, cdy, inv_cdx, inv_cdy;
   R cdx_r, cdy_r;
+  // half-cell ("sub-cell") grid of the fluid order: sizes and the pruning cut-off rc (1 + 1e-5)
+  double rcut_prune;
+  float ssx, ssy, inv_ssx, inv_ssy;
   double dxmin, dymin, dxmax, dymax;
   int nx, ny, cell_off, ncell;
+  int col_off, lb_off;  // first global sub-column; offset in the fluid row table f_lb
   int steady, viscid;
   int K, S;  // shift passes and solid sub-steps of this case
   // solid (SolidProperties, src/properties.cpp:69-89,118-161)
@@ -64,6 +70,7 @@ struct FluidBuf {
   double *x, *y, *u, *v, *rho;  // masters
   R *p;
   R *vx, *vy, *vu, *vv, *vrho;  // views
+  float *cum;  // path length since the hash (sum of step lengths): bounds |x - x at hash|
 };
 
 template <class Pol>
@@ -78,14 +85,19 @@ struct Dev {
   FluidBuf<Pol> fa, fb;  // cur (a) / alt (b)
   int8_t *f_phase, *f_phase2, *f_cls, *f_cls2;  // cur / alt (swap in the permute only)
   int *f_orig;           // slot -> global original index (sorted values)
-  int *f_key;            // slot -> global cell key (sorted keys)
-  int *f_cxy;            // slot -> (cx << 16) | cy
+  int *f_key;            // slot -> global sub-cell key (sorted keys)
+  int *f_cxy;            // slot -> (sx << 16) | sy, sub-cell coordinates (cx = sx >> 1)
   int *f_key_by_orig, *f_slot_of_orig;
   R *f_col;
   int8_t *f_near;
   R *f_L;  // 4 per slot
-  int2 *f_range;  // per global cell: [start, end) slot range
-  unsigned long long *umax2;  // per case: bit pattern of max |u|^2 (double, >= 0)
+  int2 *f_srange;  // per global sub-cell (4 per cell): [start, end) slot range, empty (0, 0)
+  // per sub-column, 2 ny + 1 entries: first slot of sub-row k or later; the sub-column's end
+  int *f_lb;
+  // max |u|^2 per case as the bit pattern of a non-negative double: blocks reduce into
+  // kSpread slots per case (slot s at umax_slot[(c kSpread + s) kSlotStride], one cache line
+  // each, so no address takes more than a few hundred atomics); umax2[c] is their maximum
+  unsigned long long *umax_slot, *umax2;
   // solid (original order, never permuted)
   double *sx, *sy, *su, *sv;     // cur
   double *sx2, *sy2, *su2, *sv2; // alt (sub-step ping-pong)
@@ -127,6 +139,8 @@ __device__ __forceinline__ int case_of(int i, const int *off, int ncase) {
   return lo;
 }
 
+constexpr int kSpread = 64, kSlotStride = 16;
+
 // One axis of findPositionCell (src/pair.cu:8-18): truncation toward zero, clamp outside [0,n).
 __device__ __forceinline__ int axis_cell(double x, double x0, double inv, int n, unsigned *fl) {
   const double t = (x - x0) * inv;
@@ -139,6 +153,23 @@ __device__ __forceinline__ int axis_cell(double x, double x0, double inv, int n,
     return n - 1;
   }
   return (int)t;
+}
+
+// Sub-cell index along one axis (0 .. 2n-1): 2 * cell + the half of the cell the coordinate
+// lies in; a clamped coordinate takes the half on the side it left by.
+__device__ __forceinline__ int axis_sub(double x, double x0, double inv, int n, unsigned *fl) {
+  const double t = (x - x0) * inv;
+  if (!(t > -1.0)) {
+    *fl |= 1u;
+    return 0;
+  }
+  if (!(t < (double)n)) {
+    *fl |= 1u;
+    return 2 * n - 1;
+  }
+  const int c = (int)t;
+  const int h = min(max((int)(2.0 * t) - 2 * c, 0), 1);
+  return 2 * c + h;
 }
 
 template <class R>
@@ -180,6 +211,11 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
     const unsigned long long w = __shfl_xor(v, o, 64);
     v = w > v ? w : v;
   }
+  return v;
+}
+
+__device__ __forceinline__ float wave_max_f(float v) {
+  for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
   return v;
 }
 

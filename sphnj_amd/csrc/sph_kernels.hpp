@@ -3,10 +3,13 @@
 // particle is found in the per-species case offset table) and templated on the precision
 // policy (P64 / M32), so there is one code path per kernel.
 //
-// Neighbour order everywhere: the 9 cells of Sim::createCell (src/sim.cu:175-193) -- outer
-// loop over the x offset, inner over the y offset -- and inside a cell fluid, then solid,
-// then wall particles (the layout of Cell::particle_list, src/sort.cu:59-92), each in slot
-// (= (cell, original index)) order.
+// Neighbour order of the wall, solid and inspection kernels: the 9 cells of Sim::createCell
+// (src/sim.cu:175-193) -- outer loop over the x offset, inner over the y offset -- and inside
+// a cell fluid (quadrant, original index), then solid, then wall particles (the layout of
+// Cell::particle_list, src/sort.cu:59-92).  The fluid pair and shift kernels visit the same
+// candidate set (the 3x3 cells of the particle's hashed cell) by half-cell sub-columns and
+// skip sub-cells that provably lie beyond the cut-off; their sums differ from the cell order
+// only in rounding.
 #pragma once
 
 #include "device.hpp"
@@ -52,6 +55,42 @@ __device__ __forceinline__ typename Pol::R cell_shift_y(int b, const CP &cp) {
     return 0.0;
 }
 
+// ---- the half-cell grid of the fluid order ----------------------------------------------
+// Fluid slots are ordered on the half-cell ("sub-cell") grid, x-major: (case, sub-column
+// sx, sub-row sy, original index), with sx = 2 cx + hx and sy = 2 cy + hy.  A run of
+// consecutive sub-rows of one sub-column is one contiguous slot range.  A cell's fluids are
+// two such runs, sub-column 2cx rows 2cy..2cy+1 then sub-column 2cx+1, which together list
+// them in (quadrant, original index) order.
+
+__device__ __forceinline__ int2 run_join(int2 a, int2 b) {
+  if (a.y <= a.x) return b;
+  if (b.y <= b.x) return a;
+  return make_int2(a.x, b.y);
+}
+
+template <class Pol, class CP>
+__device__ __forceinline__ void fluid_cell_runs(const Dev<Pol> &d, const CP &cp, int jx, int jy, int2 &r0,
+                                                int2 &r1) {
+  const int sny = 2 * cp.ny;
+  const int k = 4 * cp.cell_off + 2 * jx * sny + 2 * jy;
+  r0 = run_join(d.f_srange[k], d.f_srange[k + 1]);
+  r1 = run_join(d.f_srange[k + sny], d.f_srange[k + sny + 1]);
+}
+
+// Case owning global cell i (cases hold consecutive cell ranges).
+template <class Pol>
+__device__ __forceinline__ int case_of_cell(int i, const Dev<Pol> &d) {
+  int lo = 0, hi = d.ncase;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (d.cases[mid].cell_off <= i)
+      lo = mid;
+    else
+      hi = mid;
+  }
+  return lo;
+}
+
 // ---- small 2x2 helpers ------------------------------------------------------------------
 template <class R>
 struct Mat2 {
@@ -66,8 +105,10 @@ __device__ __forceinline__ Mat2<R> mmul(const Mat2<R> &x, const Mat2<R> &y) {
 // CELLS stage: hash, key sort (radix_sort.hpp), permute, cell ranges
 // ======================================================================================
 
-// findPositionCell + fillFluidCellID (src/pair.cu:8-27): key of every fluid slot, scattered to
-// original-index order so that a stable sort over original order yields (cell, orig) order.
+// findPositionCell + fillFluidCellID (src/pair.cu:8-27) on the half-cell grid: sub-cell key
+// of every fluid slot, (sy + sx * 2 ny) after the case's 4 ncell sub-cells offset, scattered
+// to original-index order so that a stable sort over original order yields the
+// (sub-column, sub-row, original index) slot order.
 template <class Pol>
 __global__ void __launch_bounds__(256) k_hash_fluid(Dev<Pol> d) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -75,10 +116,10 @@ __global__ void __launch_bounds__(256) k_hash_fluid(Dev<Pol> d) {
   const int c = case_of(s, d.f_case_off, d.ncase);
   const auto &cp = d.cases[c];
   unsigned fl = 0;
-  const int cx = axis_cell(d.fa.x[s], cp.cxmin, cp.inv_cdx, cp.nx, &fl);
-  const int cy = axis_cell(d.fa.y[s], cp.cymin, cp.inv_cdy, cp.ny, &fl);
+  const int sx = axis_sub(d.fa.x[s], cp.cxmin, cp.inv_cdx, cp.nx, &fl);
+  const int sy = axis_sub(d.fa.y[s], cp.cymin, cp.inv_cdy, cp.ny, &fl);
   const int o = d.f_orig[s];
-  d.f_key_by_orig[o] = cp.cell_off + cy + cx * cp.ny;
+  d.f_key_by_orig[o] = 4 * cp.cell_off + sy + sx * (2 * cp.ny);
   d.f_slot_of_orig[o] = s;
   if (fl) atomicOr(d.status, kStOutOfGrid);
 }
@@ -122,10 +163,11 @@ __global__ void __launch_bounds__(256) k_hash_wall(Dev<Pol> d, const double *x, 
 
 // Gather the fluid state into sorted order (the zip permute of Fluid_List::sort,
 // src/sort.cu:10-20 / include/particle.cuh:93-96), rebuild the views, reset the per-case
-// maximum speed of this step.
+// maximum speed slots and the path lengths of this step.
 template <class Pol>
 __global__ void __launch_bounds__(256) k_permute_fluid(Dev<Pol> d, const int *sorted_vals) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < d.ncase * kSpread) d.umax_slot[(size_t)t * kSlotStride] = 0ull;
   if (t < d.ncase) d.umax2[t] = 0ull;
   if (t >= d.nf) return;
   const int o = sorted_vals[t];
@@ -137,15 +179,17 @@ __global__ void __launch_bounds__(256) k_permute_fluid(Dev<Pol> d, const int *so
   d.fb.v[t] = v;
   d.fb.rho[t] = rho;
   d.fb.p[t] = d.fa.p[s];
+  d.fb.cum[t] = 0.f;
   d.f_phase2[t] = d.f_phase[s];
   d.f_cls2[t] = d.f_cls[s];
   d.f_orig[t] = o;
   const int c = case_of(t, d.f_case_off, d.ncase);
   const auto &cp = d.cases[c];
   unsigned fl = 0;
-  const int cx = axis_cell(x, cp.cxmin, cp.inv_cdx, cp.nx, &fl);
-  const int cy = axis_cell(y, cp.cymin, cp.inv_cdy, cp.ny, &fl);
-  d.f_cxy[t] = cxy_pack(cx, cy);
+  const int sx = axis_sub(x, cp.cxmin, cp.inv_cdx, cp.nx, &fl);
+  const int sy = axis_sub(y, cp.cymin, cp.inv_cdy, cp.ny, &fl);
+  d.f_cxy[t] = cxy_pack(sx, sy);
+  const int cx = sx >> 1, cy = sy >> 1;
   if constexpr (Pol::MIXED) {
     d.fb.vx[t] = view_x<Pol>(x, cx, cp);
     d.fb.vy[t] = view_y<Pol>(y, cy, cp);
@@ -177,11 +221,10 @@ __global__ void __launch_bounds__(256) k_permute_wall(Dev<Pol> d, const int *sor
   d.w_vy[t] = view_y<Pol>(y, cy, cp);
 }
 
-__global__ void __launch_bounds__(256) k_clear_ranges(int2 *r0, int2 *r1, int n) {
+__global__ void __launch_bounds__(256) k_clear_ranges(int2 *r0, int n0, int2 *r1, int n1) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  r0[i] = make_int2(0, 0);
-  if (r1) r1[i] = make_int2(0, 0);
+  if (i < n0) r0[i] = make_int2(0, 0);
+  if (r1 && i < n1) r1[i] = make_int2(0, 0);
 }
 
 // Cell [start, end) from the sorted keys (replaces reduce_by_key + exclusive_scan +
@@ -195,13 +238,18 @@ __global__ void __launch_bounds__(256) k_cell_bounds(const int *keys, int n, int
 }
 
 // The reference keeps at most CELL_SIZE = 64 particles per cell (include/global.h:9).
-__global__ void __launch_bounds__(256) k_cell_audit(const int2 *fr, const int2 *sr, const int2 *wr, int n,
-                                                    unsigned *status) {
+template <class Pol>
+__global__ void __launch_bounds__(256) k_cell_audit(Dev<Pol> d, unsigned *status) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   bool over = false;
-  if (i < n) {
-    const int2 a = fr[i], b = sr[i], w = wr[i];
-    over = (a.y - a.x) + (b.y - b.x) + (w.y - w.x) > 64;
+  if (i < d.total_cells) {
+    const int c = case_of_cell(i, d);
+    const auto &cp = d.cases[c];
+    const int q = i - cp.cell_off, jx = q / cp.ny, jy = q - jx * cp.ny;
+    int2 r0, r1;
+    fluid_cell_runs(d, cp, jx, jy, r0, r1);
+    const int2 b = d.s_range[i], w = d.w_range[i];
+    over = (r0.y - r0.x) + (r1.y - r1.x) + (b.y - b.x) + (w.y - w.x) > 64;
   }
   if (__ballot(over ? 1 : 0) != 0ull && (threadIdx.x & 63) == 0) atomicOr(status, kStCellOver64);
 }
@@ -275,20 +323,23 @@ __global__ void __launch_bounds__(256) k_wall_update(Dev<Pol> d) {
       const int jy = cy + b;
       if (jy < 0 || jy >= cp.ny) continue;
       const R by = yi - cell_shift_y<Pol>(b, cp);
-      const int2 r = d.f_range[cp.cell_off + jx * cp.ny + jy];
-      for (int s = r.x; s < r.y; ++s) {
-        const R dx = F.vx[s] - bx, dy = F.vy[s] - by;
-        const R r2 = dx * dx + dy * dy;
-        if (!(r2 < cp.rc2)) continue;
-        const R w = (kexp<R>(-(r2 * cp.hinv2)) - cp.eR) * cp.k2c;
-        S += w;
-        U += F.vu[s] * w;
-        V += F.vv[s] * w;
-        P += (F.p[s] - F.vrho[s] * (cp.gx * dx + cp.gy * dy)) * w;
-        if (!found || r2 < nearest) {
-          found = true;
-          nearest = r2;
-          phase = d.f_phase[s];
+      int2 run[2];
+      fluid_cell_runs(d, cp, jx, jy, run[0], run[1]);
+      for (int h = 0; h < 2; ++h) {
+        for (int s = run[h].x; s < run[h].y; ++s) {
+          const R dx = F.vx[s] - bx, dy = F.vy[s] - by;
+          const R r2 = dx * dx + dy * dy;
+          if (!(r2 < cp.rc2)) continue;
+          const R w = (kexp<R>(-(r2 * cp.hinv2)) - cp.eR) * cp.k2c;
+          S += w;
+          U += F.vu[s] * w;
+          V += F.vv[s] * w;
+          P += (F.p[s] - F.vrho[s] * (cp.gx * dx + cp.gy * dy)) * w;
+          if (!found || r2 < nearest) {
+            found = true;
+            nearest = r2;
+            phase = d.f_phase[s];
+          }
         }
       }
     }
@@ -318,20 +369,35 @@ __global__ void __launch_bounds__(256) k_wall_update(Dev<Pol> d) {
 }
 
 // ======================================================================================
-// LDS neighbour tiles
+// Tiles over the half-cell grid
 // ======================================================================================
-// A block owns blockDim.x consecutive sorted fluid slots, whose global keys lie in [k0, k1].
-// The 3x3 neighbour cells of any of them that sit in neighbour column a (-1, 0, +1) have keys
-// in [k0 + a ny - 1, k1 + a ny + 1]: a contiguous key range, hence a contiguous slot segment.
-// The block stages the three segments in LDS once (coalesced) and every lane reads its
-// candidates from there.  A tile that straddles two cases, spans more than kTileMaxCells
-// cells per segment or holds more than the kernel's capacity falls back to global loads.
-constexpr int kTileMaxCells = 96;
-
-struct TileHdr {
-  int ok;
-  int c0, c1;  // cases of the first and last slot of the block
-  int k0[3], nc[3], lo[3], hi[3], base[3];
+// Once per step (after the hash), the fluid slots are cut into tiles of at most BLOCK
+// consecutive slots of one sub-column of one case, and every tile gets a plan: the
+// candidate sub-columns [sc0, sc0 + nsc) and sub-rows [r0, r0 + nr) of its lanes -- the 3x3
+// cells of the tile's cell column over its rows -- as slot spans, and whether any solid or
+// wall particle lies in those cells.  The row table f_lb gives, per sub-column, the first
+// slot of every sub-row or later (2 ny + 1 entries, the last one the sub-column's end), so a
+// row window [ra, rb] of a sub-column is the slot range [lb(ra), lb(rb + 1)).  The pair
+// kernel and every shift pass reuse the plans: a block loads its plan, copies the spans and
+// their row tables into LDS, and its lanes read their candidates there.  A tile whose spans
+// exceed the kernel's LDS capacity reads them from global memory instead.
+//
+// Pruning.  A lane at position (tx, ty) in sub-cell units needs, in sub-column sc, only the
+// rows that meet the disc of radius Rp around it, Rp = rc (1 + 1e-5) + D, where D bounds how
+// far any candidate has moved since the hash (neighbours are binned by their hashed
+// positions): the largest path length since the hash among the tile's candidates (0 right
+// after the hash).  Every skipped candidate therefore lies beyond rc at the current
+// positions -- by a margin far above the rounding of either precision -- and would have added
+// exact zeros to every sum.
+constexpr int kTileCols = 6;  // sub-columns of the 3x3 cells of a sub-column's cell column
+// LDS budget of a tile of B slots: candidate particles and sub-rows.  A dense tile in one
+// sub-column has about 6 B + 140 candidates (six sub-columns over its rows and three more on
+// either side) over about B / 3.75 + 6 rows; a tile beyond the budget reads global memory.
+__host__ __device__ constexpr int plan_cap(int B) { return 7 * B + 256; }
+__host__ __device__ constexpr int plan_rows(int B) { return B / 2 + 32; }
+template <int BLOCK>
+struct TileRows {
+  static constexpr int value = plan_rows(BLOCK);
 };
 
 template <class R>
@@ -345,75 +411,286 @@ struct V2<double> {
   using T = double2;
 };
 
-// Every thread of the block must call this (it synchronises).  On return T.ok says whether
-// the tile is staged; cr[a][k] holds the slot range of cell T.k0[a] + k.
+struct TilePlan {
+  int c, start, count;   // case, first slot, lanes
+  int sx, cx;            // sub-column and cell column of the tile
+  int sc0, nsc, r0, nr;  // candidate sub-columns and sub-rows
+  int cy_ref;            // cell row of the M32 shift frame
+  int sw;                // a solid or wall particle lies in the tile's 3x3 cells
+  int total;             // candidate particles (sum of the span lengths)
+  int lo[kTileCols], len[kTileCols], row[kTileCols];  // slot span; f_lb index of (sc0 + a, r0)
+};
+
+// Row table of every sub-column (one wave per global sub-column): f_lb[k] for row k = the
+// suffix minimum of the non-empty rows' starts, with the sub-column's end past the last
+// non-empty row.
 template <class Pol>
-__device__ __forceinline__ void tile_setup(const Dev<Pol> &d, int i0, int cap, TileHdr &T,
-                                           int2 (*cr)[kTileMaxCells]) {
+__global__ void __launch_bounds__(256) k_subcol_rows(Dev<Pol> d, int G, const int *col_case) {
+  const int g = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (g >= G) return;
+  const auto &cp = d.cases[col_case[g]];
+  const int sx = g - cp.col_off, ny2 = 2 * cp.ny;
+  const int kb = 4 * cp.cell_off + sx * ny2;
+  int *lb = d.f_lb + cp.lb_off + sx * (ny2 + 1);
+  int end = 0;
+  for (int k = lane; k < ny2; k += 64) {
+    const int2 r = d.f_srange[kb + k];
+    if (r.y > r.x) end = max(end, r.y);
+  }
+  for (int o = 32; o > 0; o >>= 1) end = max(end, __shfl_xor(end, o, 64));
+  int carry = end;
+  for (int c0 = ((ny2 - 1) >> 6) << 6; c0 >= 0; c0 -= 64) {
+    const int k = c0 + lane;
+    int v = 0x7fffffff;
+    if (k < ny2) {
+      const int2 r = d.f_srange[kb + k];
+      if (r.y > r.x) v = r.x;
+    }
+    for (int o = 1; o < 64; o <<= 1) {
+      const int u = __shfl_down(v, o, 64);
+      if (lane + o < 64) v = min(v, u);
+    }
+    v = min(v, carry);
+    carry = __shfl(v, 0, 64);
+    if (k < ny2) lb[k] = v;
+  }
+  if (lane == 0) lb[ny2] = end;
+}
+
+// Tiles of B slots per sub-column (one thread per global sub-column).
+template <class Pol>
+__global__ void __launch_bounds__(256) k_tile_count(Dev<Pol> d, int G, const int *col_case, int B, int *cnt) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  const auto &cp = d.cases[col_case[g]];
+  const int ny2 = 2 * cp.ny;
+  const int *lb = d.f_lb + cp.lb_off + (g - cp.col_off) * (ny2 + 1);
+  cnt[g] = (lb[ny2] - lb[0] + B - 1) / B;
+}
+
+// Exclusive scan of the per-sub-column tile counts (one block of 1024 threads); the total
+// goes to off[G].
+__global__ void __launch_bounds__(1024) k_tile_scan(const int *cnt, int G, int *off) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x, per = (G + 1023) / 1024, a = min(t * per, G), b = min(a + per, G);
+  int s = 0;
+  for (int k = a; k < b; ++k) s += cnt[k];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {  // Hillis-Steele inclusive scan
+    const int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - s;
+  for (int k = a; k < b; ++k) {
+    off[k] = run;
+    run += cnt[k];
+  }
+  if (t == 1023) off[G] = part[1023];
+}
+
+// Plans of the tiles of size B (one wave per global sub-column, looping over its tiles).
+template <class Pol>
+__global__ void __launch_bounds__(256) k_tile_plan(Dev<Pol> d, int G, const int *col_case, int B, const int *off,
+                                                   TilePlan *plans) {
+  const int g = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  if (g >= G) return;
+  const int c = col_case[g];
+  const auto &cp = d.cases[c];
+  const int sx = g - cp.col_off, cx = sx >> 1, ny2 = 2 * cp.ny;
+  const int *lbc = d.f_lb + cp.lb_off;
+  const int s0 = lbc[sx * (ny2 + 1)], s1 = lbc[sx * (ny2 + 1) + ny2];
+  const int ntile = (s1 - s0 + B - 1) / B;
+  for (int k = 0; k < ntile; ++k) {
+    const int tile = off[g] + k, start = s0 + k * B, count = min(B, s1 - start);
+    const int sy0 = cxy_y(d.f_cxy[start]), sy1 = cxy_y(d.f_cxy[start + count - 1]);
+    const int sc0 = max(2 * cx - 2, 0), sc1 = min(2 * cx + 3, 2 * cp.nx - 1);
+    const int r0 = max(2 * (sy0 >> 1) - 2, 0), r1 = min(2 * (sy1 >> 1) + 3, ny2 - 1);
+    const int nsc = sc1 - sc0 + 1;
+    // spans: lanes 0..nsc-1 take one sub-column each
+    int lo = 0, len = 0, row = 0;
+    if (lane < nsc) {
+      row = (sc0 + lane) * (ny2 + 1) + r0;
+      lo = lbc[row];
+      len = lbc[row + (r1 - r0 + 1)] - lo;
+      row += cp.lb_off;
+    }
+    int total = len;
+    for (int o = 32; o > 0; o >>= 1) total += __shfl_xor(total, o, 64);
+    // solid / wall presence over the 3x3 cells of the tile's lanes
+    const int wc0 = max(cx - 1, 0), wnc = min(cx + 1, cp.nx - 1) - wc0 + 1;
+    const int wr0 = max((sy0 >> 1) - 1, 0), wnr = min((sy1 >> 1) + 1, cp.ny - 1) - wr0 + 1;
+    bool has = false;
+    for (int e = lane; e < wnc * wnr; e += 64) {
+      const int col = e / wnr, q = cp.cell_off + (wc0 + col) * cp.ny + wr0 + (e - col * wnr);
+      const int2 a = d.s_range[q], w = d.w_range[q];
+      has = has || a.y > a.x || w.y > w.x;
+    }
+    const bool sw = __ballot(has ? 1 : 0) != 0ull;
+    TilePlan &P = plans[tile];
+    if (lane < kTileCols) {
+      P.lo[lane] = lo;
+      P.len[lane] = len;
+      P.row[lane] = row;
+    }
+    if (lane == 0) {
+      P.c = c;
+      P.start = start;
+      P.count = count;
+      P.sx = sx;
+      P.cx = cx;
+      P.sc0 = sc0;
+      P.nsc = nsc;
+      P.r0 = r0;
+      P.nr = r1 - r0 + 1;
+      P.cy_ref = (sy0 + sy1) >> 2;
+      P.sw = sw ? 1 : 0;
+      P.total = total;
+    }
+  }
+}
+
+// Per-case maximum of non-negative double bit patterns: one atomic per block (a tile lies in
+// one case), into slot blockIdx.x % kSpread of the case.  Every thread must call it.
+template <int BLOCK>
+__device__ __forceinline__ void reduce_case_max(unsigned long long *slots, int c, bool valid, unsigned long long v,
+                                                unsigned long long *s_red) {
   const int t = threadIdx.x;
+  const unsigned long long w = wave_max_u64(valid ? v : 0ull);
+  if ((t & 63) == 0) s_red[t >> 6] = w;
+  __syncthreads();
   if (t == 0) {
-    const int il = min(i0 + (int)blockDim.x, d.nf) - 1;
-    const int c0 = case_of(i0, d.f_case_off, d.ncase), c1 = case_of(il, d.f_case_off, d.ncase);
-    const auto &cp = d.cases[c0];
-    const int k0 = d.f_key[i0], k1 = d.f_key[il];
-    const int cmin = cp.cell_off, cmax = cp.cell_off + cp.ncell - 1;
-    for (int a = 0; a < 3; ++a) {
-      const int ka = max(k0 + (a - 1) * cp.ny - 1, cmin), kb = min(k1 + (a - 1) * cp.ny + 1, cmax);
-      T.k0[a] = ka;
-      T.nc[a] = kb >= ka ? kb - ka + 1 : 0;
-      T.lo[a] = 0x7fffffff;
-      T.hi[a] = -1;
-    }
-    T.ok = (c0 == c1) && (k1 - k0 + 3 <= kTileMaxCells);
-    T.c0 = c0;
-    T.c1 = c1;
+    unsigned long long m = 0ull;
+    for (int k = 0; k < BLOCK / 64; ++k) m = s_red[k] > m ? s_red[k] : m;
+    if (m != 0ull) atomicMax(slots + (size_t)(c * kSpread + (int)(blockIdx.x % kSpread)) * kSlotStride, m);
   }
-  __syncthreads();
-  if (T.ok) {
-    for (int e = t; e < 3 * kTileMaxCells; e += blockDim.x) {
-      const int a = e / kTileMaxCells, k = e - a * kTileMaxCells;
-      if (k < T.nc[a]) {
-        const int2 r = d.f_range[T.k0[a] + k];
-        cr[a][k] = r;
-        if (r.y > r.x) {
-          atomicMin(&T.lo[a], r.x);
-          atomicMax(&T.hi[a], r.y);
+}
+
+// umax2[c] = max over the case's slots; one wave per case.
+__global__ void __launch_bounds__(64) k_umax_final(const unsigned long long *slots, unsigned long long *umax2,
+                                                   int ncase) {
+  const int c = blockIdx.x;
+  if (c >= ncase) return;
+  const unsigned long long m = wave_max_u64(slots[(size_t)(c * kSpread + (int)threadIdx.x) * kSlotStride]);
+  if (threadIdx.x == 0) umax2[c] = m;
+}
+
+// Copies a tile into LDS: the row tables (lb[a][k] = LDS index of the first candidate of row
+// r0 + k or later of sub-column sc0 + a, k = 0..nr) and, through ld / st, the candidates:
+// element e of the concatenated spans (LDS index e) is slot lo[a] + e - base[a] of its span a.
+// Each thread issues the loads of U elements before storing any.
+template <int BLOCK, int U, class Ld, class St>
+__device__ __forceinline__ void tile_stage(const TilePlan &P, const int *f_lb, int (*lb)[TileRows<BLOCK>::value + 1],
+                                           Ld &&ld, St &&st) {
+  const int t = threadIdx.x, n = P.nsc, nr1 = P.nr + 1;
+  int base[kTileCols], lo[kTileCols], row[kTileCols];
+  {
+    int b = 0;
+#pragma unroll
+    for (int a = 0; a < kTileCols; ++a) {
+      base[a] = a < n ? b : 0x7fffffff;
+      lo[a] = P.lo[a];
+      row[a] = P.row[a];
+      b += a < n ? P.len[a] : 0;
+    }
+  }
+  for (int e0 = t; e0 < n * nr1; e0 += 2 * BLOCK) {
+    int v[2], dst[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int e = e0 + u * BLOCK;
+      dst[u] = -1;
+      if (e < n * nr1) {
+        const int a = e / nr1, k = e - a * nr1;
+        int b = 0, l = 0, r = 0;
+#pragma unroll
+        for (int q = 0; q < kTileCols; ++q)
+          if (q == a) {
+            b = base[q];
+            l = lo[q];
+            r = row[q];
+          }
+        v[u] = f_lb[r + k] - l + b;
+        dst[u] = a * (TileRows<BLOCK>::value + 1) + k;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u)
+      if (dst[u] >= 0) (&lb[0][0])[dst[u]] = v[u];
+  }
+  const int total = P.total;
+  for (int e0 = t; e0 < total; e0 += U * BLOCK) {
+    decltype(ld(0)) v[U];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int e = e0 + k * BLOCK;
+      if (e < total) {
+        int b = base[0], l = lo[0];
+#pragma unroll
+        for (int a = 1; a < kTileCols; ++a) {
+          const bool in = e >= base[a];
+          b = in ? base[a] : b;
+          l = in ? lo[a] : l;
         }
+        v[k] = ld(l + (e - b));
       }
     }
-  }
-  __syncthreads();
-  if (t == 0 && T.ok) {
-    int base = 0;
-    for (int a = 0; a < 3; ++a) {
-      if (T.hi[a] < 0) {
-        T.lo[a] = 0;
-        T.hi[a] = 0;
-      }
-      T.base[a] = base;
-      base += T.hi[a] - T.lo[a];
+#pragma unroll
+    for (int k = 0; k < U; ++k) {
+      const int e = e0 + k * BLOCK;
+      if (e < total) st(e, v[k]);
     }
-    T.ok = base <= cap;
-  }
-  __syncthreads();
-}
-
-// Copy field f of the three segments into LDS (dst[base[a] + k] = f(lo[a] + k)).
-template <class F>
-__device__ __forceinline__ void tile_stage(const TileHdr &T, F &&f) {
-  for (int a = 0; a < 3; ++a) {
-    const int n = T.hi[a] - T.lo[a];
-    for (int k = threadIdx.x; k < n; k += blockDim.x) f(T.base[a] + k, T.lo[a] + k);
   }
 }
 
-// Gradient weight w'(r^2) = exp(-r^2 / h^2) kd c32 (Kernel::initiate, src/kernel.cpp:37-40).
-template <class R, class CP>
-__device__ __forceinline__ R grad_w(R r2, const CP &cp) {
-  if constexpr (sizeof(R) == 4)
-    return __builtin_amdgcn_exp2f(r2 * cp.nhl2) * cp.kdc;
-  else
-    return exp(-(r2 * cp.hinv2)) * cp.kdc;
+// A lane's position relative to its hashed sub-cell, in sub-cell units.
+struct LanePos {
+  float fx, fy;
+};
+template <class CP>
+__device__ __forceinline__ LanePos lane_pos(double x, double y, int sx, int sy, const CP &cp) {
+  const double tx = (x - cp.cxmin) * cp.inv_cdx * 2.0 - (double)sx;
+  const double ty = (y - cp.cymin) * cp.inv_cdy * 2.0 - (double)sy;
+  return {(float)fmin(fmax(tx, -8.0), 9.0), (float)fmin(fmax(ty, -8.0), 9.0)};
+}
+
+// Visits the candidate row windows of a lane: f(sc, ra, rb) for each sub-column sc that meets
+// the lane's disc of radius R, with the sub-rows [ra, rb] that do, clipped to the 3x3 cells
+// of the lane's cell (the tile's cell column, row cy) and to the tile's candidate range.
+template <class CP, class F>
+__device__ __forceinline__ void for_windows(const TilePlan &P, const CP &cp, int sy, int cy, LanePos lp, float R,
+                                            F &&f) {
+  const int sx = P.sx;
+  const float R2 = R * R;
+  const int a_lo = max((int)floorf(lp.fx - R * cp.inv_ssx), P.sc0 - sx);
+  const int a_hi = min((int)floorf(lp.fx + R * cp.inv_ssx), P.sc0 + P.nsc - 1 - sx);
+  const int rmin = max(2 * cy - 2, P.r0), rmax = min(2 * cy + 3, P.r0 + P.nr - 1);
+  for (int a = a_lo; a <= a_hi; ++a) {
+    const float g = fmaxf(fmaxf((float)a - lp.fx, lp.fx - (float)(a + 1)), 0.f) * cp.ssx;
+    if (g >= R) continue;
+    const float hr = fminf(__builtin_amdgcn_sqrtf(R2 - g * g) * cp.inv_ssy, 8.f);
+    const int ra = max(sy + (int)floorf(lp.fy - hr), rmin);
+    const int rb = min(sy + (int)floorf(lp.fy + hr), rmax);
+    if (ra <= rb) f(sx + a, ra, rb);
+  }
+}
+
+// Pruning radius: rc (1 + 1e-5) plus the tile's displacement bound D; a tile that is not
+// staged searches its lanes' whole 3x3 cells.
+template <class CP>
+__device__ __forceinline__ float prune_radius(const CP &cp, bool staged, float D) {
+  if (staged) return (float)cp.rcut_prune + D;
+  return (float)(16.0 * fmax(cp.cdx, cp.cdy));
+}
+
+// Largest path length among the staged candidates of the block (after the staging barrier).
+template <int BLOCK>
+__device__ __forceinline__ float staged_bound(const float *s_dmax) {
+  float D = 0.f;
+  for (int k = 0; k < BLOCK / 64; ++k) D = fmaxf(D, s_dmax[k]);
+  return D * (1.0f + 1e-6f);
 }
 
 // Calls f(s) for s in [lo, hi), U at a time so the loads of U candidates are in flight together.
@@ -425,6 +702,22 @@ __device__ __forceinline__ void for_range(int lo, int hi, F &&f) {
     for (int k = 0; k < U; ++k) f(s + k);
   }
   for (; s < hi; ++s) f(s);
+}
+
+// Calls f(s, true) for s in [lo, hi) U at a time, then covers the remainder with one more
+// group of U calls f(s, s < hi): the caller makes a call with false add exact zeros, and
+// may read up to U - 1 entries past hi.
+template <int U, class F>
+__device__ __forceinline__ void for_range_masked(int lo, int hi, F &&f) {
+  int s = lo;
+  for (const int last = hi - (U - 1); s < last; s += U) {
+#pragma unroll
+    for (int k = 0; k < U; ++k) f(s + k, true);
+  }
+  if (s < hi) {
+#pragma unroll
+    for (int k = 0; k < U; ++k) f(s + k, s + k < hi);
+  }
 }
 
 // Per-case constants a neighbour loop needs, held in registers.
@@ -488,68 +781,142 @@ __device__ __forceinline__ void riemann_pair(R dx, R dy, R r2, R w, R ui, R vi, 
   }
 }
 
-// LDS capacity of a tile (particles) for a block of BLOCK slots: about 3.4 x BLOCK particles
-// are staged in a dense region; the rest is headroom for sparse or compressed tiles.
+// LDS capacity (particles) of a tile of BLOCK slots: the planner's budget.
 template <class Pol, int BLOCK>
 struct PairCap {
-  static constexpr int value = (Pol::MIXED ? 1536 : 1024) * BLOCK / 256;
+  static constexpr int value = plan_cap(BLOCK);
 };
 template <int BLOCK>
 struct ShiftCap {
-  static constexpr int value = 2048 * BLOCK / 256;
+  static constexpr int value = plan_cap(BLOCK);
 };
 
-// findFluidPair (src/fluid.cu:193-254) + fluidUpdate (src/fluid.cu:256-287).  Reads the
-// current buffers (fa) of every neighbour and writes particle i's updated state into the
-// alternate buffers (fb), so neighbours never see a half-updated state.  Also reduces the
-// per-case max |u|^2 of the updated velocities (getMaxSpeed, src/fluid.cu:429-440).
+// findFluidPair (src/fluid.cu:193-254) + fluidUpdate (src/fluid.cu:256-287), one tile per
+// block.  Reads the current buffers (fa) of every neighbour and writes particle i's updated
+// state into the alternate buffers (fb), so neighbours never see a half-updated state.  Also
+// reduces the per-case max |u|^2 of the updated velocities (getMaxSpeed,
+// src/fluid.cu:429-440) and advances every particle's path length since the hash.
 template <class Pol, bool VISCID, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d) {
+__global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const TilePlan *plans, const int *ntiles) {
   using R = typename Pol::R;
   using R2 = typename V2<R>::T;
   constexpr int CAP = PairCap<Pol, BLOCK>::value;
-  __shared__ R2 s_pos[CAP], s_vel[CAP], s_rp[CAP];
-  __shared__ int s_fl[CAP];
-  __shared__ int2 s_cr[3][kTileMaxCells];
-  __shared__ TileHdr T;
-  const int i0 = blockIdx.x * blockDim.x;
-  const int i = i0 + threadIdx.x;
+  // + padding read by the masked tail of a window: far away, finite, no flags
+  __shared__ R2 s_pos[CAP + 2], s_vel[CAP + 2], s_rp[CAP + 2];
+  __shared__ int s_fl[CAP + 2];
+  __shared__ int s_lb[kTileCols][TileRows<BLOCK>::value + 1];
+  __shared__ unsigned long long s_red[BLOCK / 64];
+  __shared__ float s_dmax[BLOCK / 64];
+  if ((int)blockIdx.x >= *ntiles) return;
+  const TilePlan &P = plans[blockIdx.x];
+  const int t = threadIdx.x, i = P.start + t, c = P.c;
+  const bool valid = t < P.count;
   const auto &A = d.fa;
-  tile_setup(d, i0, CAP, T, s_cr);
-  const bool staged = T.ok;
+  const bool staged = P.total <= CAP && P.nr <= TileRows<BLOCK>::value;
+  float dmax = 0.f;
   if (staged) {
-    tile_stage(T, [&](int l, int g) {
-      s_pos[l] = R2{A.vx[g], A.vy[g]};
-      s_vel[l] = R2{A.vu[g], A.vv[g]};
-      s_rp[l] = R2{A.vrho[g], A.p[g]};
-      s_fl[l] = (int)d.f_phase[g] | ((int)d.f_cls[g] << 8);
-    });
+    struct Ld {
+      R2 pos, vel, rp;
+      int fl;
+      float cum;
+      int pk;
+    };
+    const auto &cq = d.cases[c];
+    const R fdx = cq.cdx_r, fdy = cq.cdy_r;
+    const int gcx = P.cx, gcy = P.cy_ref;
+    tile_stage<BLOCK, 4>(
+        P, d.f_lb, s_lb,
+        [&](int g) {
+          return Ld{R2{A.vx[g], A.vy[g]}, R2{A.vu[g], A.vv[g]}, R2{A.vrho[g], A.p[g]},
+                    (int)d.f_phase[g] | ((int)d.f_cls[g] << 8), A.cum[g], Pol::MIXED ? d.f_cxy[g] : 0};
+        },
+        [&](int l, const Ld &v) {
+          if constexpr (Pol::MIXED)  // tile frame
+            s_pos[l] = R2{v.pos.x + (float)((cxy_x(v.pk) >> 1) - gcx) * fdx,
+                          v.pos.y + (float)((cxy_y(v.pk) >> 1) - gcy) * fdy};
+          else
+            s_pos[l] = v.pos;
+          s_vel[l] = v.vel;
+          s_rp[l] = v.rp;
+          s_fl[l] = v.fl;
+          dmax = fmaxf(dmax, v.cum);
+        });
+    if (t < 2) {
+      s_pos[P.total + t] = R2{R(1e18), R(1e18)};
+      s_vel[P.total + t] = R2{R(0), R(0)};
+      s_rp[P.total + t] = R2{R(1), R(0)};
+      s_fl[P.total + t] = 0;
+    }
   }
+  dmax = wave_max_f(dmax);
+  if ((t & 63) == 0) s_dmax[t >> 6] = dmax;
   __syncthreads();
   unsigned long long my_u2 = 0ull;
-  const int c = T.c0 == T.c1 ? T.c0 : (i < d.nf ? case_of(i, d.f_case_off, d.ncase) : T.c1);
-  if (i < d.nf) {
+  if (valid) {
     const auto &cp = d.cases[c];
-    const int pk = d.f_cxy[i], cx = cxy_x(pk), cy = cxy_y(pk);
+    const int pk = d.f_cxy[i], sy = cxy_y(pk), cx = P.cx, cy = sy >> 1;
     const R xi = A.vx[i], yi = A.vy[i], ui = A.vu[i], vi = A.vv[i], rhoi = A.vrho[i], pi = A.p[i];
+    const double x0 = A.x[i], y0 = A.y[i];
+    const LanePos lp = lane_pos(x0, y0, P.sx, sy, cp);
     const int phi = d.f_phase[i];
     const bool liq = phi == 0;
     const R cs = cp.c[phi], csinv = cp.cinv[phi], mu8 = cp.mu8[phi];
     const R rhoi_inv = krcp<R>(rhoi);
     const LoopConsts<R> K(cp);
     const R eps_h2 = cp.eps_h2, qrc2 = cp.quarter_rc2;
+    // LDS index of the lane's own slot (its span is the tile's own sub-column)
+    int own = i;
+    if (staged) {
+      int b = 0;
+      for (int a = 0; a < P.sx - P.sc0; ++a) b += P.len[a];
+      own = b + (i - P.lo[P.sx - P.sc0]);
+    }
     PairSums<R> S;
     // fluidFluidCalculate on one candidate (w = 0 outside the cut-off: exact zero terms)
-    auto fluid_j = [&](int s, R dx, R dy, R r2, R uj, R vj, R rhoj, R pj, int fl) {
-      const R w = K.w(r2);
+    auto fluid_j = [&](bool self, R dx, R dy, R r2, R uj, R vj, R rhoj, R pj, int fl, bool in = true) {
+      const R wk = K.w(r2);
+      const R w = in ? wk : R(0);
       riemann_pair<R, VISCID>(dx, dy, r2, w, ui, vi, uj, vj, rhoi, rhoi_inv, rhoj, pi, pj, cs, csinv, mu8, eps_h2, S);
       if (!(liq && (fl & 0xff) == 1)) S.col += r2 * w;
       // FREE_SURFACE neighbour: HAS_NEAR (2) within half the cut-off, else HAS (1); the
       // reference's sticky update (src/fluid.cu:67-73) ends at the maximum over neighbours.
-      const bool fs = r2 < K.rc2 && liq && s != i && (fl >> 8) == 1;
+      const bool fs = in & (r2 < K.rc2) & liq & !self & ((fl >> 8) == 1);
       S.near = max(S.near, fs ? (r2 < qrc2 ? 2 : 1) : 0);
     };
-    for (int a = -1; a <= 1; ++a) {
+    const int *lbc = d.f_lb + cp.lb_off;
+    const int ny21 = 2 * cp.ny + 1;
+    // rows [qa, qb] of sub-column sc from global memory, whose views share the cell origin
+    // behind (bx, by)
+    auto rows = [&](int sc, int qa, int qb, R bx, R by) {  // global memory
+      for_range<2>(lbc[sc * ny21 + qa], lbc[sc * ny21 + qb + 1], [&](int s) {
+        const R dx = A.vx[s] - bx, dy = A.vy[s] - by;
+        fluid_j(s == i, dx, dy, dx * dx + dy * dy, A.vu[s], A.vv[s], A.vrho[s], A.p[s],
+                (int)d.f_phase[s] | ((int)d.f_cls[s] << 8));
+      });
+    };
+    const float Rp = prune_radius(cp, staged, staged_bound<BLOCK>(s_dmax));
+    // the lane in the tile frame (M32 staged positions are re-based on it)
+    const R yg = Pol::MIXED ? yi + (R)(cy - P.cy_ref) * cp.cdy_r : yi;
+    for_windows(P, cp, sy, cy, lp, Rp, [&](int sc, int ra, int rb) {
+      if (staged) {  // one run per window
+        const int sa = sc - P.sc0;
+        for_range_masked<2>(s_lb[sa][ra - P.r0], s_lb[sa][rb + 1 - P.r0], [&](int l, bool in) {
+          const R2 pj = s_pos[l], vj = s_vel[l], rp = s_rp[l];
+          const R dx = pj.x - xi, dy = pj.y - yg;
+          fluid_j(l == own, dx, dy, dx * dx + dy * dy, vj.x, vj.y, rp.x, rp.y, s_fl[l], in);
+        });
+        return;
+      }
+      const R bx = xi - cell_shift_x<Pol>((sc >> 1) - cx, cp);
+      if constexpr (Pol::MIXED) {
+        // M32 views are relative to their own cell: one run per cell row
+        for (int cr = ra >> 1; cr <= (rb >> 1); ++cr)
+          rows(sc, max(ra, 2 * cr), min(rb, 2 * cr + 1), bx, yi - cell_shift_y<Pol>(cr - cy, cp));
+      } else {
+        rows(sc, ra, rb, bx, yi);
+      }
+    });
+    for (int a = -1; a <= 1 && P.sw; ++a) {
       const int jx = cx + a;
       if (jx < 0 || jx >= cp.nx) continue;
       const R bx = xi - cell_shift_x<Pol>(a, cp);
@@ -558,23 +925,6 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d) {
         if (jy < 0 || jy >= cp.ny) continue;
         const R by = yi - cell_shift_y<Pol>(b, cp);
         const int q = cp.cell_off + jx * cp.ny + jy;
-        if (staged) {
-          const int2 rf = s_cr[a + 1][q - T.k0[a + 1]];
-          const int off = T.base[a + 1] - T.lo[a + 1];
-          for_range<2>(rf.x, rf.y, [&](int s) {
-            const R2 pj = s_pos[s + off], vj = s_vel[s + off], rp = s_rp[s + off];
-            const R dx = pj.x - bx, dy = pj.y - by;
-            fluid_j(s, dx, dy, dx * dx + dy * dy, vj.x, vj.y, rp.x, rp.y, s_fl[s + off]);
-          });
-        } else {
-          const int2 rf = d.f_range[q];
-          for (int s = rf.x; s < rf.y; ++s) {
-            const R dx = A.vx[s] - bx, dy = A.vy[s] - by;
-            const R r2 = dx * dx + dy * dy;
-            if (!(r2 < K.rc2)) continue;
-            fluid_j(s, dx, dy, r2, A.vu[s], A.vv[s], A.vrho[s], A.p[s], (int)d.f_phase[s] | ((int)d.f_cls[s] << 8));
-          }
-        }
         const int2 rs = d.s_range[q];
         for (int k = rs.x; k < rs.y; ++k) {  // fluidSolidCalculate: rho_j = rho_i
           const int j = d.s_sorted[k];
@@ -605,7 +955,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d) {
     d.f_L[4 * i + 2] = S.L10;
     d.f_L[4 * i + 3] = S.L11;
     // fluidUpdate, integrated in double
-    const double x0 = A.x[i], y0 = A.y[i], u0 = A.u[i], v0 = A.v[i], r0 = A.rho[i];
+    const double u0 = A.u[i], v0 = A.v[i], r0 = A.rho[i];
     double ax = (double)S.ax, ay = (double)S.ay;
     if (cp.steady) {
       ax += -0.1 * u0;
@@ -636,17 +986,12 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d) {
     }
     const double u2 = u * u + v * v;
     my_u2 = (unsigned long long)__double_as_longlong(u2);
+    const double mx = x - x0, my = y - y0;
+    B.cum[i] = (float)((double)A.cum[i] + sqrt(mx * mx + my * my));
     if (isnan(rho) || isnan(x) || isnan(y) || isnan(u2)) atomicOr(d.status, kStNaN);
   }
-  // per-case max |u|^2 (non-negative doubles order like their bit patterns): one atomic per
-  // wave when the wave lies in one case, else one per lane
-  const int c_lane0 = __shfl(c, 0, 64);
-  if (__ballot(c != c_lane0 ? 1 : 0) == 0ull) {
-    const unsigned long long m = wave_max_u64(my_u2);
-    if ((threadIdx.x & 63) == 0 && m != 0ull) atomicMax(&d.umax2[c], m);
-  } else if (i < d.nf && my_u2 != 0ull) {
-    atomicMax(&d.umax2[c], my_u2);
-  }
+  // per-case max |u|^2 (non-negative doubles order like their bit patterns)
+  reduce_case_max<BLOCK>(d.umax_slot, c, valid, my_u2, s_red);
 }
 
 // ======================================================================================
@@ -654,36 +999,66 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d) {
 // ======================================================================================
 
 // correctionCalculate (src/fluid.cu:316-367) + correctionUpdate / shiftDisplacementCalculate
-// (src/fluid.cu:369-427).  The gradient sums over all species with no phase filter.
+// (src/fluid.cu:369-427), one tile per block.  The gradient sums over all species with no
+// phase filter.  For M32 the staged fluid positions are re-based on the tile frame (the
+// tile's cell column, and the cell row cy_ref), so each sub-column window is one run; the
+// frame costs at most a few float ulps of a tile height in the gradient of a small
+// correction.
 template <class Pol, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, int pass) {
+__global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePlan *plans, const int *ntiles,
+                                                       int pass) {
   using R = typename Pol::R;
   using R2 = typename V2<R>::T;
   constexpr int CAP = ShiftCap<BLOCK>::value;
-  __shared__ R2 s_pos[CAP];
-  __shared__ int2 s_cr[3][kTileMaxCells];
-  __shared__ TileHdr T;
-  const int i0 = blockIdx.x * blockDim.x;
-  const int i = i0 + threadIdx.x;
+  __shared__ R2 s_pos[CAP + 4];  // + far-away padding read by the masked tail of a window
+  __shared__ int s_lb[kTileCols][TileRows<BLOCK>::value + 1];
+  __shared__ float s_dmax[BLOCK / 64];
+  if ((int)blockIdx.x >= *ntiles) return;
+  const TilePlan &P = plans[blockIdx.x];
+  const int t = threadIdx.x, i = P.start + t, c = P.c;
+  const bool valid = t < P.count;
   const auto &A = d.fa;
-  tile_setup(d, i0, CAP, T, s_cr);
-  const bool staged = T.ok;
-  if (staged) tile_stage(T, [&](int l, int g) { s_pos[l] = R2{A.vx[g], A.vy[g]}; });
-  __syncthreads();
-  if (i >= d.nf) return;
-  const int c = T.c0 == T.c1 ? T.c0 : case_of(i, d.f_case_off, d.ncase);
   const auto &cp = d.cases[c];
+  const bool staged = P.total <= CAP && P.nr <= TileRows<BLOCK>::value;
+  float dmax = 0.f;
+  if (staged) {
+    struct Ld {
+      R2 pos;
+      int pk;
+      float cum;
+    };
+    const R fdx = cp.cdx_r, fdy = cp.cdy_r, gs = cp.gscale;
+    const int gcx = P.cx, gcy = P.cy_ref;
+    tile_stage<BLOCK, 4>(
+        P, d.f_lb, s_lb, [&](int g) { return Ld{R2{A.vx[g], A.vy[g]}, Pol::MIXED ? d.f_cxy[g] : 0, A.cum[g]}; },
+        [&](int l, const Ld &v) {
+          if constexpr (Pol::MIXED)  // tile frame, scaled units
+            s_pos[l] = R2{(v.pos.x + (float)((cxy_x(v.pk) >> 1) - gcx) * fdx) * gs,
+                          (v.pos.y + (float)((cxy_y(v.pk) >> 1) - gcy) * fdy) * gs};
+          else
+            s_pos[l] = v.pos;
+          dmax = fmaxf(dmax, v.cum);
+        });
+    if (t < 4) s_pos[P.total + t] = R2{R(1e18), R(1e18)};
+  }
+  dmax = wave_max_f(dmax);
+  if ((t & 63) == 0) s_dmax[t >> 6] = dmax;
+  __syncthreads();
+  auto lane = [&]() {
   if (pass >= cp.K) {  // this case has fewer passes than the batch: carry positions over
     d.fb.x[i] = A.x[i];
     d.fb.y[i] = A.y[i];
+    d.fb.cum[i] = A.cum[i];
     if constexpr (Pol::MIXED) {
       d.fb.vx[i] = A.vx[i];
       d.fb.vy[i] = A.vy[i];
     }
     return;
   }
-  const int pk = d.f_cxy[i], cx = cxy_x(pk), cy = cxy_y(pk);
+  const int pk = d.f_cxy[i], sy = cxy_y(pk), cx = P.cx, cy = sy >> 1;
   const R xi = A.vx[i], yi = A.vy[i];
+  const double x0 = A.x[i], y0 = A.y[i];
+  const LanePos lp = lane_pos(x0, y0, P.sx, sy, cp);
   const LoopConsts<R> K(cp);
   R gx = 0, gy = 0;
   auto add = [&](R dx, R dy) {
@@ -691,7 +1066,43 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, int pass) {
     gx += dx * w;
     gy += dy * w;
   };
-  for (int a = -1; a <= 1; ++a) {
+  // the lane in the tile frame
+  const R yg = Pol::MIXED ? yi + (R)(cy - P.cy_ref) * cp.cdy_r : yi;
+  const float Rp = prune_radius(cp, staged, staged_bound<BLOCK>(s_dmax));
+  const int *lbc = d.f_lb + cp.lb_off;
+  const int ny21 = 2 * cp.ny + 1;
+  // staged fluid in scaled units: w = 2^-r'^2 inside the cut-off; sums rescaled below
+  R gxs = 0, gys = 0;
+  const R xs = xi * cp.gscale, ys = yg * cp.gscale, rc2s = cp.rc2s;
+  for_windows(P, cp, sy, cy, lp, Rp, [&](int sc, int ra, int rb) {
+    if (staged) {
+      const int sa = sc - P.sc0;
+      for_range_masked<4>(s_lb[sa][ra - P.r0], s_lb[sa][rb + 1 - P.r0], [&](int l, bool in) {
+        const R2 pj = s_pos[l];
+        if constexpr (Pol::MIXED) {
+          const R dx = pj.x - xs, dy = pj.y - ys, r2 = dx * dx + dy * dy;
+          const R e = __builtin_amdgcn_exp2f(-r2);
+          const R w = (in & (r2 < rc2s)) ? e : R(0);
+          gxs += dx * w;
+          gys += dy * w;
+        } else {
+          const R dx = pj.x - xi, dy = pj.y - yg;
+          const R wk = K.w(dx * dx + dy * dy);
+          const R w = in ? wk : R(0);
+          gx += dx * w;
+          gy += dy * w;
+        }
+      });
+    } else {  // global views, relative to their own cell: one run per cell row
+      const R bx = xi - cell_shift_x<Pol>((sc >> 1) - cx, cp);
+      for (int cr = ra >> 1; cr <= (rb >> 1); ++cr) {
+        const R by = yi - cell_shift_y<Pol>(cr - cy, cp);
+        const int lo = lbc[sc * ny21 + max(ra, 2 * cr)], hi = lbc[sc * ny21 + min(rb, 2 * cr + 1) + 1];
+        for_range<4>(lo, hi, [&](int s) { add(A.vx[s] - bx, A.vy[s] - by); });
+      }
+    }
+  });
+  for (int a = -1; a <= 1 && P.sw; ++a) {
     const int jx = cx + a;
     if (jx < 0 || jx >= cp.nx) continue;
     const R bx = xi - cell_shift_x<Pol>(a, cp);
@@ -700,17 +1111,6 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, int pass) {
       if (jy < 0 || jy >= cp.ny) continue;
       const R by = yi - cell_shift_y<Pol>(b, cp);
       const int q = cp.cell_off + jx * cp.ny + jy;
-      if (staged) {
-        const int2 rf = s_cr[a + 1][q - T.k0[a + 1]];
-        const int off = T.base[a + 1] - T.lo[a + 1];
-        for_range<4>(rf.x + off, rf.y + off, [&](int s) {
-          const R2 pj = s_pos[s];
-          add(pj.x - bx, pj.y - by);
-        });
-      } else {
-        const int2 rf = d.f_range[q];
-        for (int s = rf.x; s < rf.y; ++s) add(A.vx[s] - bx, A.vy[s] - by);
-      }
       const int2 rs = d.s_range[q];
       for (int k = rs.x; k < rs.y; ++k) {
         const int j = d.s_sorted[k];
@@ -720,9 +1120,13 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, int pass) {
       for (int s = rw.x; s < rw.y; ++s) add(d.w_vx[s] - bx, d.w_vy[s] - by);
     }
   }
+  if constexpr (Pol::MIXED) {
+    gx += gxs * cp.kdc_s;
+    gy += gys * cp.kdc_s;
+  }
   const R umax = fmin((R)sqrt(__longlong_as_double((long long)d.umax2[c])), cp.umax_cap);
   const R dxs = -cp.shift_k * gx, dys = -cp.shift_k * gy;
-  R sx = dxs, sy = dys;
+  R sxv = dxs, syv = dys;
   int cls;
   bool project = false;
   const R col = d.f_col[i];
@@ -732,8 +1136,8 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, int pass) {
   } else if (col < R(1.5)) {
     if (near == 0) {
       cls = 0;  // SPLASH
-      sx = 0;
-      sy = 0;
+      sxv = 0;
+      syv = 0;
     } else {
       cls = 1;  // FREE_SURFACE
       project = true;
@@ -756,13 +1160,13 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, int pass) {
       nx = 0;
       ny = 0;
     }
-    sx = (R(1) - nx * nx) * dxs - nx * ny * dys;
-    sy = -ny * nx * dxs + (R(1) - ny * ny) * dys;
+    sxv = (R(1) - nx * nx) * dxs - nx * ny * dys;
+    syv = -ny * nx * dxs + (R(1) - ny * ny) * dys;
   }
   d.f_cls[i] = (int8_t)cls;
-  sx *= umax;
-  sy *= umax;
-  const double x = A.x[i] + (double)sx, y = A.y[i] + (double)sy;
+  sxv *= umax;
+  syv *= umax;
+  const double x = x0 + (double)sxv, y = y0 + (double)syv;
   const auto &B = d.fb;
   B.x[i] = x;
   B.y[i] = y;
@@ -771,8 +1175,12 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, int pass) {
     B.vy[i] = view_y<Pol>(y, cy, cp);
   }
   const R *L = d.f_L + 4 * i;
-  A.u[i] += (double)(L[0] * sx + L[1] * sy);
-  A.v[i] += (double)(L[2] * sx + L[3] * sy);
+  A.u[i] += (double)(L[0] * sxv + L[1] * syv);
+  A.v[i] += (double)(L[2] * sxv + L[3] * syv);
+  const double mx = x - x0, my = y - y0;
+  B.cum[i] = (float)((double)A.cum[i] + sqrt(mx * mx + my * my));
+  };
+  if (valid) lane();
 }
 
 // ======================================================================================
@@ -824,14 +1232,17 @@ __global__ void __launch_bounds__(256) k_solid_couple(Dev<Pol> d) {
         if (jy < 0 || jy >= cp.ny) continue;
         const R by = yi - cell_shift_y<Pol>(b, cp);
         const int q = cp.cell_off + jx * cp.ny + jy;
-        const int2 rf = d.f_range[q];
-        for (int s = rf.x + l; s < rf.y; s += kSolidLanes) {
-          const R dx = F.vx[s] - bx, dy = F.vy[s] - by;
-          const R r2 = dx * dx + dy * dy;
-          if (r2 < K.rc2) {
-            const R w = (kexp<R>(-(r2 * hinv2)) - eR) * k2c;
-            S += w;
-            P += F.p[s] * w;
+        int2 run[2];
+        fluid_cell_runs(d, cp, jx, jy, run[0], run[1]);
+        for (int h = 0; h < 2; ++h) {
+          for (int s = run[h].x + l; s < run[h].y; s += kSolidLanes) {
+            const R dx = F.vx[s] - bx, dy = F.vy[s] - by;
+            const R r2 = dx * dx + dy * dy;
+            if (r2 < K.rc2) {
+              const R w = (kexp<R>(-(r2 * hinv2)) - eR) * k2c;
+              S += w;
+              P += F.p[s] * w;
+            }
           }
         }
         const int2 rs = d.s_range[q];
@@ -1026,20 +1437,23 @@ __global__ void __launch_bounds__(256) k_solid_force_update(Dev<Pol> d, int sub)
           const int jy = cy + b;
           if (jy < 0 || jy >= cp.ny) continue;
           const R by = yiv - cell_shift_y<Pol>(b, cp);
-          const int2 rf = d.f_range[cp.cell_off + jx * cp.ny + jy];
-          for (int s = rf.x + l; s < rf.y; s += kSolidLanes) {
-            const R dx = F.vx[s] - bx, dy = F.vy[s] - by;
-            const R r2 = dx * dx + dy * dy;
-            if (!(r2 < K.rc2)) continue;
-            const R w = K.w(r2);
-            const R inv_r = r2 > R(0) ? krsqrt<R>(r2) : R(0);
-            const R rhoj = F.vrho[s];
-            R ps = R(0.5) * (F.p[s] + psi);
-            const R du = ((uiv - F.vu[s]) * dx + (viv - F.vv[s]) * dy) * inv_r;
-            if (du >= R(0)) ps += R(0.5) * rhoj * du * fmin(R(3) * du, d.f_phase[s] ? c1 : c0);
-            const R t = R(-2) * ps * rr_inv * w;
-            cpx += t * dx;
-            cpy += t * dy;
+          int2 run[2];
+          fluid_cell_runs(d, cp, jx, jy, run[0], run[1]);
+          for (int h = 0; h < 2; ++h) {
+            for (int s = run[h].x + l; s < run[h].y; s += kSolidLanes) {
+              const R dx = F.vx[s] - bx, dy = F.vy[s] - by;
+              const R r2 = dx * dx + dy * dy;
+              if (!(r2 < K.rc2)) continue;
+              const R w = K.w(r2);
+              const R inv_r = r2 > R(0) ? krsqrt<R>(r2) : R(0);
+              const R rhoj = F.vrho[s];
+              R ps = R(0.5) * (F.p[s] + psi);
+              const R du = ((uiv - F.vu[s]) * dx + (viv - F.vv[s]) * dy) * inv_r;
+              if (du >= R(0)) ps += R(0.5) * rhoj * du * fmin(R(3) * du, d.f_phase[s] ? c1 : c0);
+              const R t = R(-2) * ps * rr_inv * w;
+              cpx += t * dx;
+              cpy += t * dy;
+            }
           }
         }
       }
@@ -1217,7 +1631,7 @@ __global__ void __launch_bounds__(256) k_neighbors(Dev<Pol> d, int lo, int hi, c
   const int c = case_of(i, d.f_case_off, d.ncase);
   const auto &cp = d.cases[c];
   const auto &A = d.fa;
-  const int pk = d.f_cxy[i], cx = cxy_x(pk), cy = cxy_y(pk);
+  const int pk = d.f_cxy[i], cx = cxy_x(pk) >> 1, cy = cxy_y(pk) >> 1;
   const R xi = A.vx[i], yi = A.vy[i];
   int n = 0;
   int *out = list ? list + offset[i - lo] : nullptr;
@@ -1231,12 +1645,15 @@ __global__ void __launch_bounds__(256) k_neighbors(Dev<Pol> d, int lo, int hi, c
       if (jy < 0 || jy >= cp.ny) continue;
       const R by = yi - cell_shift_y<Pol>(b, cp);
       const int q = cp.cell_off + jx * cp.ny + jy;
-      const int2 rf = d.f_range[q];
-      for (int s = rf.x; s < rf.y; ++s) {
-        const R dx = A.vx[s] - bx, dy = A.vy[s] - by;
-        if (dx * dx + dy * dy < cp.rc2) {
-          if (out && n < lim) out[n] = (0 << 28) | (d.f_orig[s] - cp.f_off);
-          ++n;
+      int2 run[2];
+      fluid_cell_runs(d, cp, jx, jy, run[0], run[1]);
+      for (int h = 0; h < 2; ++h) {
+        for (int s = run[h].x; s < run[h].y; ++s) {
+          const R dx = A.vx[s] - bx, dy = A.vy[s] - by;
+          if (dx * dx + dy * dy < cp.rc2) {
+            if (out && n < lim) out[n] = (0 << 28) | (d.f_orig[s] - cp.f_off);
+            ++n;
+          }
         }
       }
       const int2 rs = d.s_range[q];

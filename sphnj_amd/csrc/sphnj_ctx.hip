@@ -111,6 +111,20 @@ struct Engine : EngineBase {
   std::vector<int> f_off, s_off, w_off;  // ncase + 1
   std::vector<Derived> Dh;
   int ncase = 0, total_cells = 0, S = 1, K = 0;
+  int move_epoch = 0;  // position-updating fluid launches since the last hash
+  // fluid tiles: global sub-columns G, row table size; per tile size 64 / 128 / 256: tile
+  // counts per sub-column, their exclusive scan (+ total), plans, and whether the plans
+  // match the current hash
+  int G = 0;
+  int64_t lb_total = 0;
+  int *col_case = nullptr;
+  int *tile_cnt[3] = {nullptr, nullptr, nullptr}, *tile_off[3] = {nullptr, nullptr, nullptr};
+  TilePlan *plans[3] = {nullptr, nullptr, nullptr};
+  bool plan_ok[3] = {false, false, false};
+  static int bidx(int B) { return B == 64 ? 0 : B == 128 ? 1 : 2; }
+  // tiles of B slots never span sub-columns: at most one partial tile per sub-column
+  int max_tiles(int B) const { return nblk(d.nf, B) + G; }
+  int grid_tiles(int B) const { return max_tiles(B); }
   bool viscid = false;
   RadixSorter fsort, ssort;
   int *f_sorted_keys = nullptr, *s_sorted_keys = nullptr;
@@ -168,6 +182,12 @@ struct Engine : EngineBase {
     c.hinv2 = (R)D.hinv2;
     c.nhl2 = (R)(-D.hinv2 * 1.4426950408889634);
     c.kdc = (R)D.kdc;
+    {
+      const double sc = std::sqrt(D.hinv2 * 1.4426950408889634);
+      c.gscale = (float)sc;
+      c.rc2s = (float)(D.rc2 * D.hinv2 * 1.4426950408889634);
+      c.kdc_s = (float)(D.kdc / sc);
+    }
     c.k2c = (R)D.k2c;
     c.eR = (R)D.eR;
     c.h = (R)D.h;
@@ -204,6 +224,11 @@ struct Engine : EngineBase {
     c.inv_cdy = D.inv_cdy;
     c.cdx_r = (R)D.cdx;
     c.cdy_r = (R)D.cdy;
+    c.rcut_prune = std::sqrt(D.rc2) * (1.0 + 1e-5);
+    c.ssx = (float)(0.5 * D.cdx);
+    c.ssy = (float)(0.5 * D.cdy);
+    c.inv_ssx = (float)(2.0 * D.inv_cdx);
+    c.inv_ssy = (float)(2.0 * D.inv_cdy);
     c.dxmin = D.dxmin;
     c.dymin = D.dymin;
     c.dxmax = D.dxmax;
@@ -247,11 +272,25 @@ struct Engine : EngineBase {
     w_off.assign(ncase + 1, 0);
     hc.resize(ncase);
     total_cells = 0;
+    int64_t cells64 = 0;
+    for (int c = 0; c < ncase; ++c) cells64 += (int64_t)D[c].nx * D[c].ny;
+    if (4 * cells64 >= (int64_t)0x7fffffff || ncase > 32767) {
+      err = "cell grids too large for 32-bit sub-cell keys (4 x cells >= 2^31) or more than 32767 cases";
+      return SPHNJ_E_UNSUPPORTED;
+    }
+    G = 0;
+    lb_total = 0;
+    std::vector<int> colc;
     for (int c = 0; c < ncase; ++c) {
       fill_case(D[c], hc[c]);
       viscid = viscid || D[c].viscid;
       hc[c].cell_off = total_cells;
       total_cells += hc[c].ncell;
+      hc[c].col_off = G;
+      hc[c].lb_off = (int)lb_total;
+      G += 2 * hc[c].nx;
+      lb_total += (int64_t)(2 * hc[c].nx) * (2 * hc[c].ny + 1);
+      colc.insert(colc.end(), 2 * hc[c].nx, c);
       f_off[c + 1] = f_off[c] + (int)St[c].fx.size();
       s_off[c + 1] = s_off[c] + (int)St[c].sx.size();
       w_off[c + 1] = w_off[c] + (int)St[c].wx.size();
@@ -261,6 +300,10 @@ struct Engine : EngineBase {
       hc[c].s_n = s_off[c + 1] - s_off[c];
       hc[c].w_off = w_off[c];
       hc[c].w_n = w_off[c + 1] - w_off[c];
+    }
+    if (lb_total >= (int64_t)0x7fffffff) {
+      err = "fluid row table too large for 32-bit indices";
+      return SPHNJ_E_UNSUPPORTED;
     }
     const int nf = f_off[ncase], ns = s_off[ncase], nw = w_off[ncase], nc = total_cells;
     d.ncase = ncase;
@@ -284,7 +327,8 @@ struct Engine : EngineBase {
     // ---- fluid
     for (FluidBuf<Pol> *B : {&d.fa, &d.fb}) {
       if ((rc = dalloc(&B->x, nf)) || (rc = dalloc(&B->y, nf)) || (rc = dalloc(&B->u, nf)) ||
-          (rc = dalloc(&B->v, nf)) || (rc = dalloc(&B->rho, nf)) || (rc = dalloc(&B->p, nf)))
+          (rc = dalloc(&B->v, nf)) || (rc = dalloc(&B->rho, nf)) || (rc = dalloc(&B->p, nf)) ||
+          (rc = dalloc(&B->cum, nf)))
         return rc;
       if constexpr (Pol::MIXED) {
         if ((rc = dalloc(&B->vx, nf)) || (rc = dalloc(&B->vy, nf)) || (rc = dalloc(&B->vu, nf)) ||
@@ -302,8 +346,17 @@ struct Engine : EngineBase {
         (rc = dalloc(&d.f_cls2, nf)) || (rc = dalloc(&d.f_orig, nf)) || (rc = dalloc(&d.f_cxy, nf)) ||
         (rc = dalloc(&d.f_key_by_orig, nf)) || (rc = dalloc(&d.f_slot_of_orig, nf)) ||
         (rc = dalloc(&d.f_col, nf)) || (rc = dalloc(&d.f_near, nf)) || (rc = dalloc(&d.f_L, 4 * (size_t)nf)) ||
-        (rc = dalloc(&d.f_range, nc)) || (rc = dalloc(&d.umax2, ncase)) || (rc = dalloc(&d.status, 1)))
+        (rc = dalloc(&d.f_srange, 4 * (size_t)nc)) || (rc = dalloc(&d.umax2, ncase)) ||
+        (rc = dalloc(&d.umax_slot, (size_t)ncase * kSpread * kSlotStride)) || (rc = dalloc(&d.status, 1)) ||
+        (rc = dalloc(&d.f_lb, (size_t)lb_total)) || (rc = dalloc(&col_case, G)))
       return rc;
+    if ((rc = h2d(col_case, colc.data(), G))) return rc;
+    for (int b = 0; b < 3; ++b) {
+      const int B = 64 << b;
+      if ((rc = dalloc(&tile_cnt[b], G)) || (rc = dalloc(&tile_off[b], G + 1)) ||
+          (rc = dalloc(&plans[b], max_tiles(B))))
+        return rc;
+    }
     {
       std::vector<double> x, y, u, v, rho;
       std::vector<R> p;
@@ -331,7 +384,7 @@ struct Engine : EngineBase {
         return rc;
       HIPC(hipStreamSynchronize(stream));
     }
-    if ((rc = sorter_alloc(fsort, nf, std::max(nc - 1, 0)))) return rc;
+    if ((rc = sorter_alloc(fsort, nf, std::max(4 * nc - 1, 0)))) return rc;
 
     // ---- solid
     if ((rc = dalloc(&d.sx, ns)) || (rc = dalloc(&d.sy, ns)) || (rc = dalloc(&d.su, ns)) || (rc = dalloc(&d.sv, ns)) ||
@@ -439,12 +492,12 @@ struct Engine : EngineBase {
         HIPC(radix_sort_pairs(ws, keys, &sk, &sv, stream));
         k_permute_wall<Pol><<<nblk(nw, 256), 256, 0, stream>>>(d, sv, x0, y0, bc0);
         LAUNCH_CHECK();
-        k_clear_ranges<<<nblk(nc, 256), 256, 0, stream>>>(d.w_range, nullptr, nc);
+        k_clear_ranges<<<nblk(nc, 256), 256, 0, stream>>>(d.w_range, nc, nullptr, 0);
         k_cell_bounds<<<nblk(nw, 256), 256, 0, stream>>>(sk, nw, d.w_range);
         k_wall_normal<Pol><<<nblk(nw, 256), 256, 0, stream>>>(d);
         LAUNCH_CHECK();
       } else {
-        k_clear_ranges<<<nblk(nc, 256), 256, 0, stream>>>(d.w_range, nullptr, nc);
+        k_clear_ranges<<<nblk(nc, 256), 256, 0, stream>>>(d.w_range, nc, nullptr, 0);
         LAUNCH_CHECK();
       }
       HIPC(hipStreamSynchronize(stream));
@@ -474,7 +527,7 @@ struct Engine : EngineBase {
         LAUNCH_CHECK();
         int *sk, *sv;
         HIPC(radix_sort_pairs(rs, keys, &sk, &sv, stream));
-        k_clear_ranges<<<nblk(nc, 256), 256, 0, stream>>>(range, nullptr, nc);
+        k_clear_ranges<<<nblk(nc, 256), 256, 0, stream>>>(range, nc, nullptr, 0);
         k_cell_bounds<<<nblk(ns, 256), 256, 0, stream>>>(sk, ns, range);
         k_ref_count<Pol><<<nblk(ns, 256), 256, 0, stream>>>(d, cxy, range, sv, rc2, cnt);
         LAUNCH_CHECK();
@@ -511,7 +564,7 @@ struct Engine : EngineBase {
       int *sv;
       HIPC(radix_sort_pairs(fsort, d.f_key_by_orig, &f_sorted_keys, &sv, stream));
       d.f_key = f_sorted_keys;
-      k_permute_fluid<Pol><<<nblk(std::max(nf, ncase), blk_stream), blk_stream, 0, stream>>>(d, sv);
+      k_permute_fluid<Pol><<<nblk(std::max(nf, ncase * kSpread), blk_stream), blk_stream, 0, stream>>>(d, sv);
       LAUNCH_CHECK();
       swap_fluid_all();
       std::swap(d.f_phase, d.f_phase2);
@@ -522,23 +575,48 @@ struct Engine : EngineBase {
       LAUNCH_CHECK();
       HIPC(radix_sort_pairs(ssort, d.s_key, &s_sorted_keys, &d.s_sorted, stream));
     }
-    k_clear_ranges<<<nblk(nc, 256), 256, 0, stream>>>(d.f_range, d.s_range, nc);
+    k_clear_ranges<<<nblk(4 * nc, 256), 256, 0, stream>>>(d.f_srange, 4 * nc, d.s_range, nc);
     LAUNCH_CHECK();
-    if (nf > 0) k_cell_bounds<<<nblk(nf, 256), 256, 0, stream>>>(f_sorted_keys, nf, d.f_range);
+    if (nf > 0) k_cell_bounds<<<nblk(nf, 256), 256, 0, stream>>>(f_sorted_keys, nf, d.f_srange);
     if (ns > 0) k_cell_bounds<<<nblk(ns, 256), 256, 0, stream>>>(s_sorted_keys, ns, d.s_range);
-    k_cell_audit<<<nblk(nc, 256), 256, 0, stream>>>(d.f_range, d.s_range, d.w_range, nc, d.status);
+    k_cell_audit<Pol><<<nblk(nc, 256), 256, 0, stream>>>(d, d.status);
     LAUNCH_CHECK();
+    move_epoch = 0;
+    plan_ok[0] = plan_ok[1] = plan_ok[2] = false;
+    if (nf > 0) {
+      k_subcol_rows<Pol><<<nblk(G, 4), 256, 0, stream>>>(d, G, col_case);
+      LAUNCH_CHECK();
+      int rc;
+      if ((rc = ensure_plans(blk_pair)) || (rc = ensure_plans(blk_shift))) return rc;
+    }
+    return SPHNJ_OK;
+  }
+
+  // Tile plans of size B for the current hash (the row table and tile counts are built by
+  // the CELLS stage).
+  int ensure_plans(int B) {
+    const int b = bidx(B);
+    if (plan_ok[b] || d.nf == 0) return SPHNJ_OK;
+    k_tile_count<Pol><<<nblk(G, 256), 256, 0, stream>>>(d, G, col_case, B, tile_cnt[b]);
+    k_tile_scan<<<1, 1024, 0, stream>>>(tile_cnt[b], G, tile_off[b]);
+    k_tile_plan<Pol><<<nblk(G, 4), 256, 0, stream>>>(d, G, col_case, B, tile_off[b], plans[b]);
+    LAUNCH_CHECK();
+    plan_ok[b] = true;
     return SPHNJ_OK;
   }
 
   template <int B>
   void pair_at() {
+    const int b = bidx(B);
+    const int *nt = tile_off[b] + G;
     if (viscid)
-      k_fluid_pair_update<Pol, true, B><<<nblk(d.nf, B), B, 0, stream>>>(d);
+      k_fluid_pair_update<Pol, true, B><<<grid_tiles(B), B, 0, stream>>>(d, plans[b], nt);
     else
-      k_fluid_pair_update<Pol, false, B><<<nblk(d.nf, B), B, 0, stream>>>(d);
+      k_fluid_pair_update<Pol, false, B><<<grid_tiles(B), B, 0, stream>>>(d, plans[b], nt);
   }
   int launch_pair(int blk) {
+    int rc;
+    if ((rc = ensure_plans(blk))) return rc;
     if (blk == 64)
       pair_at<64>();
     else if (blk == 128)
@@ -548,13 +626,20 @@ struct Engine : EngineBase {
     LAUNCH_CHECK();
     return SPHNJ_OK;
   }
+  template <int B>
+  void shift_at(int pass) {
+    const int b = bidx(B);
+    k_fluid_shift<Pol, B><<<grid_tiles(B), B, 0, stream>>>(d, plans[b], tile_off[b] + G, pass);
+  }
   int launch_shift(int blk, int pass) {
+    int rc;
+    if ((rc = ensure_plans(blk))) return rc;
     if (blk == 64)
-      k_fluid_shift<Pol, 64><<<nblk(d.nf, 64), 64, 0, stream>>>(d, pass);
+      shift_at<64>(pass);
     else if (blk == 128)
-      k_fluid_shift<Pol, 128><<<nblk(d.nf, 128), 128, 0, stream>>>(d, pass);
+      shift_at<128>(pass);
     else
-      k_fluid_shift<Pol, 256><<<nblk(d.nf, 256), 256, 0, stream>>>(d, pass);
+      shift_at<256>(pass);
     LAUNCH_CHECK();
     return SPHNJ_OK;
   }
@@ -642,7 +727,10 @@ struct Engine : EngineBase {
       case SPHNJ_STAGE_FLUID:
         if (nf > 0) {
           const int rc = launch_pair(blk_pair);
+          ++move_epoch;
           if (rc) return rc;
+          k_umax_final<<<ncase, 64, 0, stream>>>(d.umax_slot, d.umax2, ncase);
+          LAUNCH_CHECK();
           swap_fluid_all();
         }
         return SPHNJ_OK;
@@ -665,9 +753,11 @@ struct Engine : EngineBase {
         if (nf > 0) {
           for (int k = 0; k < K; ++k) {
             const int rc = launch_shift(blk_shift, k);
+            ++move_epoch;
             if (rc) return rc;
             std::swap(d.fa.x, d.fb.x);
             std::swap(d.fa.y, d.fb.y);
+            std::swap(d.fa.cum, d.fb.cum);
             std::swap(d.fa.vx, d.fb.vx);
             std::swap(d.fa.vy, d.fb.vy);
           }
@@ -680,6 +770,7 @@ struct Engine : EngineBase {
 
   int launches_of(int st) const {
     if (st == SPHNJ_STAGE_SHIFT) return d.nf > 0 ? K : 0;
+    if (st == SPHNJ_STAGE_FLUID) return d.nf > 0 ? 2 : 0;
     if (st == SPHNJ_STAGE_SOLID) return d.ns > 0 ? 2 * S : 0;
     return 1;
   }
@@ -761,6 +852,7 @@ struct Engine : EngineBase {
       }
       HIPC(hipGraphLaunch(it->second.exec, stream));
       d = it->second.post;
+      move_epoch = (d.nf > 0) ? 1 + K : 0;
     }
     return SPHNJ_OK;
   }
@@ -897,14 +989,17 @@ struct Engine : EngineBase {
     const auto &cp = hc[c];
     int lo, n;
     std::vector<int> k, o;
-    std::vector<int2> rg(cp.ncell);
+    // fluid: one start per sub-cell (4 per cell); solid and wall: one per cell
+    const int nb = species == SPHNJ_FLUID ? 4 * cp.ncell : cp.ncell;
+    std::vector<int2> rg(nb);
     if (species == SPHNJ_FLUID) {
       lo = f_off[c];
       n = f_off[c + 1] - lo;
       k.resize(n);
       o.resize(n);
       if (n && f_sorted_keys && (rc = d2h(k.data(), f_sorted_keys + lo, n))) return rc;
-      if ((rc = d2h(o.data(), d.f_orig + lo, n)) || (rc = d2h(rg.data(), d.f_range + cp.cell_off, cp.ncell))) return rc;
+      if ((rc = d2h(o.data(), d.f_orig + lo, n)) || (rc = d2h(rg.data(), d.f_srange + 4 * (size_t)cp.cell_off, nb)))
+        return rc;
     } else if (species == SPHNJ_SOLID) {
       lo = s_off[c];
       n = s_off[c + 1] - lo;
@@ -932,8 +1027,8 @@ struct Engine : EngineBase {
     }
     if (start) {
       int next = n;
-      start[cp.ncell] = n;
-      for (int q = cp.ncell - 1; q >= 0; --q) {
+      start[nb] = n;
+      for (int q = nb - 1; q >= 0; --q) {
         if (rg[q].y > rg[q].x) next = rg[q].x - lo;
         start[q] = next;
       }

@@ -220,12 +220,15 @@ class ContextBase:
         return v.value
 
     def sorted(self, case: int, species: int):
-        """(keys, original index per sorted slot, cell start table) of the last CELLS stage."""
+        """(keys, original index per sorted slot, start table) of the last CELLS stage.
+
+        Solids and walls are binned by cell (n_cells + 1 starts); fluids by half-cell
+        sub-cell, x-major (4 n_cells + 1 starts)."""
         n = self.cases[case].counts[species]
         nx, ny, _, _ = grid(self.cases[case].params)
         keys = np.zeros(n, np.int32)
         orig = np.zeros(n, np.int32)
-        start = np.zeros(nx * ny + 1, np.int32)
+        start = np.zeros((4 if species == 0 else 1) * nx * ny + 1, np.int32)
         self._call("stage_sorted", case, species, _iptr(keys), _iptr(orig), _iptr(start))
         return keys, orig, start
 

@@ -21,7 +21,10 @@ def two_phase_small():
 
 @pytest.mark.parametrize("precision", ["p64", "m32"])
 def test_block_size_does_not_change_results(gpu, precision):
-    """Every block size stages different tiles but performs the same per-particle sums."""
+    """Every block size stages different tiles but performs the same per-particle sums in the
+    same order: P64 is bit-identical.  The M32 shift kernel re-bases staged positions on the
+    tile's frame, so its gradient differs by float rounding of a tile height (bounded here
+    at 1e-6 relative after 4 steps); the pair kernel and the classes stay identical."""
     c = two_phase_small()
     ref = Simulation(c, precision)
     ref.step(4)
@@ -33,7 +36,10 @@ def test_block_size_does_not_change_results(gpu, precision):
         s.step(4)
         b = s.download()
         for k in ("x", "y", "u", "v", "rho", "p", "boundary_type"):
-            assert np.array_equal(a["fluid"][k], b["fluid"][k]), (blk, k)
+            if precision == "p64" or k == "boundary_type":
+                assert np.array_equal(a["fluid"][k], b["fluid"][k]), (blk, k)
+            else:
+                assert rel_l2(a["fluid"][k], b["fluid"][k]) <= 1e-6, (blk, k, rel_l2(a["fluid"][k], b["fluid"][k]))
 
 
 def test_autotune_keeps_state(gpu):
@@ -49,6 +55,9 @@ def test_autotune_keeps_state(gpu):
     after = s.download()["fluid"]
     for k in ("x", "y", "u", "v", "rho", "p", "boundary_type"):
         assert np.array_equal(before[k], after[k]), k
+    # same tiles as the tuned context, so M32 sums round identically
+    t.set_block(0, rep["pair"]["block"])
+    t.set_block(1, rep["shift"]["block"])
     s.step(3)
     t.step(3)
     a, b = s.download()["fluid"], t.download()["fluid"]

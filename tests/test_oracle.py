@@ -371,12 +371,22 @@ def test_hash_sort_and_cell_starts_match_closed_form(oracle_mod):
     keys, orig, start = o.sorted(0, A.FLUID)
     cell = 3 * 0.026
     nx = math.floor(1.4 / cell)
-    ix = np.trunc((x + 0.2) * (1 / (1.4 / nx))).astype(int)
-    iy = np.trunc((y + 0.2) * (1 / (1.4 / nx))).astype(int)
+    tx = (x + 0.2) * (1 / (1.4 / nx))
+    ty = (y + 0.2) * (1 / (1.4 / nx))
+    ix, iy = np.trunc(tx).astype(int), np.trunc(ty).astype(int)
     k = iy + ix * nx
-    order = np.lexsort((np.arange(400), k))             # (cell, original index)
-    assert np.array_equal(orig, order) and np.array_equal(keys, k[order])
-    assert np.array_equal(start, np.searchsorted(k[order], np.arange(nx * nx + 1), side="left"))
+    hx, hy = np.trunc(2 * tx).astype(int) - 2 * ix, np.trunc(2 * ty).astype(int) - 2 * iy
+    sk = (2 * iy + hy) + (2 * ix + hx) * (2 * nx)       # half-cell grid, x-major
+    order = np.lexsort((np.arange(400), sk))            # (sub-cell, original index)
+    assert np.array_equal(orig, order) and np.array_equal(keys, sk[order])
+    assert np.array_equal(start, np.searchsorted(sk[order], np.arange(4 * nx * nx + 1), side="left"))
+    # a cell's fluids, read in (quadrant, original index) order, are its four sub-cells in
+    # x-major order: the neighbour iteration order within a cell
+    ks = k[order]
+    for cell in np.unique(k)[:20]:
+        quad = 2 * hx + hy
+        want = np.lexsort((np.arange(400)[k == cell], quad[k == cell]))
+        assert np.array_equal(order[ks == cell], np.arange(400)[k == cell][want])
 
 
 def test_neighbour_lists_equal_brute_force(oracle_mod):

@@ -91,6 +91,9 @@ struct Dev {
   R *f_col;
   int8_t *f_near;
   R *f_L;  // 4 per slot
+  // shift displacement accumulated over the passes of a step (2 per slot; velocities and
+  // M32 masters take it on the case's last pass)
+  R *f_sacc;
   int2 *f_srange;  // per global sub-cell (4 per cell): [start, end) slot range, empty (0, 0)
   // per sub-column, 2 ny + 1 entries: first slot of sub-row k or later; the sub-column's end
   int *f_lb;

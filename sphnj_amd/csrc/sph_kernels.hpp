@@ -552,6 +552,15 @@ __global__ void __launch_bounds__(256) k_tile_plan(Dev<Pol> d, int G, const int 
   }
 }
 
+// Blocks are dealt to the 8 XCDs round-robin, and each XCD has its own L2.  Giving XCD x the
+// contiguous tile range [x per, (x + 1) per) makes the tiles an XCD runs at the same time
+// neighbours, so the halo they stage is read from its L2.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_tile(int b, int grid) {
+  const int per = (grid + kXcds - 1) / kXcds;
+  return (b % kXcds) * per + b / kXcds;
+}
+
 // Per-case maximum of non-negative double bit patterns: one atomic per block (a tile lies in
 // one case), into slot blockIdx.x % kSpread of the case.  Every thread must call it.
 template <int BLOCK>
@@ -807,8 +816,9 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
   __shared__ int s_lb[kTileCols][TileRows<BLOCK>::value + 1];
   __shared__ unsigned long long s_red[BLOCK / 64];
   __shared__ float s_dmax[BLOCK / 64];
-  if ((int)blockIdx.x >= *ntiles) return;
-  const TilePlan &P = plans[blockIdx.x];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  if (tile >= *ntiles) return;
+  const TilePlan &P = plans[tile];
   const int t = threadIdx.x, i = P.start + t, c = P.c;
   const bool valid = t < P.count;
   const auto &A = d.fa;
@@ -950,6 +960,18 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
     }
     d.f_col[i] = S.col;
     d.f_near[i] = (int8_t)S.near;
+    // the fluid class of this step (correctionUpdate, src/fluid.cu:369-427), a function of
+    // phase, colour and near flag only; it is set only where the case shifts
+    int cls = d.f_cls[i];
+    if (cp.K > 0) {
+      if (phi == 1)
+        cls = 3;  // gas: INNER
+      else if (S.col < R(1.5))
+        cls = S.near == 0 ? 0 : 1;  // SPLASH / FREE_SURFACE
+      else
+        cls = (S.col < R(2.0) && S.near == 2) ? 2 : 3;  // VICINITY / INNER
+    }
+    d.f_cls2[i] = (int8_t)cls;
     d.f_L[4 * i + 0] = S.L00;
     d.f_L[4 * i + 1] = S.L01;
     d.f_L[4 * i + 2] = S.L10;
@@ -1000,10 +1022,13 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
 
 // correctionCalculate (src/fluid.cu:316-367) + correctionUpdate / shiftDisplacementCalculate
 // (src/fluid.cu:369-427), one tile per block.  The gradient sums over all species with no
-// phase filter.  For M32 the staged fluid positions are re-based on the tile frame (the
-// tile's cell column, and the cell row cy_ref), so each sub-column window is one run; the
-// frame costs at most a few float ulps of a tile height in the gradient of a small
-// correction.
+// phase filter; the class comes from the pair kernel.  A pass moves the positions the
+// neighbour loops read (the views, and for P64 the positions themselves) and accumulates the
+// step's shift per particle; the case's last pass applies the accumulated shift to the
+// velocities (u += L s is linear in s) and to the M32 masters.  For M32 the staged fluid
+// positions are re-based on the tile frame (the tile's cell column, and the cell row
+// cy_ref), in units scaled so that the weight is 2^-r'^2; the frame costs at most a few
+// float ulps of a tile height in the gradient of a small correction.
 template <class Pol, int BLOCK>
 __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePlan *plans, const int *ntiles,
                                                        int pass) {
@@ -1013,13 +1038,31 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
   __shared__ R2 s_pos[CAP + 4];  // + far-away padding read by the masked tail of a window
   __shared__ int s_lb[kTileCols][TileRows<BLOCK>::value + 1];
   __shared__ float s_dmax[BLOCK / 64];
-  if ((int)blockIdx.x >= *ntiles) return;
-  const TilePlan &P = plans[blockIdx.x];
+  const int tile = xcd_tile(blockIdx.x, gridDim.x);
+  if (tile >= *ntiles) return;
+  const TilePlan &P = plans[tile];
   const int t = threadIdx.x, i = P.start + t, c = P.c;
   const bool valid = t < P.count;
   const auto &A = d.fa;
+  const auto &B = d.fb;
   const auto &cp = d.cases[c];
   const bool staged = P.total <= CAP && P.nr <= TileRows<BLOCK>::value;
+  const bool active = pass < cp.K, last = pass == cp.K - 1;
+  // the lane's own state, loaded ahead of the staging so that the loads overlap it
+  int pk = 0, cls = 3;
+  double x0 = 0.0, y0 = 0.0;
+  R2 sacc{R(0), R(0)};
+  float cum0 = 0.f;
+  if (valid) {
+    pk = d.f_cxy[i];
+    x0 = A.x[i];
+    y0 = A.y[i];
+    cum0 = A.cum[i];
+    if (active) {
+      cls = d.f_cls[i];
+      if (pass > 0) sacc = R2{d.f_sacc[2 * i], d.f_sacc[2 * i + 1]};
+    }
+  }
   float dmax = 0.f;
   if (staged) {
     struct Ld {
@@ -1044,21 +1087,18 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
   dmax = wave_max_f(dmax);
   if ((t & 63) == 0) s_dmax[t >> 6] = dmax;
   __syncthreads();
-  auto lane = [&]() {
-  if (pass >= cp.K) {  // this case has fewer passes than the batch: carry positions over
-    d.fb.x[i] = A.x[i];
-    d.fb.y[i] = A.y[i];
-    d.fb.cum[i] = A.cum[i];
-    if constexpr (Pol::MIXED) {
-      d.fb.vx[i] = A.vx[i];
-      d.fb.vy[i] = A.vy[i];
-    }
+  if (!valid) return;
+  if (!active) {  // this case has fewer passes than the batch: carry the moving fields over
+    B.vx[i] = A.vx[i];
+    B.vy[i] = A.vy[i];
+    B.cum[i] = cum0;
     return;
   }
-  const int pk = d.f_cxy[i], sy = cxy_y(pk), cx = P.cx, cy = sy >> 1;
-  const R xi = A.vx[i], yi = A.vy[i];
-  const double x0 = A.x[i], y0 = A.y[i];
-  const LanePos lp = lane_pos(x0, y0, P.sx, sy, cp);
+  const int sy = cxy_y(pk), cx = P.cx, cy = sy >> 1;
+  // current position: the M32 masters stay at the pair kernel's result until the last pass
+  const double xc = Pol::MIXED ? x0 + (double)sacc.x : x0, yc = Pol::MIXED ? y0 + (double)sacc.y : y0;
+  const R xi = view_x<Pol>(xc, cx, cp), yi = view_y<Pol>(yc, cy, cp);  // = the stored views
+  const LanePos lp = lane_pos(xc, yc, P.sx, sy, cp);
   const LoopConsts<R> K(cp);
   R gx = 0, gy = 0;
   auto add = [&](R dx, R dy) {
@@ -1066,28 +1106,34 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
     gx += dx * w;
     gy += dy * w;
   };
-  // the lane in the tile frame
-  const R yg = Pol::MIXED ? yi + (R)(cy - P.cy_ref) * cp.cdy_r : yi;
   const float Rp = prune_radius(cp, staged, staged_bound<BLOCK>(s_dmax));
   const int *lbc = d.f_lb + cp.lb_off;
   const int ny21 = 2 * cp.ny + 1;
-  // staged fluid in scaled units: w = 2^-r'^2 inside the cut-off; sums rescaled below
+  // staged fluid: M32 in the scaled tile frame (w = 2^-r'^2 inside the cut-off, sums
+  // rescaled below), the lane's own entry giving its position there
   R gxs = 0, gys = 0;
-  const R xs = xi * cp.gscale, ys = yg * cp.gscale, rc2s = cp.rc2s;
+  R xs = xi, ys = yi;
+  if (staged) {
+    int own = i - P.lo[P.sx - P.sc0];
+    for (int a = 0; a < P.sx - P.sc0; ++a) own += P.len[a];
+    const R2 o = s_pos[own];
+    xs = o.x;
+    ys = o.y;
+  }
+  const R rc2s = cp.rc2s;
   for_windows(P, cp, sy, cy, lp, Rp, [&](int sc, int ra, int rb) {
     if (staged) {
       const int sa = sc - P.sc0;
       for_range_masked<4>(s_lb[sa][ra - P.r0], s_lb[sa][rb + 1 - P.r0], [&](int l, bool in) {
         const R2 pj = s_pos[l];
+        const R dx = pj.x - xs, dy = pj.y - ys, r2 = dx * dx + dy * dy;
         if constexpr (Pol::MIXED) {
-          const R dx = pj.x - xs, dy = pj.y - ys, r2 = dx * dx + dy * dy;
           const R e = __builtin_amdgcn_exp2f(-r2);
           const R w = (in & (r2 < rc2s)) ? e : R(0);
           gxs += dx * w;
           gys += dy * w;
         } else {
-          const R dx = pj.x - xi, dy = pj.y - yg;
-          const R wk = K.w(dx * dx + dy * dy);
+          const R wk = K.w(r2);
           const R w = in ? wk : R(0);
           gx += dx * w;
           gy += dy * w;
@@ -1127,28 +1173,10 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
   const R umax = fmin((R)sqrt(__longlong_as_double((long long)d.umax2[c])), cp.umax_cap);
   const R dxs = -cp.shift_k * gx, dys = -cp.shift_k * gy;
   R sxv = dxs, syv = dys;
-  int cls;
-  bool project = false;
-  const R col = d.f_col[i];
-  const int near = d.f_near[i];
-  if (d.f_phase[i] == 1) {
-    cls = 3;  // gas: INNER
-  } else if (col < R(1.5)) {
-    if (near == 0) {
-      cls = 0;  // SPLASH
-      sxv = 0;
-      syv = 0;
-    } else {
-      cls = 1;  // FREE_SURFACE
-      project = true;
-    }
-  } else if (col < R(2.0) && near == 2) {
-    cls = 2;  // VICINITY
-    project = true;
-  } else {
-    cls = 3;  // INNER
-  }
-  if (project) {
+  if (cls == 0) {  // SPLASH: no shift
+    sxv = 0;
+    syv = 0;
+  } else if (cls == 1 || cls == 2) {  // FREE_SURFACE / VICINITY: tangential part only
     R nx = 0, ny = 0;
     const R g2 = gx * gx + gy * gy;
     if (g2 > R(0)) {
@@ -1163,24 +1191,35 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
     sxv = (R(1) - nx * nx) * dxs - nx * ny * dys;
     syv = -ny * nx * dxs + (R(1) - ny * ny) * dys;
   }
-  d.f_cls[i] = (int8_t)cls;
   sxv *= umax;
   syv *= umax;
-  const double x = x0 + (double)sxv, y = y0 + (double)syv;
-  const auto &B = d.fb;
-  B.x[i] = x;
-  B.y[i] = y;
+  const R2 sn{sacc.x + sxv, sacc.y + syv};
+  double x, y;
   if constexpr (Pol::MIXED) {
+    x = x0 + (double)sn.x;
+    y = y0 + (double)sn.y;
     B.vx[i] = view_x<Pol>(x, cx, cp);
     B.vy[i] = view_y<Pol>(y, cy, cp);
+  } else {
+    x = x0 + sxv;
+    y = y0 + syv;
+    B.x[i] = x;
+    B.y[i] = y;
   }
-  const R *L = d.f_L + 4 * i;
-  A.u[i] += (double)(L[0] * sxv + L[1] * syv);
-  A.v[i] += (double)(L[2] * sxv + L[3] * syv);
-  const double mx = x - x0, my = y - y0;
-  B.cum[i] = (float)((double)A.cum[i] + sqrt(mx * mx + my * my));
-  };
-  if (valid) lane();
+  const double mx = x - xc, my = y - yc;
+  B.cum[i] = (float)((double)cum0 + sqrt(mx * mx + my * my));
+  if (last) {
+    if constexpr (Pol::MIXED) {
+      A.x[i] = x;
+      A.y[i] = y;
+    }
+    const R *L = d.f_L + 4 * i;
+    A.u[i] += (double)(L[0] * sn.x + L[1] * sn.y);
+    A.v[i] += (double)(L[2] * sn.x + L[3] * sn.y);
+  } else {
+    d.f_sacc[2 * i] = sn.x;
+    d.f_sacc[2 * i + 1] = sn.y;
+  }
 }
 
 // ======================================================================================

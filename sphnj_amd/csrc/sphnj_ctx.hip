@@ -124,7 +124,7 @@ struct Engine : EngineBase {
   static int bidx(int B) { return B == 64 ? 0 : B == 128 ? 1 : 2; }
   // tiles of B slots never span sub-columns: at most one partial tile per sub-column
   int max_tiles(int B) const { return nblk(d.nf, B) + G; }
-  int grid_tiles(int B) const { return max_tiles(B); }
+  int grid_tiles(int B) const { return (max_tiles(B) + kXcds - 1) / kXcds * kXcds; }
   bool viscid = false;
   RadixSorter fsort, ssort;
   int *f_sorted_keys = nullptr, *s_sorted_keys = nullptr;
@@ -346,6 +346,7 @@ struct Engine : EngineBase {
         (rc = dalloc(&d.f_cls2, nf)) || (rc = dalloc(&d.f_orig, nf)) || (rc = dalloc(&d.f_cxy, nf)) ||
         (rc = dalloc(&d.f_key_by_orig, nf)) || (rc = dalloc(&d.f_slot_of_orig, nf)) ||
         (rc = dalloc(&d.f_col, nf)) || (rc = dalloc(&d.f_near, nf)) || (rc = dalloc(&d.f_L, 4 * (size_t)nf)) ||
+        (rc = dalloc(&d.f_sacc, 2 * (size_t)nf)) ||
         (rc = dalloc(&d.f_srange, 4 * (size_t)nc)) || (rc = dalloc(&d.umax2, ncase)) ||
         (rc = dalloc(&d.umax_slot, (size_t)ncase * kSpread * kSlotStride)) || (rc = dalloc(&d.status, 1)) ||
         (rc = dalloc(&d.f_lb, (size_t)lb_total)) || (rc = dalloc(&col_case, G)))
@@ -646,17 +647,21 @@ struct Engine : EngineBase {
 
   // Autotuner: times the pair and shift kernels at each candidate block size (= tile size in
   // slots) on the current state with HIP events and keeps the fastest.  The pair kernel only
-  // writes the alternate buffers and the per-step transients; the shift kernel also updates
-  // u, v and the fluid class in place, which are saved before and restored after.
+  // writes the alternate buffers and the per-step transients; a case's last shift pass also
+  // updates u, v and (M32) x, y in place, which are saved before and restored after.
   int autotune() override {
     if (d.nf == 0) return SPHNJ_OK;
     const int nf = d.nf;
-    double *su = nullptr, *sv = nullptr;
+    double *su = nullptr, *sv = nullptr, *sx_ = nullptr, *sy_ = nullptr;
     int8_t *scls = nullptr;
     int rc;
-    if ((rc = dalloc(&su, nf)) || (rc = dalloc(&sv, nf)) || (rc = dalloc(&scls, nf))) return rc;
+    if ((rc = dalloc(&su, nf)) || (rc = dalloc(&sv, nf)) || (rc = dalloc(&scls, nf)) || (rc = dalloc(&sx_, nf)) ||
+        (rc = dalloc(&sy_, nf)))
+      return rc;
     HIPC(hipMemcpyAsync(su, d.fa.u, nf * sizeof(double), hipMemcpyDeviceToDevice, stream));
     HIPC(hipMemcpyAsync(sv, d.fa.v, nf * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    HIPC(hipMemcpyAsync(sx_, d.fa.x, nf * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    HIPC(hipMemcpyAsync(sy_, d.fa.y, nf * sizeof(double), hipMemcpyDeviceToDevice, stream));
     HIPC(hipMemcpyAsync(scls, d.f_cls, nf, hipMemcpyDeviceToDevice, stream));
     hipEvent_t a = take_event(), b = take_event();
     if (!a || !b) {
@@ -697,9 +702,11 @@ struct Engine : EngineBase {
     ev_pool.push_back(b);
     HIPC(hipMemcpyAsync(d.fa.u, su, nf * sizeof(double), hipMemcpyDeviceToDevice, stream));
     HIPC(hipMemcpyAsync(d.fa.v, sv, nf * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    HIPC(hipMemcpyAsync(d.fa.x, sx_, nf * sizeof(double), hipMemcpyDeviceToDevice, stream));
+    HIPC(hipMemcpyAsync(d.fa.y, sy_, nf * sizeof(double), hipMemcpyDeviceToDevice, stream));
     HIPC(hipMemcpyAsync(d.f_cls, scls, nf, hipMemcpyDeviceToDevice, stream));
     HIPC(hipStreamSynchronize(stream));
-    for (void *q : {(void *)su, (void *)sv, (void *)scls}) {
+    for (void *q : {(void *)su, (void *)sv, (void *)scls, (void *)sx_, (void *)sy_}) {
       (void)hipFree(q);
       allocs.erase(std::remove(allocs.begin(), allocs.end(), q), allocs.end());
     }
@@ -732,6 +739,7 @@ struct Engine : EngineBase {
           k_umax_final<<<ncase, 64, 0, stream>>>(d.umax_slot, d.umax2, ncase);
           LAUNCH_CHECK();
           swap_fluid_all();
+          std::swap(d.f_cls, d.f_cls2);  // the class of this step
         }
         return SPHNJ_OK;
       case SPHNJ_STAGE_SOLID:
@@ -755,11 +763,14 @@ struct Engine : EngineBase {
             const int rc = launch_shift(blk_shift, k);
             ++move_epoch;
             if (rc) return rc;
-            std::swap(d.fa.x, d.fb.x);
-            std::swap(d.fa.y, d.fb.y);
-            std::swap(d.fa.cum, d.fb.cum);
+            // views and path lengths ping-pong (P64 views are the positions)
             std::swap(d.fa.vx, d.fb.vx);
             std::swap(d.fa.vy, d.fb.vy);
+            std::swap(d.fa.cum, d.fb.cum);
+            if (!Pol::MIXED) {
+              std::swap(d.fa.x, d.fb.x);
+              std::swap(d.fa.y, d.fb.y);
+            }
           }
         }
         return SPHNJ_OK;

@@ -588,69 +588,27 @@ __global__ void __launch_bounds__(64) k_umax_final(const unsigned long long *slo
 
 // Copies a tile into LDS: the row tables (lb[a][k] = LDS index of the first candidate of row
 // r0 + k or later of sub-column sc0 + a, k = 0..nr) and, through ld / st, the candidates:
-// element e of the concatenated spans (LDS index e) is slot lo[a] + e - base[a] of its span a.
-// Each thread issues the loads of U elements before storing any.
+// span a occupies LDS [base[a], base[a] + len[a]) in slot order.  Each span is served by a
+// contiguous group of threads (its rows and its particles, coalesced), so every element's
+// span is known without a search; the loads of U particles are issued before any is
+// stored.  st(l, v, a) receives the span too.
 template <int BLOCK, int U, class Ld, class St>
 __device__ __forceinline__ void tile_stage(const TilePlan &P, const int *f_lb, int (*lb)[TileRows<BLOCK>::value + 1],
                                            Ld &&ld, St &&st) {
-  const int t = threadIdx.x, n = P.nsc, nr1 = P.nr + 1;
-  int base[kTileCols], lo[kTileCols], row[kTileCols];
-  {
-    int b = 0;
-#pragma unroll
-    for (int a = 0; a < kTileCols; ++a) {
-      base[a] = a < n ? b : 0x7fffffff;
-      lo[a] = P.lo[a];
-      row[a] = P.row[a];
-      b += a < n ? P.len[a] : 0;
-    }
-  }
-  for (int e0 = t; e0 < n * nr1; e0 += 2 * BLOCK) {
-    int v[2], dst[2];
-#pragma unroll
-    for (int u = 0; u < 2; ++u) {
-      const int e = e0 + u * BLOCK;
-      dst[u] = -1;
-      if (e < n * nr1) {
-        const int a = e / nr1, k = e - a * nr1;
-        int b = 0, l = 0, r = 0;
-#pragma unroll
-        for (int q = 0; q < kTileCols; ++q)
-          if (q == a) {
-            b = base[q];
-            l = lo[q];
-            r = row[q];
-          }
-        v[u] = f_lb[r + k] - l + b;
-        dst[u] = a * (TileRows<BLOCK>::value + 1) + k;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < 2; ++u)
-      if (dst[u] >= 0) (&lb[0][0])[dst[u]] = v[u];
-  }
-  const int total = P.total;
-  for (int e0 = t; e0 < total; e0 += U * BLOCK) {
+  const int t = threadIdx.x, n = P.nsc, nr = P.nr, gsz = BLOCK / n;
+  const int a = min(t / gsz, n - 1), k0 = t - a * gsz, stride = a == n - 1 ? BLOCK - a * gsz : gsz;
+  int base = 0;
+  for (int q = 0; q < a; ++q) base += P.len[q];
+  const int lo = P.lo[a], len = P.len[a], row = P.row[a];
+  for (int k = k0; k <= nr; k += stride) lb[a][k] = f_lb[row + k] - lo + base;
+  for (int k = k0; k < len; k += U * stride) {
     decltype(ld(0)) v[U];
 #pragma unroll
-    for (int k = 0; k < U; ++k) {
-      const int e = e0 + k * BLOCK;
-      if (e < total) {
-        int b = base[0], l = lo[0];
+    for (int u = 0; u < U; ++u)
+      if (k + u * stride < len) v[u] = ld(lo + k + u * stride);
 #pragma unroll
-        for (int a = 1; a < kTileCols; ++a) {
-          const bool in = e >= base[a];
-          b = in ? base[a] : b;
-          l = in ? lo[a] : l;
-        }
-        v[k] = ld(l + (e - b));
-      }
-    }
-#pragma unroll
-    for (int k = 0; k < U; ++k) {
-      const int e = e0 + k * BLOCK;
-      if (e < total) st(e, v[k]);
-    }
+    for (int u = 0; u < U; ++u)
+      if (k + u * stride < len) st(base + k + u * stride, v[u], a);
   }
 }
 
@@ -684,6 +642,23 @@ __device__ __forceinline__ void for_windows(const TilePlan &P, const CP &cp, int
     const int rb = min(sy + (int)floorf(lp.fy + hr), rmax);
     if (ra <= rb) f(sx + a, ra, rb);
   }
+}
+
+// The row window [ra, rb] of a lane in candidate sub-column sc0 + sa (for_windows for one
+// sub-column); false when the sub-column holds no candidate of the lane.
+template <class CP>
+__device__ __forceinline__ bool span_window(const TilePlan &P, const CP &cp, int sa, int sy, int cy, LanePos lp,
+                                            float R, int &ra, int &rb) {
+  if (sa >= P.nsc) return false;
+  const int a = P.sc0 + sa - P.sx;
+  const float R2 = R * R;
+  if (a < (int)floorf(lp.fx - R * cp.inv_ssx) || a > (int)floorf(lp.fx + R * cp.inv_ssx)) return false;
+  const float g = fmaxf(fmaxf((float)a - lp.fx, lp.fx - (float)(a + 1)), 0.f) * cp.ssx;
+  if (g >= R) return false;
+  const float hr = fminf(__builtin_amdgcn_sqrtf(R2 - g * g) * cp.inv_ssy, 8.f);
+  ra = max(sy + (int)floorf(lp.fy - hr), max(2 * cy - 2, P.r0));
+  rb = min(sy + (int)floorf(lp.fy + hr), min(2 * cy + 3, P.r0 + P.nr - 1));
+  return ra <= rb;
 }
 
 // Pruning radius: rc (1 + 1e-5) plus the tile's displacement bound D; a tile that is not
@@ -840,7 +815,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
           return Ld{R2{A.vx[g], A.vy[g]}, R2{A.vu[g], A.vv[g]}, R2{A.vrho[g], A.p[g]},
                     (int)d.f_phase[g] | ((int)d.f_cls[g] << 8), A.cum[g], Pol::MIXED ? d.f_cxy[g] : 0};
         },
-        [&](int l, const Ld &v) {
+        [&](int l, const Ld &v, int) {
           if constexpr (Pol::MIXED)  // tile frame
             s_pos[l] = R2{v.pos.x + (float)((cxy_x(v.pk) >> 1) - gcx) * fdx,
                           v.pos.y + (float)((cxy_y(v.pk) >> 1) - gcy) * fdy};
@@ -907,16 +882,50 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
     const float Rp = prune_radius(cp, staged, staged_bound<BLOCK>(s_dmax));
     // the lane in the tile frame (M32 staged positions are re-based on it)
     const R yg = Pol::MIXED ? yi + (R)(cy - P.cy_ref) * cp.cdy_r : yi;
-    for_windows(P, cp, sy, cy, lp, Rp, [&](int sc, int ra, int rb) {
-      if (staged) {  // one run per window
-        const int sa = sc - P.sc0;
-        for_range_masked<2>(s_lb[sa][ra - P.r0], s_lb[sa][rb + 1 - P.r0], [&](int l, bool in) {
-          const R2 pj = s_pos[l], vj = s_vel[l], rp = s_rp[l];
+    auto staged_j = [&](int l, bool in) {
+      const R2 pj = s_pos[l], vj = s_vel[l], rp = s_rp[l];
+      const R dx = pj.x - xi, dy = pj.y - yg;
+      fluid_j(l == own, dx, dy, dx * dx + dy * dy, vj.x, vj.y, rp.x, rp.y, s_fl[l], in);
+    };
+    if (staged) {
+      // staged windows (one run each): first mark the candidates inside the cut-off (64-bit
+      // mask per window), then evaluate the Riemann terms for those only, in slot order --
+      // the same non-zero terms in the same order as a pass over every candidate
+      unsigned long long mk[kTileCols];
+      int w0[kTileCols];
+#pragma unroll
+      for (int sa = 0; sa < kTileCols; ++sa) {
+        mk[sa] = 0ull;
+        w0[sa] = 0;
+        int ra, rb;
+        if (!span_window(P, cp, sa, sy, cy, lp, Rp, ra, rb)) continue;
+        const int s0 = s_lb[sa][ra - P.r0], s1 = s_lb[sa][rb + 1 - P.r0];
+        w0[sa] = s0;
+        if (s1 - s0 > 64) {  // a long window: evaluate every candidate
+          for_range_masked<2>(s0, s1, staged_j);
+          continue;
+        }
+        unsigned long long m = 0ull;
+        for_range_masked<4>(s0, s1, [&](int l, bool in) {
+          const R2 pj = s_pos[l];
           const R dx = pj.x - xi, dy = pj.y - yg;
-          fluid_j(l == own, dx, dy, dx * dx + dy * dy, vj.x, vj.y, rp.x, rp.y, s_fl[l], in);
+          const bool ok = in & (dx * dx + dy * dy < K.rc2);
+          m |= (ok ? 1ull : 0ull) << ((l - s0) & 63);
         });
-        return;
+        mk[sa] = m;
       }
+#pragma unroll
+      for (int sa = 0; sa < kTileCols; ++sa) {
+        unsigned long long m = mk[sa];
+        while (m) {
+          const int b = __builtin_ctzll(m);
+          m &= m - 1ull;
+          staged_j(w0[sa] + b, true);
+        }
+      }
+    }
+    for_windows(P, cp, sy, cy, lp, Rp, [&](int sc, int ra, int rb) {
+      if (staged) return;
       const R bx = xi - cell_shift_x<Pol>((sc >> 1) - cx, cp);
       if constexpr (Pol::MIXED) {
         // M32 views are relative to their own cell: one run per cell row
@@ -1070,14 +1079,16 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
       int pk;
       float cum;
     };
-    const R fdx = cp.cdx_r, fdy = cp.cdy_r, gs = cp.gscale;
-    const int gcx = P.cx, gcy = P.cy_ref;
+    const R fdy = cp.cdy_r, gs = cp.gscale;
+    // this thread's span lies in one cell column: its x offset in the tile frame, scaled
+    const int sa = min((int)threadIdx.x / (BLOCK / P.nsc), P.nsc - 1);
+    const R xo = (R)(((P.sc0 + sa) >> 1) - P.cx) * cp.cdx_r * gs;
+    const int gcy = P.cy_ref;
     tile_stage<BLOCK, 4>(
         P, d.f_lb, s_lb, [&](int g) { return Ld{R2{A.vx[g], A.vy[g]}, Pol::MIXED ? d.f_cxy[g] : 0, A.cum[g]}; },
-        [&](int l, const Ld &v) {
+        [&](int l, const Ld &v, int) {
           if constexpr (Pol::MIXED)  // tile frame, scaled units
-            s_pos[l] = R2{(v.pos.x + (float)((cxy_x(v.pk) >> 1) - gcx) * fdx) * gs,
-                          (v.pos.y + (float)((cxy_y(v.pk) >> 1) - gcy) * fdy) * gs};
+            s_pos[l] = R2{fmaf(v.pos.x, gs, xo), (v.pos.y + (float)(((v.pk & 0xffff) >> 1) - gcy) * fdy) * gs};
           else
             s_pos[l] = v.pos;
           dmax = fmaxf(dmax, v.cum);

@@ -47,6 +47,7 @@ struct CaseDev {
   double dxmin, dymin, dxmax, dymax;
   int nx, ny, cell_off, ncell;
   int col_off, lb_off;  // first global sub-column; offset in the fluid row table f_lb
+  int swp_off;          // offset of the case's cell columns in f_swp
   int steady, viscid;
   int K, S;  // shift passes and solid sub-steps of this case
   // solid (SolidProperties, src/properties.cpp:69-89,118-161)
@@ -97,6 +98,8 @@ struct Dev {
   int2 *f_srange;  // per global sub-cell (4 per cell): [start, end) slot range, empty (0, 0)
   // per sub-column, 2 ny + 1 entries: first slot of sub-row k or later; the sub-column's end
   int *f_lb;
+  // per cell column, ny + 1 entries: cells among rows [0, k) that hold a solid or wall particle
+  int *f_swp;
   // max |u|^2 per case as the bit pattern of a non-negative double: blocks reduce into
   // kSpread slots per case (slot s at umax_slot[(c kSpread + s) kSlotStride], one cache line
   // each, so no address takes more than a few hundred atomics); umax2[c] is their maximum
@@ -224,6 +227,20 @@ __device__ __forceinline__ float wave_max_f(float v) {
 
 __device__ __forceinline__ unsigned wave_or(unsigned v) {
   for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ int wave_sum(int v) {
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// inclusive prefix sum over the lanes of a wave
+__device__ __forceinline__ int wave_incl_sum(int v, int lane) {
+  for (int o = 1; o < 64; o <<= 1) {
+    const int u = __shfl_up(v, o, 64);
+    if (lane >= o) v += u;
+  }
   return v;
 }
 

@@ -371,8 +371,10 @@ __global__ void __launch_bounds__(256) k_wall_update(Dev<Pol> d) {
 // ======================================================================================
 // Tiles over the half-cell grid
 // ======================================================================================
-// Once per step (after the hash), the fluid slots are cut into tiles of at most BLOCK
-// consecutive slots of one sub-column of one case, and every tile gets a plan: the
+// Once per step (after the hash), the fluid slots of every sub-column are cut into chunks
+// of BLOCK consecutive slots, and a chunk whose slots span more than plan_span(BLOCK)
+// sub-rows (a sparse sub-column, whose neighbours would overflow the LDS budget) is split
+// into kMaxPieces tiles of equal slot counts.  Every tile gets a plan: the
 // candidate sub-columns [sc0, sc0 + nsc) and sub-rows [r0, r0 + nr) of its lanes -- the 3x3
 // cells of the tile's cell column over its rows -- as slot spans, and whether any solid or
 // wall particle lies in those cells.  The row table f_lb gives, per sub-column, the first
@@ -395,6 +397,17 @@ constexpr int kTileCols = 6;  // sub-columns of the 3x3 cells of a sub-column's 
 // either side) over about B / 3.75 + 6 rows; a tile beyond the budget reads global memory.
 __host__ __device__ constexpr int plan_cap(int B) { return 7 * B + 256; }
 __host__ __device__ constexpr int plan_rows(int B) { return B / 2 + 32; }
+// Sub-rows a tile of B slots may span: a dense sub-column holds about 3.9 slots per sub-row,
+// so a full chunk spans about B / 3.9; a tile within the span has at most about
+// 6 (plan_span + 7) 3.9 candidates, inside plan_cap.
+__host__ __device__ constexpr int plan_span(int B) { return 9 * B / 32 + 2; }
+constexpr int kMaxPieces = 2;
+
+// Tiles of the chunk [start, start + count) of one sub-column.
+__device__ __forceinline__ int chunk_pieces(const int *f_cxy, int start, int count, int B) {
+  const int rows = cxy_y(f_cxy[start + count - 1]) - cxy_y(f_cxy[start]) + 1;
+  return min(min((rows + plan_span(B) - 1) / plan_span(B), kMaxPieces), count);
+}
 template <int BLOCK>
 struct TileRows {
   static constexpr int value = plan_rows(BLOCK);
@@ -438,6 +451,23 @@ __global__ void __launch_bounds__(256) k_subcol_rows(Dev<Pol> d, int G, const in
     if (r.y > r.x) end = max(end, r.y);
   }
   for (int o = 32; o > 0; o >>= 1) end = max(end, __shfl_xor(end, o, 64));
+  if ((sx & 1) == 0) {  // the wave of an even sub-column also builds its cell column's f_swp
+    const int qc = cp.cell_off + (sx >> 1) * cp.ny;
+    int *swp = d.f_swp + cp.swp_off + (sx >> 1) * (cp.ny + 1);
+    int run = 0;
+    for (int c0 = 0; c0 < cp.ny; c0 += 64) {
+      const int k = c0 + lane;
+      int v = 0;
+      if (k < cp.ny) {
+        const int2 a = d.s_range[qc + k], w = d.w_range[qc + k];
+        v = (a.y > a.x || w.y > w.x) ? 1 : 0;
+      }
+      v = wave_incl_sum(v, lane) + run;
+      if (k < cp.ny) swp[k + 1] = v;
+      run = __shfl(v, 63, 64);
+    }
+    if (lane == 0) swp[0] = 0;
+  }
   int carry = end;
   for (int c0 = ((ny2 - 1) >> 6) << 6; c0 >= 0; c0 -= 64) {
     const int k = c0 + lane;
@@ -457,15 +487,19 @@ __global__ void __launch_bounds__(256) k_subcol_rows(Dev<Pol> d, int G, const in
   if (lane == 0) lb[ny2] = end;
 }
 
-// Tiles of B slots per sub-column (one thread per global sub-column).
+// Tiles per sub-column (one wave per global sub-column, a lane per chunk).
 template <class Pol>
 __global__ void __launch_bounds__(256) k_tile_count(Dev<Pol> d, int G, const int *col_case, int B, int *cnt) {
-  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  const int g = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (g >= G) return;
   const auto &cp = d.cases[col_case[g]];
   const int ny2 = 2 * cp.ny;
   const int *lb = d.f_lb + cp.lb_off + (g - cp.col_off) * (ny2 + 1);
-  cnt[g] = (lb[ny2] - lb[0] + B - 1) / B;
+  const int s0 = lb[0], n = lb[ny2] - s0, nch = (n + B - 1) / B;
+  int t = 0;
+  for (int k = lane; k < nch; k += 64) t += chunk_pieces(d.f_cxy, s0 + k * B, min(B, n - k * B), B);
+  t = wave_sum(t);
+  if (lane == 0) cnt[g] = t;
 }
 
 // Exclusive scan of the per-sub-column tile counts (one block of 1024 threads); the total
@@ -491,7 +525,7 @@ __global__ void __launch_bounds__(1024) k_tile_scan(const int *cnt, int G, int *
   if (t == 1023) off[G] = part[1023];
 }
 
-// Plans of the tiles of size B (one wave per global sub-column, looping over its tiles).
+// Plans of the tiles of size B (one wave per global sub-column, a lane per chunk).
 template <class Pol>
 __global__ void __launch_bounds__(256) k_tile_plan(Dev<Pol> d, int G, const int *col_case, int B, const int *off,
                                                    TilePlan *plans) {
@@ -501,41 +535,39 @@ __global__ void __launch_bounds__(256) k_tile_plan(Dev<Pol> d, int G, const int 
   const auto &cp = d.cases[c];
   const int sx = g - cp.col_off, cx = sx >> 1, ny2 = 2 * cp.ny;
   const int *lbc = d.f_lb + cp.lb_off;
-  const int s0 = lbc[sx * (ny2 + 1)], s1 = lbc[sx * (ny2 + 1) + ny2];
-  const int ntile = (s1 - s0 + B - 1) / B;
-  for (int k = 0; k < ntile; ++k) {
-    const int tile = off[g] + k, start = s0 + k * B, count = min(B, s1 - start);
-    const int sy0 = cxy_y(d.f_cxy[start]), sy1 = cxy_y(d.f_cxy[start + count - 1]);
-    const int sc0 = max(2 * cx - 2, 0), sc1 = min(2 * cx + 3, 2 * cp.nx - 1);
-    const int r0 = max(2 * (sy0 >> 1) - 2, 0), r1 = min(2 * (sy1 >> 1) + 3, ny2 - 1);
-    const int nsc = sc1 - sc0 + 1;
-    // spans: lanes 0..nsc-1 take one sub-column each
-    int lo = 0, len = 0, row = 0;
-    if (lane < nsc) {
-      row = (sc0 + lane) * (ny2 + 1) + r0;
-      lo = lbc[row];
-      len = lbc[row + (r1 - r0 + 1)] - lo;
-      row += cp.lb_off;
-    }
-    int total = len;
-    for (int o = 32; o > 0; o >>= 1) total += __shfl_xor(total, o, 64);
-    // solid / wall presence over the 3x3 cells of the tile's lanes
-    const int wc0 = max(cx - 1, 0), wnc = min(cx + 1, cp.nx - 1) - wc0 + 1;
-    const int wr0 = max((sy0 >> 1) - 1, 0), wnr = min((sy1 >> 1) + 1, cp.ny - 1) - wr0 + 1;
-    bool has = false;
-    for (int e = lane; e < wnc * wnr; e += 64) {
-      const int col = e / wnr, q = cp.cell_off + (wc0 + col) * cp.ny + wr0 + (e - col * wnr);
-      const int2 a = d.s_range[q], w = d.w_range[q];
-      has = has || a.y > a.x || w.y > w.x;
-    }
-    const bool sw = __ballot(has ? 1 : 0) != 0ull;
-    TilePlan &P = plans[tile];
-    if (lane < kTileCols) {
-      P.lo[lane] = lo;
-      P.len[lane] = len;
-      P.row[lane] = row;
-    }
-    if (lane == 0) {
+  const int s0 = lbc[sx * (ny2 + 1)], n = lbc[sx * (ny2 + 1) + ny2] - s0, nch = (n + B - 1) / B;
+  const int sc0 = max(2 * cx - 2, 0), sc1 = min(2 * cx + 3, 2 * cp.nx - 1), nsc = sc1 - sc0 + 1;
+  const int wc0 = max(cx - 1, 0), wc1 = min(cx + 1, cp.nx - 1);
+  const int *swp = d.f_swp + cp.swp_off;
+  int tile = off[g];
+  for (int k0 = 0; k0 < nch; k0 += 64) {
+    const int k = k0 + lane, cstart = s0 + k * B, ccount = k < nch ? min(B, n - k * B) : 0;
+    const int np = k < nch ? chunk_pieces(d.f_cxy, cstart, ccount, B) : 0;
+    const int incl = wave_incl_sum(np, lane);
+    for (int q = 0; q < np; ++q) {
+      TilePlan &P = plans[tile + incl - np + q];
+      const int start = cstart + q * ccount / np, count = cstart + (q + 1) * ccount / np - start;
+      const int sy0 = cxy_y(d.f_cxy[start]), sy1 = cxy_y(d.f_cxy[start + count - 1]);
+      const int r0 = max(2 * (sy0 >> 1) - 2, 0), r1 = min(2 * (sy1 >> 1) + 3, ny2 - 1), nr = r1 - r0 + 1;
+      int total = 0;
+#pragma unroll
+      for (int a = 0; a < kTileCols; ++a) {
+        int lo = 0, len = 0, row = 0;
+        if (a < nsc) {
+          row = (sc0 + a) * (ny2 + 1) + r0;
+          lo = lbc[row];
+          len = lbc[row + nr] - lo;
+          row += cp.lb_off;
+        }
+        P.lo[a] = lo;
+        P.len[a] = len;
+        P.row[a] = row;
+        total += len;
+      }
+      // a solid or wall particle in the 3x3 cells of the tile's lanes
+      const int wr0 = max((sy0 >> 1) - 1, 0), wr1 = min((sy1 >> 1) + 1, cp.ny - 1);
+      int sw = 0;
+      for (int col = wc0; col <= wc1; ++col) sw += swp[col * (cp.ny + 1) + wr1 + 1] - swp[col * (cp.ny + 1) + wr0];
       P.c = c;
       P.start = start;
       P.count = count;
@@ -544,21 +576,23 @@ __global__ void __launch_bounds__(256) k_tile_plan(Dev<Pol> d, int G, const int 
       P.sc0 = sc0;
       P.nsc = nsc;
       P.r0 = r0;
-      P.nr = r1 - r0 + 1;
+      P.nr = nr;
       P.cy_ref = (sy0 + sy1) >> 2;
-      P.sw = sw ? 1 : 0;
+      P.sw = sw > 0 ? 1 : 0;
       P.total = total;
     }
+    tile += __shfl(incl, 63, 64);
   }
 }
 
 // Blocks are dealt to the 8 XCDs round-robin, and each XCD has its own L2.  Giving XCD x the
-// contiguous tile range [x per, (x + 1) per) makes the tiles an XCD runs at the same time
-// neighbours, so the halo they stage is read from its L2.
+// contiguous range [x per, (x + 1) per) of the n tiles makes the tiles an XCD runs at the
+// same time neighbours, so the halo they stage is read from its L2.  The grid is sized for
+// the largest possible tile count; a block with no tile gets n.
 constexpr int kXcds = 8;
-__device__ __forceinline__ int xcd_tile(int b, int grid) {
-  const int per = (grid + kXcds - 1) / kXcds;
-  return (b % kXcds) * per + b / kXcds;
+__device__ __forceinline__ int xcd_tile(int b, int n) {
+  const int per = (n + kXcds - 1) / kXcds, k = b / kXcds;
+  return k < per ? (b % kXcds) * per + k : n;
 }
 
 // Per-case maximum of non-negative double bit patterns: one atomic per block (a tile lies in
@@ -791,8 +825,8 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
   __shared__ int s_lb[kTileCols][TileRows<BLOCK>::value + 1];
   __shared__ unsigned long long s_red[BLOCK / 64];
   __shared__ float s_dmax[BLOCK / 64];
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  if (tile >= *ntiles) return;
+  const int nt = *ntiles, tile = xcd_tile(blockIdx.x, nt);
+  if (tile >= nt) return;
   const TilePlan &P = plans[tile];
   const int t = threadIdx.x, i = P.start + t, c = P.c;
   const bool valid = t < P.count;
@@ -1047,8 +1081,8 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
   __shared__ R2 s_pos[CAP + 4];  // + far-away padding read by the masked tail of a window
   __shared__ int s_lb[kTileCols][TileRows<BLOCK>::value + 1];
   __shared__ float s_dmax[BLOCK / 64];
-  const int tile = xcd_tile(blockIdx.x, gridDim.x);
-  if (tile >= *ntiles) return;
+  const int nt = *ntiles, tile = xcd_tile(blockIdx.x, nt);
+  if (tile >= nt) return;
   const TilePlan &P = plans[tile];
   const int t = threadIdx.x, i = P.start + t, c = P.c;
   const bool valid = t < P.count;

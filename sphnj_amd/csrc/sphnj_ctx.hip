@@ -116,14 +116,14 @@ struct Engine : EngineBase {
   // counts per sub-column, their exclusive scan (+ total), plans, and whether the plans
   // match the current hash
   int G = 0;
-  int64_t lb_total = 0;
+  int64_t lb_total = 0, swp_total = 0;
   int *col_case = nullptr;
   int *tile_cnt[3] = {nullptr, nullptr, nullptr}, *tile_off[3] = {nullptr, nullptr, nullptr};
   TilePlan *plans[3] = {nullptr, nullptr, nullptr};
   bool plan_ok[3] = {false, false, false};
   static int bidx(int B) { return B == 64 ? 0 : B == 128 ? 1 : 2; }
   // tiles of B slots never span sub-columns: at most one partial tile per sub-column
-  int max_tiles(int B) const { return nblk(d.nf, B) + G; }
+  int max_tiles(int B) const { return kMaxPieces * (nblk(d.nf, B) + G); }
   int grid_tiles(int B) const { return (max_tiles(B) + kXcds - 1) / kXcds * kXcds; }
   bool viscid = false;
   RadixSorter fsort, ssort;
@@ -280,6 +280,7 @@ struct Engine : EngineBase {
     }
     G = 0;
     lb_total = 0;
+    swp_total = 0;
     std::vector<int> colc;
     for (int c = 0; c < ncase; ++c) {
       fill_case(D[c], hc[c]);
@@ -290,6 +291,8 @@ struct Engine : EngineBase {
       hc[c].lb_off = (int)lb_total;
       G += 2 * hc[c].nx;
       lb_total += (int64_t)(2 * hc[c].nx) * (2 * hc[c].ny + 1);
+      hc[c].swp_off = (int)swp_total;
+      swp_total += (int64_t)hc[c].nx * (hc[c].ny + 1);
       colc.insert(colc.end(), 2 * hc[c].nx, c);
       f_off[c + 1] = f_off[c] + (int)St[c].fx.size();
       s_off[c + 1] = s_off[c] + (int)St[c].sx.size();
@@ -349,7 +352,8 @@ struct Engine : EngineBase {
         (rc = dalloc(&d.f_sacc, 2 * (size_t)nf)) ||
         (rc = dalloc(&d.f_srange, 4 * (size_t)nc)) || (rc = dalloc(&d.umax2, ncase)) ||
         (rc = dalloc(&d.umax_slot, (size_t)ncase * kSpread * kSlotStride)) || (rc = dalloc(&d.status, 1)) ||
-        (rc = dalloc(&d.f_lb, (size_t)lb_total)) || (rc = dalloc(&col_case, G)))
+        (rc = dalloc(&d.f_lb, (size_t)lb_total)) || (rc = dalloc(&d.f_swp, (size_t)swp_total)) ||
+        (rc = dalloc(&col_case, G)))
       return rc;
     if ((rc = h2d(col_case, colc.data(), G))) return rc;
     for (int b = 0; b < 3; ++b) {
@@ -598,7 +602,7 @@ struct Engine : EngineBase {
   int ensure_plans(int B) {
     const int b = bidx(B);
     if (plan_ok[b] || d.nf == 0) return SPHNJ_OK;
-    k_tile_count<Pol><<<nblk(G, 256), 256, 0, stream>>>(d, G, col_case, B, tile_cnt[b]);
+    k_tile_count<Pol><<<nblk(G, 4), 256, 0, stream>>>(d, G, col_case, B, tile_cnt[b]);
     k_tile_scan<<<1, 1024, 0, stream>>>(tile_cnt[b], G, tile_off[b]);
     k_tile_plan<Pol><<<nblk(G, 4), 256, 0, stream>>>(d, G, col_case, B, tile_off[b], plans[b]);
     LAUNCH_CHECK();

@@ -104,6 +104,7 @@ struct Dev {
   // kSpread slots per case (slot s at umax_slot[(c kSpread + s) kSlotStride], one cache line
   // each, so no address takes more than a few hundred atomics); umax2[c] is their maximum
   unsigned long long *umax_slot, *umax2;
+  R *umax;  // per case: min(sqrt(umax2), umax_cap), the speed the shift scales with
   // solid (original order, never permuted)
   double *sx, *sy, *su, *sv;     // cur
   double *sx2, *sy2, *su2, *sv2; // alt (sub-step ping-pong)
@@ -228,6 +229,14 @@ __device__ __forceinline__ float wave_max_f(float v) {
 __device__ __forceinline__ unsigned wave_or(unsigned v) {
   for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
   return v;
+}
+
+// Element i of an array whose base is wave-uniform, addressed by a 32-bit byte offset (the
+// arrays here stay far below 4 GiB), so the load takes the base from scalar registers and
+// needs no 64-bit address arithmetic per lane.
+template <class T>
+__device__ __forceinline__ T ld32(const T *base, int i) {
+  return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (unsigned)i * (unsigned)sizeof(T));
 }
 
 __device__ __forceinline__ int wave_sum(int v) {

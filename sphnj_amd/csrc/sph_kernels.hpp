@@ -168,7 +168,10 @@ template <class Pol>
 __global__ void __launch_bounds__(256) k_permute_fluid(Dev<Pol> d, const int *sorted_vals) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < d.ncase * kSpread) d.umax_slot[(size_t)t * kSlotStride] = 0ull;
-  if (t < d.ncase) d.umax2[t] = 0ull;
+  if (t < d.ncase) {
+    d.umax2[t] = 0ull;
+    d.umax[t] = 0;
+  }
   if (t >= d.nf) return;
   const int o = sorted_vals[t];
   const int s = d.f_slot_of_orig[o];
@@ -432,6 +435,7 @@ struct TilePlan {
   int sw;                // a solid or wall particle lies in the tile's 3x3 cells
   int total;             // candidate particles (sum of the span lengths)
   int lo[kTileCols], len[kTileCols], row[kTileCols];  // slot span; f_lb index of (sc0 + a, r0)
+  int gsz, gmul;         // staging: threads per span B / nsc; ceil(2^20 / gsz) (t / gsz = t gmul >> 20)
 };
 
 // Row table of every sub-column (one wave per global sub-column): f_lb[k] for row k = the
@@ -580,6 +584,8 @@ __global__ void __launch_bounds__(256) k_tile_plan(Dev<Pol> d, int G, const int 
       P.cy_ref = (sy0 + sy1) >> 2;
       P.sw = sw > 0 ? 1 : 0;
       P.total = total;
+      P.gsz = B / nsc;
+      P.gmul = ((1 << 20) + P.gsz - 1) / P.gsz;
     }
     tile += __shfl(incl, 63, 64);
   }
@@ -611,13 +617,16 @@ __device__ __forceinline__ void reduce_case_max(unsigned long long *slots, int c
   }
 }
 
-// umax2[c] = max over the case's slots; one wave per case.
-__global__ void __launch_bounds__(64) k_umax_final(const unsigned long long *slots, unsigned long long *umax2,
-                                                   int ncase) {
+// umax2[c] = max over the case's slots, and the capped speed umax[c]; one wave per case.
+template <class Pol>
+__global__ void __launch_bounds__(64) k_umax_final(Dev<Pol> d) {
   const int c = blockIdx.x;
-  if (c >= ncase) return;
-  const unsigned long long m = wave_max_u64(slots[(size_t)(c * kSpread + (int)threadIdx.x) * kSlotStride]);
-  if (threadIdx.x == 0) umax2[c] = m;
+  if (c >= d.ncase) return;
+  const unsigned long long m = wave_max_u64(d.umax_slot[(size_t)(c * kSpread + (int)threadIdx.x) * kSlotStride]);
+  if (threadIdx.x == 0) {
+    d.umax2[c] = m;
+    d.umax[c] = fmin((typename Pol::R)sqrt(__longlong_as_double((long long)m)), d.cases[c].umax_cap);
+  }
 }
 
 // Copies a tile into LDS: the row tables (lb[a][k] = LDS index of the first candidate of row
@@ -629,8 +638,9 @@ __global__ void __launch_bounds__(64) k_umax_final(const unsigned long long *slo
 template <int BLOCK, int U, class Ld, class St>
 __device__ __forceinline__ void tile_stage(const TilePlan &P, const int *f_lb, int (*lb)[TileRows<BLOCK>::value + 1],
                                            Ld &&ld, St &&st) {
-  const int t = threadIdx.x, n = P.nsc, nr = P.nr, gsz = BLOCK / n;
-  const int a = min(t / gsz, n - 1), k0 = t - a * gsz, stride = a == n - 1 ? BLOCK - a * gsz : gsz;
+  const int t = threadIdx.x, n = P.nsc, nr = P.nr, gsz = P.gsz;
+  const int a = min((int)(((unsigned)t * (unsigned)P.gmul) >> 20), n - 1), k0 = t - a * gsz;
+  const int stride = a == n - 1 ? BLOCK - a * gsz : gsz;
   int base = 0;
   for (int q = 0; q < a; ++q) base += P.len[q];
   const int lo = P.lo[a], len = P.len[a], row = P.row[a];
@@ -655,6 +665,13 @@ __device__ __forceinline__ LanePos lane_pos(double x, double y, int sx, int sy, 
   const double tx = (x - cp.cxmin) * cp.inv_cdx * 2.0 - (double)sx;
   const double ty = (y - cp.cymin) * cp.inv_cdy * 2.0 - (double)sy;
   return {(float)fmin(fmax(tx, -8.0), 9.0), (float)fmin(fmax(ty, -8.0), 9.0)};
+}
+// The same from an M32 view (the offset from the origin of the lane's cell (cx, cy)): float
+// arithmetic, whose rounding stays far inside the pruning margin.
+template <class CP>
+__device__ __forceinline__ LanePos lane_pos_view(float xi, float yi, int cx, int cy, int sx, int sy, const CP &cp) {
+  const float tx = fmaf(xi, cp.inv_ssx, (float)(2 * cx - sx)), ty = fmaf(yi, cp.inv_ssy, (float)(2 * cy - sy));
+  return {fminf(fmaxf(tx, -8.f), 9.f), fminf(fmaxf(ty, -8.f), 9.f)};
 }
 
 // Visits the candidate row windows of a lane: f(sc, ra, rb) for each sub-column sc that meets
@@ -832,12 +849,12 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
   const bool valid = t < P.count;
   const auto &A = d.fa;
   const bool staged = P.total <= CAP && P.nr <= TileRows<BLOCK>::value;
-  float dmax = 0.f;
+  int dmax = 0;  // bit pattern of the largest path length (non-negative floats order as ints)
   if (staged) {
     struct Ld {
       R2 pos, vel, rp;
       int fl;
-      float cum;
+      int cum;
       int pk;
     };
     const auto &cq = d.cases[c];
@@ -846,8 +863,9 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
     tile_stage<BLOCK, 4>(
         P, d.f_lb, s_lb,
         [&](int g) {
-          return Ld{R2{A.vx[g], A.vy[g]}, R2{A.vu[g], A.vv[g]}, R2{A.vrho[g], A.p[g]},
-                    (int)d.f_phase[g] | ((int)d.f_cls[g] << 8), A.cum[g], Pol::MIXED ? d.f_cxy[g] : 0};
+          return Ld{R2{ld32(A.vx, g), ld32(A.vy, g)}, R2{ld32(A.vu, g), ld32(A.vv, g)},
+                    R2{ld32(A.vrho, g), ld32(A.p, g)}, (int)ld32(d.f_phase, g) | ((int)ld32(d.f_cls, g) << 8),
+                    ld32(reinterpret_cast<const int *>(A.cum), g), Pol::MIXED ? ld32(d.f_cxy, g) : 0};
         },
         [&](int l, const Ld &v, int) {
           if constexpr (Pol::MIXED)  // tile frame
@@ -858,7 +876,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
           s_vel[l] = v.vel;
           s_rp[l] = v.rp;
           s_fl[l] = v.fl;
-          dmax = fmaxf(dmax, v.cum);
+          dmax = max(dmax, v.cum);
         });
     if (t < 2) {
       s_pos[P.total + t] = R2{R(1e18), R(1e18)};
@@ -867,8 +885,8 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
       s_fl[P.total + t] = 0;
     }
   }
-  dmax = wave_max_f(dmax);
-  if ((t & 63) == 0) s_dmax[t >> 6] = dmax;
+  for (int o = 32; o > 0; o >>= 1) dmax = max(dmax, __shfl_xor(dmax, o, 64));
+  if ((t & 63) == 0) s_dmax[t >> 6] = __int_as_float(dmax);
   __syncthreads();
   unsigned long long my_u2 = 0ull;
   if (valid) {
@@ -876,7 +894,11 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
     const int pk = d.f_cxy[i], sy = cxy_y(pk), cx = P.cx, cy = sy >> 1;
     const R xi = A.vx[i], yi = A.vy[i], ui = A.vu[i], vi = A.vv[i], rhoi = A.vrho[i], pi = A.p[i];
     const double x0 = A.x[i], y0 = A.y[i];
-    const LanePos lp = lane_pos(x0, y0, P.sx, sy, cp);
+    LanePos lp;
+    if constexpr (Pol::MIXED)
+      lp = lane_pos_view(xi, yi, cx, cy, P.sx, sy, cp);
+    else
+      lp = lane_pos(x0, y0, P.sx, sy, cp);
     const int phi = d.f_phase[i];
     const bool liq = phi == 0;
     const R cs = cp.c[phi], csinv = cp.cinv[phi], mu8 = cp.mu8[phi];
@@ -1106,31 +1128,35 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
       if (pass > 0) sacc = R2{d.f_sacc[2 * i], d.f_sacc[2 * i + 1]};
     }
   }
-  float dmax = 0.f;
+  int dmax = 0;  // bit pattern of the largest path length (non-negative floats order as ints)
   if (staged) {
     struct Ld {
       R2 pos;
       int pk;
-      float cum;
+      int cum;
     };
     const R fdy = cp.cdy_r, gs = cp.gscale;
     // this thread's span lies in one cell column: its x offset in the tile frame, scaled
-    const int sa = min((int)threadIdx.x / (BLOCK / P.nsc), P.nsc - 1);
+    const int sa = min((int)(((unsigned)t * (unsigned)P.gmul) >> 20), P.nsc - 1);
     const R xo = (R)(((P.sc0 + sa) >> 1) - P.cx) * cp.cdx_r * gs;
     const int gcy = P.cy_ref;
     tile_stage<BLOCK, 4>(
-        P, d.f_lb, s_lb, [&](int g) { return Ld{R2{A.vx[g], A.vy[g]}, Pol::MIXED ? d.f_cxy[g] : 0, A.cum[g]}; },
+        P, d.f_lb, s_lb,
+        [&](int g) {
+          return Ld{R2{ld32(A.vx, g), ld32(A.vy, g)}, Pol::MIXED ? ld32(d.f_cxy, g) : 0,
+                    ld32(reinterpret_cast<const int *>(A.cum), g)};
+        },
         [&](int l, const Ld &v, int) {
           if constexpr (Pol::MIXED)  // tile frame, scaled units
             s_pos[l] = R2{fmaf(v.pos.x, gs, xo), (v.pos.y + (float)(((v.pk & 0xffff) >> 1) - gcy) * fdy) * gs};
           else
             s_pos[l] = v.pos;
-          dmax = fmaxf(dmax, v.cum);
+          dmax = max(dmax, v.cum);
         });
     if (t < 4) s_pos[P.total + t] = R2{R(1e18), R(1e18)};
   }
-  dmax = wave_max_f(dmax);
-  if ((t & 63) == 0) s_dmax[t >> 6] = dmax;
+  for (int o = 32; o > 0; o >>= 1) dmax = max(dmax, __shfl_xor(dmax, o, 64));
+  if ((t & 63) == 0) s_dmax[t >> 6] = __int_as_float(dmax);
   __syncthreads();
   if (!valid) return;
   if (!active) {  // this case has fewer passes than the batch: carry the moving fields over
@@ -1143,7 +1169,11 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
   // current position: the M32 masters stay at the pair kernel's result until the last pass
   const double xc = Pol::MIXED ? x0 + (double)sacc.x : x0, yc = Pol::MIXED ? y0 + (double)sacc.y : y0;
   const R xi = view_x<Pol>(xc, cx, cp), yi = view_y<Pol>(yc, cy, cp);  // = the stored views
-  const LanePos lp = lane_pos(xc, yc, P.sx, sy, cp);
+  LanePos lp;
+  if constexpr (Pol::MIXED)
+    lp = lane_pos_view(xi, yi, cx, cy, P.sx, sy, cp);
+  else
+    lp = lane_pos(xc, yc, P.sx, sy, cp);
   const LoopConsts<R> K(cp);
   R gx = 0, gy = 0;
   auto add = [&](R dx, R dy) {
@@ -1215,7 +1245,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
     gx += gxs * cp.kdc_s;
     gy += gys * cp.kdc_s;
   }
-  const R umax = fmin((R)sqrt(__longlong_as_double((long long)d.umax2[c])), cp.umax_cap);
+  const R umax = d.umax[c];
   const R dxs = -cp.shift_k * gx, dys = -cp.shift_k * gy;
   R sxv = dxs, syv = dys;
   if (cls == 0) {  // SPLASH: no shift
@@ -1251,8 +1281,9 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
     B.x[i] = x;
     B.y[i] = y;
   }
-  const double mx = x - xc, my = y - yc;
-  B.cum[i] = (float)((double)cum0 + sqrt(mx * mx + my * my));
+  // path length: this pass moved the particle by (sxv, syv)
+  const float sx2 = (float)sxv, sy2 = (float)syv;
+  B.cum[i] = cum0 + __builtin_amdgcn_sqrtf(sx2 * sx2 + sy2 * sy2);
   if (last) {
     if constexpr (Pol::MIXED) {
       A.x[i] = x;

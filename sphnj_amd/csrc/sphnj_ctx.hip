@@ -350,7 +350,7 @@ struct Engine : EngineBase {
         (rc = dalloc(&d.f_key_by_orig, nf)) || (rc = dalloc(&d.f_slot_of_orig, nf)) ||
         (rc = dalloc(&d.f_col, nf)) || (rc = dalloc(&d.f_near, nf)) || (rc = dalloc(&d.f_L, 4 * (size_t)nf)) ||
         (rc = dalloc(&d.f_sacc, 2 * (size_t)nf)) ||
-        (rc = dalloc(&d.f_srange, 4 * (size_t)nc)) || (rc = dalloc(&d.umax2, ncase)) ||
+        (rc = dalloc(&d.f_srange, 4 * (size_t)nc)) || (rc = dalloc(&d.umax2, ncase)) || (rc = dalloc(&d.umax, ncase)) ||
         (rc = dalloc(&d.umax_slot, (size_t)ncase * kSpread * kSlotStride)) || (rc = dalloc(&d.status, 1)) ||
         (rc = dalloc(&d.f_lb, (size_t)lb_total)) || (rc = dalloc(&d.f_swp, (size_t)swp_total)) ||
         (rc = dalloc(&col_case, G)))
@@ -740,7 +740,7 @@ struct Engine : EngineBase {
           const int rc = launch_pair(blk_pair);
           ++move_epoch;
           if (rc) return rc;
-          k_umax_final<<<ncase, 64, 0, stream>>>(d.umax_slot, d.umax2, ncase);
+          k_umax_final<Pol><<<ncase, 64, 0, stream>>>(d);
           LAUNCH_CHECK();
           swap_fluid_all();
           std::swap(d.f_cls, d.f_cls2);  // the class of this step

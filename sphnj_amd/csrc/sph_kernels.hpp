@@ -772,6 +772,13 @@ struct LoopConsts {
       e = exp(-(r2 * hinv2));
     return r2 < rc2 ? e * kdc : R(0);
   }
+  // w'(r^2) of a candidate known to lie inside the cut-off
+  __device__ __forceinline__ R w_in(R r2) const {
+    if constexpr (sizeof(R) == 4)
+      return __builtin_amdgcn_exp2f(r2 * nhl2) * kdc;
+    else
+      return exp(-(r2 * hinv2)) * kdc;
+  }
 };
 
 // ======================================================================================
@@ -781,7 +788,7 @@ struct LoopConsts {
 template <class R>
 struct PairSums {
   R ax = 0, ay = 0, drho = 0, L00 = 0, L01 = 0, L10 = 0, L11 = 0, col = 0;
-  int near = 0;
+  R fsr = R(1e30);  // smallest r^2 of a FREE_SURFACE neighbour inside the cut-off (liquid lanes)
 };
 
 // Riemann terms of fluidFluidCalculate / fluidSolidCalculate / fluidVirtCalculate
@@ -837,8 +844,8 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
   using R2 = typename V2<R>::T;
   constexpr int CAP = PairCap<Pol, BLOCK>::value;
   // + padding read by the masked tail of a window: far away, finite, no flags
-  __shared__ R2 s_pos[CAP + 2], s_vel[CAP + 2], s_rp[CAP + 2];
-  __shared__ int s_fl[CAP + 2];
+  __shared__ R2 s_pos[CAP + 4], s_vel[CAP + 4], s_rp[CAP + 4];
+  __shared__ int s_fl[CAP + 4];
   __shared__ int s_lb[kTileCols][TileRows<BLOCK>::value + 1];
   __shared__ unsigned long long s_red[BLOCK / 64];
   __shared__ float s_dmax[BLOCK / 64];
@@ -878,7 +885,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
           s_fl[l] = v.fl;
           dmax = max(dmax, v.cum);
         });
-    if (t < 2) {
+    if (t < 4) {
       s_pos[P.total + t] = R2{R(1e18), R(1e18)};
       s_vel[P.total + t] = R2{R(0), R(0)};
       s_rp[P.total + t] = R2{R(1), R(0)};
@@ -913,17 +920,27 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
       own = b + (i - P.lo[P.sx - P.sc0]);
     }
     PairSums<R> S;
-    // fluidFluidCalculate on one candidate (w = 0 outside the cut-off: exact zero terms)
-    auto fluid_j = [&](bool self, R dx, R dy, R r2, R uj, R vj, R rhoj, R pj, int fl, bool in = true) {
-      const R wk = K.w(r2);
-      const R w = in ? wk : R(0);
+    // fluidFluidCalculate on one candidate (w = 0 outside the cut-off: exact zero terms);
+    // INSIDE: the candidate is known to lie inside the cut-off.  The reference's sticky near
+    // flag (src/fluid.cu:67-73) ends at HAS_NEAR (2) if a FREE_SURFACE neighbour lies within
+    // half the cut-off, else HAS (1) if one lies within it: the smallest such r^2 decides.
+    auto fluid_j = [&](auto inside, bool self, R dx, R dy, R r2, R uj, R vj, R rhoj, R pj, int fl, bool in) {
+      constexpr bool INSIDE = decltype(inside)::value;
+      R w;
+      bool fs = liq & !self & ((fl >> 8) == 1);
+      if constexpr (INSIDE) {
+        w = K.w_in(r2);
+      } else {
+        const R wk = K.w(r2);
+        w = in ? wk : R(0);
+        fs = fs & in & (r2 < K.rc2);
+      }
       riemann_pair<R, VISCID>(dx, dy, r2, w, ui, vi, uj, vj, rhoi, rhoi_inv, rhoj, pi, pj, cs, csinv, mu8, eps_h2, S);
       if (!(liq && (fl & 0xff) == 1)) S.col += r2 * w;
-      // FREE_SURFACE neighbour: HAS_NEAR (2) within half the cut-off, else HAS (1); the
-      // reference's sticky update (src/fluid.cu:67-73) ends at the maximum over neighbours.
-      const bool fs = in & (r2 < K.rc2) & liq & !self & ((fl >> 8) == 1);
-      S.near = max(S.near, fs ? (r2 < qrc2 ? 2 : 1) : 0);
+      S.fsr = (fs && r2 < S.fsr) ? r2 : S.fsr;
     };
+    using Cut = std::false_type;
+    using Inside = std::true_type;
     const int *lbc = d.f_lb + cp.lb_off;
     const int ny21 = 2 * cp.ny + 1;
     // rows [qa, qb] of sub-column sc from global memory, whose views share the cell origin
@@ -931,52 +948,52 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
     auto rows = [&](int sc, int qa, int qb, R bx, R by) {  // global memory
       for_range<2>(lbc[sc * ny21 + qa], lbc[sc * ny21 + qb + 1], [&](int s) {
         const R dx = A.vx[s] - bx, dy = A.vy[s] - by;
-        fluid_j(s == i, dx, dy, dx * dx + dy * dy, A.vu[s], A.vv[s], A.vrho[s], A.p[s],
-                (int)d.f_phase[s] | ((int)d.f_cls[s] << 8));
+        fluid_j(Cut{}, s == i, dx, dy, dx * dx + dy * dy, A.vu[s], A.vv[s], A.vrho[s], A.p[s],
+                (int)d.f_phase[s] | ((int)d.f_cls[s] << 8), true);
       });
     };
     const float Rp = prune_radius(cp, staged, staged_bound<BLOCK>(s_dmax));
     // the lane in the tile frame (M32 staged positions are re-based on it)
     const R yg = Pol::MIXED ? yi + (R)(cy - P.cy_ref) * cp.cdy_r : yi;
-    auto staged_j = [&](int l, bool in) {
+    auto staged_j = [&](auto inside, int l, bool in) {
       const R2 pj = s_pos[l], vj = s_vel[l], rp = s_rp[l];
       const R dx = pj.x - xi, dy = pj.y - yg;
-      fluid_j(l == own, dx, dy, dx * dx + dy * dy, vj.x, vj.y, rp.x, rp.y, s_fl[l], in);
+      fluid_j(inside, l == own, dx, dy, dx * dx + dy * dy, vj.x, vj.y, rp.x, rp.y, s_fl[l], in);
     };
     if (staged) {
-      // staged windows (one run each): first mark the candidates inside the cut-off (64-bit
+      // staged windows (one run each): first mark the candidates inside the cut-off (32-bit
       // mask per window), then evaluate the Riemann terms for those only, in slot order --
       // the same non-zero terms in the same order as a pass over every candidate
-      unsigned long long mk[kTileCols];
+      unsigned mk[kTileCols];
       int w0[kTileCols];
 #pragma unroll
       for (int sa = 0; sa < kTileCols; ++sa) {
-        mk[sa] = 0ull;
+        mk[sa] = 0u;
         w0[sa] = 0;
         int ra, rb;
         if (!span_window(P, cp, sa, sy, cy, lp, Rp, ra, rb)) continue;
         const int s0 = s_lb[sa][ra - P.r0], s1 = s_lb[sa][rb + 1 - P.r0];
         w0[sa] = s0;
-        if (s1 - s0 > 64) {  // a long window: evaluate every candidate
-          for_range_masked<2>(s0, s1, staged_j);
+        if (s1 - s0 > 32) {  // a long window: evaluate every candidate
+          for_range_masked<2>(s0, s1, [&](int l, bool in) { staged_j(Cut{}, l, in); });
           continue;
         }
-        unsigned long long m = 0ull;
+        unsigned m = 0u;
         for_range_masked<4>(s0, s1, [&](int l, bool in) {
           const R2 pj = s_pos[l];
           const R dx = pj.x - xi, dy = pj.y - yg;
           const bool ok = in & (dx * dx + dy * dy < K.rc2);
-          m |= (ok ? 1ull : 0ull) << ((l - s0) & 63);
+          m |= (ok ? 1u : 0u) << ((l - s0) & 31);
         });
         mk[sa] = m;
       }
 #pragma unroll
       for (int sa = 0; sa < kTileCols; ++sa) {
-        unsigned long long m = mk[sa];
+        unsigned m = mk[sa];
         while (m) {
-          const int b = __builtin_ctzll(m);
-          m &= m - 1ull;
-          staged_j(w0[sa] + b, true);
+          const int b = __builtin_ctz(m);
+          m &= m - 1u;
+          staged_j(Inside{}, w0[sa] + b, true);
         }
       }
     }
@@ -1024,7 +1041,8 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
       }
     }
     d.f_col[i] = S.col;
-    d.f_near[i] = (int8_t)S.near;
+    const int near = S.fsr < qrc2 ? 2 : (S.fsr < K.rc2 ? 1 : 0);
+    d.f_near[i] = (int8_t)near;
     // the fluid class of this step (correctionUpdate, src/fluid.cu:369-427), a function of
     // phase, colour and near flag only; it is set only where the case shifts
     int cls = d.f_cls[i];
@@ -1032,9 +1050,9 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
       if (phi == 1)
         cls = 3;  // gas: INNER
       else if (S.col < R(1.5))
-        cls = S.near == 0 ? 0 : 1;  // SPLASH / FREE_SURFACE
+        cls = near == 0 ? 0 : 1;  // SPLASH / FREE_SURFACE
       else
-        cls = (S.col < R(2.0) && S.near == 2) ? 2 : 3;  // VICINITY / INNER
+        cls = (S.col < R(2.0) && near == 2) ? 2 : 3;  // VICINITY / INNER
     }
     d.f_cls2[i] = (int8_t)cls;
     d.f_L[4 * i + 0] = S.L00;

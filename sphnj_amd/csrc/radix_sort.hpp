@@ -7,10 +7,13 @@
 //              counter, so equal digits (common: particles share cells) cost one LDS atomic
 //              per group instead of one per lane;
 //   rowscan    exclusive scan of each digit's row of tile counts (digit-major layout);
-//   downsweep  re-reads the tile, ranks each item inside its peer group (popcount of the
-//              lower lanes), turns per-wave digit counts into offsets in LDS, and scatters.
-// Items of a tile are visited round by round, wave by wave, lane by lane in index order, so
-// the sort is stable.  Values may be implicit (value = input index) on the first pass.
+//   downsweep  re-reads the tile: wave w takes the tile's w-th quarter, 64 consecutive items
+//              a round; an item's place in its wave's run of its digit is the run's count so
+//              far (kept by the peer group's first lane in LDS) plus its rank among its peers
+//              (popcount of the lower lanes); one block scan then turns the per-wave digit
+//              counts into offsets, and the items, held in registers, are scattered.
+// Items are ranked in index order (quarter by quarter, round by round, lane by lane), so the
+// sort is stable.  Values may be implicit (value = input index) on the first pass.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -107,46 +110,55 @@ __global__ void __launch_bounds__(kSortThreads) radix_downsweep(const int *__res
                                                                 int n, int shift, int nblocks,
                                                                 const int *__restrict__ hist,
                                                                 const int *__restrict__ digit_total) {
+  constexpr int kWaves = kSortThreads / 64;
   __shared__ int base[kRadix];
-  __shared__ int woff[4][kRadix];
+  __shared__ int woff[kWaves][kRadix];  // per-wave digit counts, then the waves' offsets
   __shared__ int tmp[4];
-  const int t = threadIdx.x, w = t >> 6;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+#pragma unroll
+  for (int k = 0; k < kWaves; ++k) woff[k][t] = 0;
   int tot;
-  const int ex = block_exscan256(digit_total[t], tmp, &tot);
+  const int ex = block_exscan256(digit_total[t], tmp, &tot);  // its barriers order the clears
   base[t] = ex + hist[t * nblocks + blockIdx.x];
-  const int tile = blockIdx.x * kSortTile;
+  const int first = blockIdx.x * kSortTile + w * (kSortItems * 64);
   const unsigned long long lt = lanemask_lt();
+  int key[kSortItems], val[kSortItems], loc[kSortItems];
+#pragma unroll
   for (int r = 0; r < kSortItems; ++r) {
-    const int idx = tile + r * kSortThreads + t;
+    const int idx = first + r * 64 + lane;
     const bool valid = idx < n;
-    const int k = valid ? kin[idx] : 0;
-    const int v = valid ? (vin ? vin[idx] : idx) : 0;
-    const unsigned d = (((unsigned)k) >> shift) & (kRadix - 1);
+    key[r] = valid ? kin[idx] : 0;
+    val[r] = valid ? (vin ? vin[idx] : idx) : 0;
+    const unsigned d = (((unsigned)key[r]) >> shift) & (kRadix - 1);
     const unsigned long long peers = match_digit(d, valid);
     const int rank = (int)__popcll(peers & lt);
-    woff[0][t] = 0;
-    woff[1][t] = 0;
-    woff[2][t] = 0;
-    woff[3][t] = 0;
-    __syncthreads();
-    if (valid && rank == 0) woff[w][d] = (int)__popcll(peers);
-    __syncthreads();
-    {
-      const int c0 = woff[0][t], c1 = woff[1][t], c2 = woff[2][t], c3 = woff[3][t];
-      const int b = base[t];
-      woff[0][t] = b;
-      woff[1][t] = b + c0;
-      woff[2][t] = b + c0 + c1;
-      woff[3][t] = b + c0 + c1 + c2;
-      base[t] = b + c0 + c1 + c2 + c3;
+    int run = 0;
+    if (valid && rank == 0) {  // the group's first lane owns the wave's count of digit d
+      run = woff[w][d];
+      woff[w][d] = run + (int)__popcll(peers);
     }
-    __syncthreads();
-    if (valid) {
-      const int pos = woff[w][d] + rank;
-      kout[pos] = k;
-      vout[pos] = v;
+    run = __shfl(run, valid ? (int)__builtin_ctzll(peers) : lane, 64);
+    loc[r] = run + rank;
+  }
+  __syncthreads();
+  {
+    int b = base[t];
+#pragma unroll
+    for (int k = 0; k < kWaves; ++k) {
+      const int c = woff[k][t];
+      woff[k][t] = b;
+      b += c;
     }
-    __syncthreads();
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < kSortItems; ++r) {
+    if (first + r * 64 + lane < n) {
+      const unsigned d = (((unsigned)key[r]) >> shift) & (kRadix - 1);
+      const int pos = woff[w][d] + loc[r];
+      kout[pos] = key[r];
+      vout[pos] = val[r];
+    }
   }
 }
 

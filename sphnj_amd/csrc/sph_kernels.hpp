@@ -440,7 +440,7 @@ struct TilePlan {
 
 // Row table of every sub-column (one wave per global sub-column): f_lb[k] for row k = the
 // suffix minimum of the non-empty rows' starts, with the sub-column's end past the last
-// non-empty row.
+// non-empty row.  The waves of even sub-columns also build their cell column's f_swp.
 template <class Pol>
 __global__ void __launch_bounds__(256) k_subcol_rows(Dev<Pol> d, int G, const int *col_case) {
   const int g = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -449,53 +449,93 @@ __global__ void __launch_bounds__(256) k_subcol_rows(Dev<Pol> d, int G, const in
   const int sx = g - cp.col_off, ny2 = 2 * cp.ny;
   const int kb = 4 * cp.cell_off + sx * ny2;
   int *lb = d.f_lb + cp.lb_off + sx * (ny2 + 1);
-  int end = 0;
-  for (int k = lane; k < ny2; k += 64) {
-    const int2 r = d.f_srange[kb + k];
-    if (r.y > r.x) end = max(end, r.y);
+  // each lane takes a contiguous run of rows, [k0, k1), read kU at a time so the loads overlap
+  constexpr int kU = 8;
+  const int per = (ny2 + 63) >> 6, k0 = min(lane * per, ny2), k1 = min(k0 + per, ny2);
+  int lmin = 0x7fffffff, end = 0;
+  for (int ka = k0; ka < k1; ka += kU) {
+    int2 r[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) r[u] = ka + u < k1 ? d.f_srange[kb + ka + u] : make_int2(0, 0);
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+      if (r[u].y > r[u].x) {
+        lmin = min(lmin, r[u].x);
+        end = max(end, r[u].y);
+      }
   }
   for (int o = 32; o > 0; o >>= 1) end = max(end, __shfl_xor(end, o, 64));
+  for (int o = 1; o < 64; o <<= 1) {  // suffix minimum over the lanes
+    const int u = __shfl_down(lmin, o, 64);
+    if (lane + o < 64) lmin = min(lmin, u);
+  }
+  int run = __shfl_down(lmin, 1, 64);
+  run = min(lane == 63 ? 0x7fffffff : run, end);  // rows past the last non-empty one: the end
+  for (int ka = k1 - 1; ka >= k0; ka -= kU) {
+    int2 r[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) r[u] = ka - u >= k0 ? d.f_srange[kb + ka - u] : make_int2(0, 0);
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+      if (ka - u >= k0) {
+        if (r[u].y > r[u].x) run = min(run, r[u].x);
+        lb[ka - u] = run;
+      }
+  }
+  if (lane == 0) lb[ny2] = end;
   if ((sx & 1) == 0) {  // the wave of an even sub-column also builds its cell column's f_swp
     const int qc = cp.cell_off + (sx >> 1) * cp.ny;
     int *swp = d.f_swp + cp.swp_off + (sx >> 1) * (cp.ny + 1);
-    int run = 0;
-    for (int c0 = 0; c0 < cp.ny; c0 += 64) {
-      const int k = c0 + lane;
-      int v = 0;
-      if (k < cp.ny) {
-        const int2 a = d.s_range[qc + k], w = d.w_range[qc + k];
-        v = (a.y > a.x || w.y > w.x) ? 1 : 0;
+    const int pc = (cp.ny + 63) >> 6, c0 = min(lane * pc, cp.ny), c1 = min(c0 + pc, cp.ny);
+    int n = 0;
+    for (int ka = c0; ka < c1; ka += kU) {
+      int2 a[kU], w[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        a[u] = ka + u < c1 ? d.s_range[qc + ka + u] : make_int2(0, 0);
+        w[u] = ka + u < c1 ? d.w_range[qc + ka + u] : make_int2(0, 0);
       }
-      v = wave_incl_sum(v, lane) + run;
-      if (k < cp.ny) swp[k + 1] = v;
-      run = __shfl(v, 63, 64);
+#pragma unroll
+      for (int u = 0; u < kU; ++u) n += (a[u].y > a[u].x || w[u].y > w[u].x) ? 1 : 0;
+    }
+    int acc = wave_incl_sum(n, lane) - n;
+    for (int ka = c0; ka < c1; ka += kU) {
+      int2 a[kU], w[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        a[u] = ka + u < c1 ? d.s_range[qc + ka + u] : make_int2(0, 0);
+        w[u] = ka + u < c1 ? d.w_range[qc + ka + u] : make_int2(0, 0);
+      }
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (ka + u < c1) {
+          acc += (a[u].y > a[u].x || w[u].y > w[u].x) ? 1 : 0;
+          swp[ka + u + 1] = acc;
+        }
     }
     if (lane == 0) swp[0] = 0;
   }
-  int carry = end;
-  for (int c0 = ((ny2 - 1) >> 6) << 6; c0 >= 0; c0 -= 64) {
-    const int k = c0 + lane;
-    int v = 0x7fffffff;
-    if (k < ny2) {
-      const int2 r = d.f_srange[kb + k];
-      if (r.y > r.x) v = r.x;
-    }
-    for (int o = 1; o < 64; o <<= 1) {
-      const int u = __shfl_down(v, o, 64);
-      if (lane + o < 64) v = min(v, u);
-    }
-    v = min(v, carry);
-    carry = __shfl(v, 0, 64);
-    if (k < ny2) lb[k] = v;
-  }
-  if (lane == 0) lb[ny2] = end;
 }
+
+// The tiles of one block size: counts per sub-column, their exclusive scan (G + 1 entries, the
+// total last) and the plans.  The planning kernels take two jobs and pick one by blockIdx.y,
+// so the tiles of the pair and the shift block sizes are planned by the same launches.
+struct PlanJob {
+  int B;
+  int *cnt, *off;
+  TilePlan *plans;
+};
+struct PlanJobs {
+  PlanJob j[2];
+};
 
 // Tiles per sub-column (one wave per global sub-column, a lane per chunk).
 template <class Pol>
-__global__ void __launch_bounds__(256) k_tile_count(Dev<Pol> d, int G, const int *col_case, int B, int *cnt) {
+__global__ void __launch_bounds__(256) k_tile_count(Dev<Pol> d, int G, const int *col_case, PlanJobs jobs) {
   const int g = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (g >= G) return;
+  const int B = jobs.j[blockIdx.y].B;
+  int *cnt = jobs.j[blockIdx.y].cnt;
   const auto &cp = d.cases[col_case[g]];
   const int ny2 = 2 * cp.ny;
   const int *lb = d.f_lb + cp.lb_off + (g - cp.col_off) * (ny2 + 1);
@@ -506,10 +546,12 @@ __global__ void __launch_bounds__(256) k_tile_count(Dev<Pol> d, int G, const int
   if (lane == 0) cnt[g] = t;
 }
 
-// Exclusive scan of the per-sub-column tile counts (one block of 1024 threads); the total
-// goes to off[G].
-__global__ void __launch_bounds__(1024) k_tile_scan(const int *cnt, int G, int *off) {
+// Exclusive scan of the per-sub-column tile counts (one block of 1024 threads per job); the
+// total goes to off[G].
+__global__ void __launch_bounds__(1024) k_tile_scan(PlanJobs jobs, int G) {
   __shared__ int part[1024];
+  const int *cnt = jobs.j[blockIdx.x].cnt;
+  int *off = jobs.j[blockIdx.x].off;
   const int t = threadIdx.x, per = (G + 1023) / 1024, a = min(t * per, G), b = min(a + per, G);
   int s = 0;
   for (int k = a; k < b; ++k) s += cnt[k];
@@ -531,10 +573,12 @@ __global__ void __launch_bounds__(1024) k_tile_scan(const int *cnt, int G, int *
 
 // Plans of the tiles of size B (one wave per global sub-column, a lane per chunk).
 template <class Pol>
-__global__ void __launch_bounds__(256) k_tile_plan(Dev<Pol> d, int G, const int *col_case, int B, const int *off,
-                                                   TilePlan *plans) {
+__global__ void __launch_bounds__(256) k_tile_plan(Dev<Pol> d, int G, const int *col_case, PlanJobs jobs) {
   const int g = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (g >= G) return;
+  const int B = jobs.j[blockIdx.y].B;
+  const int *off = jobs.j[blockIdx.y].off;
+  TilePlan *plans = jobs.j[blockIdx.y].plans;
   const int c = col_case[g];
   const auto &cp = d.cases[c];
   const int sx = g - cp.col_off, cx = sx >> 1, ny2 = 2 * cp.ny;

@@ -592,21 +592,29 @@ struct Engine : EngineBase {
       k_subcol_rows<Pol><<<nblk(G, 4), 256, 0, stream>>>(d, G, col_case);
       LAUNCH_CHECK();
       int rc;
-      if ((rc = ensure_plans(blk_pair)) || (rc = ensure_plans(blk_shift))) return rc;
+      if ((rc = ensure_plans(blk_pair, blk_shift))) return rc;
     }
     return SPHNJ_OK;
   }
 
   // Tile plans of size B for the current hash (the row table and tile counts are built by
   // the CELLS stage).
-  int ensure_plans(int B) {
-    const int b = bidx(B);
-    if (plan_ok[b] || d.nf == 0) return SPHNJ_OK;
-    k_tile_count<Pol><<<nblk(G, 4), 256, 0, stream>>>(d, G, col_case, B, tile_cnt[b]);
-    k_tile_scan<<<1, 1024, 0, stream>>>(tile_cnt[b], G, tile_off[b]);
-    k_tile_plan<Pol><<<nblk(G, 4), 256, 0, stream>>>(d, G, col_case, B, tile_off[b], plans[b]);
+  int ensure_plans(int B1, int B2 = 0) {
+    if (d.nf == 0) return SPHNJ_OK;
+    PlanJobs jobs{};
+    int nj = 0;
+    for (int B : {B1, B2}) {
+      if (B == 0) continue;
+      const int b = bidx(B);
+      if (plan_ok[b] || (nj == 1 && jobs.j[0].B == B)) continue;
+      jobs.j[nj++] = PlanJob{B, tile_cnt[b], tile_off[b], plans[b]};
+    }
+    if (nj == 0) return SPHNJ_OK;
+    k_tile_count<Pol><<<dim3(nblk(G, 4), nj), 256, 0, stream>>>(d, G, col_case, jobs);
+    k_tile_scan<<<nj, 1024, 0, stream>>>(jobs, G);
+    k_tile_plan<Pol><<<dim3(nblk(G, 4), nj), 256, 0, stream>>>(d, G, col_case, jobs);
     LAUNCH_CHECK();
-    plan_ok[b] = true;
+    for (int k = 0; k < nj; ++k) plan_ok[bidx(jobs.j[k].B)] = true;
     return SPHNJ_OK;
   }
 

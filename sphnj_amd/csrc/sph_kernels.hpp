@@ -1062,26 +1062,26 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
         const R by = yi - cell_shift_y<Pol>(b, cp);
         const int q = cp.cell_off + jx * cp.ny + jy;
         const int2 rs = d.s_range[q];
-        for (int k = rs.x; k < rs.y; ++k) {  // fluidSolidCalculate: rho_j = rho_i
+        for_range<2>(rs.x, rs.y, [&](int k) {  // fluidSolidCalculate: rho_j = rho_i
           const int j = d.s_sorted[k];
           const R dx = d.s_vx[j] - bx, dy = d.s_vy[j] - by;
           const R r2 = dx * dx + dy * dy;
-          if (!(r2 < K.rc2)) continue;
+          if (!(r2 < K.rc2)) return;
           const R w = K.w(r2);
           riemann_pair<R, VISCID>(dx, dy, r2, w, ui, vi, (R)d.su[j], (R)d.sv[j], rhoi, rhoi_inv, rhoi, pi, d.s_p[j],
                                   cs, csinv, mu8, eps_h2, S);
-        }
+        });
         const int2 rw = d.w_range[q];
-        for (int s = rw.x; s < rw.y; ++s) {  // fluidVirtCalculate: rho_j = rho0[wall phase]
-          const R dx = d.w_vx[s] - bx, dy = d.w_vy[s] - by;
+        for_range<4>(rw.x, rw.y, [&](int s) {  // fluidVirtCalculate: rho_j = rho0[wall phase]
+          const R dx = ld32(d.w_vx, s) - bx, dy = ld32(d.w_vy, s) - by;
           const R r2 = dx * dx + dy * dy;
-          if (!(r2 < K.rc2)) continue;
+          if (!(r2 < K.rc2)) return;
           const R w = K.w(r2);
           const int phj = d.w_phase[s];
           riemann_pair<R, VISCID>(dx, dy, r2, w, ui, vi, d.w_u[s], d.w_v[s], rhoi, rhoi_inv, cp.rho0[phj], pi,
                                   d.w_p[s], cs, csinv, mu8, eps_h2, S);
           if (!(liq && phj == 1)) S.col += r2 * w;
-        }
+        });
       }
     }
     d.f_col[i] = S.col;
@@ -1295,12 +1295,12 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
       const R by = yi - cell_shift_y<Pol>(b, cp);
       const int q = cp.cell_off + jx * cp.ny + jy;
       const int2 rs = d.s_range[q];
-      for (int k = rs.x; k < rs.y; ++k) {
+      for_range<2>(rs.x, rs.y, [&](int k) {
         const int j = d.s_sorted[k];
         add(d.s_vx[j] - bx, d.s_vy[j] - by);
-      }
+      });
       const int2 rw = d.w_range[q];
-      for (int s = rw.x; s < rw.y; ++s) add(d.w_vx[s] - bx, d.w_vy[s] - by);
+      for_range<4>(rw.x, rw.y, [&](int s) { add(ld32(d.w_vx, s) - bx, ld32(d.w_vy, s) - by); });
     }
   }
   if constexpr (Pol::MIXED) {

@@ -100,7 +100,7 @@ struct Dev {
   int2 *f_srange;  // per global sub-cell (4 per cell): [start, end) slot range, empty (0, 0)
   // per sub-column, 2 ny + 1 entries: first slot of sub-row k or later; the sub-column's end
   int *f_lb;
-  // per cell column, ny + 1 entries: cells among rows [0, k) that hold a solid or wall particle
+  // per cell column, ny + 1 entries: cells among rows [0, k) that hold a solid particle
   int *f_swp;
   // max |u|^2 per case as the bit pattern of a non-negative double: blocks reduce into
   // kSpread slots per case (slot s at umax_slot[(c kSpread + s) kSlotStride], one cache line
@@ -130,6 +130,9 @@ struct Dev {
   int8_t *w_phase, *w_bc;
   int *w_orig, *w_cxy;
   int2 *w_range;
+  // per cell column (case offset swp_off), ny + 1 entries: first wall slot of cell row k or
+  // later, the column's end last (walls are sorted by cell once)
+  int *w_lb;
   // status word (SPHNJ_ST_* bits)
   unsigned *status;
 };

@@ -432,15 +432,42 @@ struct TilePlan {
   int sx, cx;            // sub-column and cell column of the tile
   int sc0, nsc, r0, nr;  // candidate sub-columns and sub-rows
   int cy_ref;            // cell row of the M32 shift frame
-  int sw;                // a solid or wall particle lies in the tile's 3x3 cells
+  int sw;                // wall (bit 0) / solid (bit 1) particles lie in the tile's 3x3 cells
   int total;             // candidate particles (sum of the span lengths)
   int lo[kTileCols], len[kTileCols], row[kTileCols];  // slot span; f_lb index of (sc0 + a, r0)
   int gsz, gmul;         // staging: threads per span B / nsc; ceil(2^20 / gsz) (t / gsz = t gmul >> 20)
 };
 
+// Wall row table w_lb of every cell column (once, after the walls are sorted; a thread per
+// even global sub-column): the suffix minimum of the non-empty cells' starts, with the
+// column's end past its last non-empty cell.
+template <class Pol>
+__global__ void __launch_bounds__(256) k_wall_rows(Dev<Pol> d, int G, const int *col_case) {
+  const int g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= G) return;
+  const auto &cp = d.cases[col_case[g]];
+  const int sx = g - cp.col_off;
+  if (sx & 1) return;
+  const int qc = cp.cell_off + (sx >> 1) * cp.ny;
+  int *lb = d.w_lb + cp.swp_off + (sx >> 1) * (cp.ny + 1);
+  int end = 0;
+  for (int k = 0; k < cp.ny; ++k) {
+    const int2 r = d.w_range[qc + k];
+    if (r.y > r.x) end = max(end, r.y);
+  }
+  int run = end;
+  for (int k = cp.ny - 1; k >= 0; --k) {
+    const int2 r = d.w_range[qc + k];
+    if (r.y > r.x) run = min(run, r.x);
+    lb[k] = run;
+  }
+  lb[cp.ny] = end;
+}
+
 // Row table of every sub-column (one wave per global sub-column): f_lb[k] for row k = the
 // suffix minimum of the non-empty rows' starts, with the sub-column's end past the last
-// non-empty row.  The waves of even sub-columns also build their cell column's f_swp.
+// non-empty row.  The waves of even sub-columns also build their cell column's f_swp (when
+// the batch has solids).
 template <class Pol>
 __global__ void __launch_bounds__(256) k_subcol_rows(Dev<Pol> d, int G, const int *col_case) {
   const int g = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
@@ -483,33 +510,27 @@ __global__ void __launch_bounds__(256) k_subcol_rows(Dev<Pol> d, int G, const in
       }
   }
   if (lane == 0) lb[ny2] = end;
-  if ((sx & 1) == 0) {  // the wave of an even sub-column also builds its cell column's f_swp
+  if ((sx & 1) == 0 && d.ns > 0) {  // the wave of an even sub-column also builds its column's f_swp
     const int qc = cp.cell_off + (sx >> 1) * cp.ny;
     int *swp = d.f_swp + cp.swp_off + (sx >> 1) * (cp.ny + 1);
     const int pc = (cp.ny + 63) >> 6, c0 = min(lane * pc, cp.ny), c1 = min(c0 + pc, cp.ny);
     int n = 0;
     for (int ka = c0; ka < c1; ka += kU) {
-      int2 a[kU], w[kU];
+      int2 a[kU];
 #pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        a[u] = ka + u < c1 ? d.s_range[qc + ka + u] : make_int2(0, 0);
-        w[u] = ka + u < c1 ? d.w_range[qc + ka + u] : make_int2(0, 0);
-      }
+      for (int u = 0; u < kU; ++u) a[u] = ka + u < c1 ? d.s_range[qc + ka + u] : make_int2(0, 0);
 #pragma unroll
-      for (int u = 0; u < kU; ++u) n += (a[u].y > a[u].x || w[u].y > w[u].x) ? 1 : 0;
+      for (int u = 0; u < kU; ++u) n += a[u].y > a[u].x ? 1 : 0;
     }
     int acc = wave_incl_sum(n, lane) - n;
     for (int ka = c0; ka < c1; ka += kU) {
-      int2 a[kU], w[kU];
+      int2 a[kU];
 #pragma unroll
-      for (int u = 0; u < kU; ++u) {
-        a[u] = ka + u < c1 ? d.s_range[qc + ka + u] : make_int2(0, 0);
-        w[u] = ka + u < c1 ? d.w_range[qc + ka + u] : make_int2(0, 0);
-      }
+      for (int u = 0; u < kU; ++u) a[u] = ka + u < c1 ? d.s_range[qc + ka + u] : make_int2(0, 0);
 #pragma unroll
       for (int u = 0; u < kU; ++u)
         if (ka + u < c1) {
-          acc += (a[u].y > a[u].x || w[u].y > w[u].x) ? 1 : 0;
+          acc += a[u].y > a[u].x ? 1 : 0;
           swp[ka + u + 1] = acc;
         }
     }
@@ -586,7 +607,7 @@ __global__ void __launch_bounds__(256) k_tile_plan(Dev<Pol> d, int G, const int 
   const int s0 = lbc[sx * (ny2 + 1)], n = lbc[sx * (ny2 + 1) + ny2] - s0, nch = (n + B - 1) / B;
   const int sc0 = max(2 * cx - 2, 0), sc1 = min(2 * cx + 3, 2 * cp.nx - 1), nsc = sc1 - sc0 + 1;
   const int wc0 = max(cx - 1, 0), wc1 = min(cx + 1, cp.nx - 1);
-  const int *swp = d.f_swp + cp.swp_off;
+  const int *swp = d.f_swp + cp.swp_off, *wlb = d.w_lb + cp.swp_off;
   int tile = off[g];
   for (int k0 = 0; k0 < nch; k0 += 64) {
     const int k = k0 + lane, cstart = s0 + k * B, ccount = k < nch ? min(B, n - k * B) : 0;
@@ -612,10 +633,15 @@ __global__ void __launch_bounds__(256) k_tile_plan(Dev<Pol> d, int G, const int 
         P.row[a] = row;
         total += len;
       }
-      // a solid or wall particle in the 3x3 cells of the tile's lanes
+      // wall (bit 0) and solid (bit 1) particles in the 3x3 cells of the tile's lanes
       const int wr0 = max((sy0 >> 1) - 1, 0), wr1 = min((sy1 >> 1) + 1, cp.ny - 1);
-      int sw = 0;
-      for (int col = wc0; col <= wc1; ++col) sw += swp[col * (cp.ny + 1) + wr1 + 1] - swp[col * (cp.ny + 1) + wr0];
+      int nw = 0, ns = 0;
+      for (int col = wc0; col <= wc1; ++col) {
+        const int q = col * (cp.ny + 1);
+        nw += wlb[q + wr1 + 1] - wlb[q + wr0];
+        if (d.ns > 0) ns += swp[q + wr1 + 1] - swp[q + wr0];
+      }
+      const int sw = (nw > 0 ? 1 : 0) | (ns > 0 ? 2 : 0);
       P.c = c;
       P.start = start;
       P.count = count;
@@ -626,7 +652,7 @@ __global__ void __launch_bounds__(256) k_tile_plan(Dev<Pol> d, int G, const int 
       P.r0 = r0;
       P.nr = nr;
       P.cy_ref = (sy0 + sy1) >> 2;
-      P.sw = sw > 0 ? 1 : 0;
+      P.sw = sw;
       P.total = total;
       P.gsz = B / nsc;
       P.gmul = ((1 << 20) + P.gsz - 1) / P.gsz;
@@ -1162,8 +1188,12 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
   using R = typename Pol::R;
   using R2 = typename V2<R>::T;
   constexpr int CAP = ShiftCap<BLOCK>::value;
-  __shared__ R2 s_pos[CAP + 4];  // + far-away padding read by the masked tail of a window
+  // walls of the tile's 3x3 cell columns over its cell rows follow the fluid candidates
+  constexpr int WCAP = Pol::MIXED ? 2 * BLOCK : BLOCK, WROWS = TileRows<BLOCK>::value / 2 + 2;
+  // fluid candidates, padding, walls, padding (the padding is far away: read by masked tails)
+  __shared__ R2 s_pos[CAP + WCAP + 8];
   __shared__ int s_lb[kTileCols][TileRows<BLOCK>::value + 1];
+  __shared__ int s_wlb[3][WROWS + 1];  // LDS index of the first wall of cell row wcr0 + k or later
   __shared__ float s_dmax[BLOCK / 64];
   const int nt = *ntiles, tile = xcd_tile(blockIdx.x, nt);
   if (tile >= nt) return;
@@ -1217,6 +1247,45 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
         });
     if (t < 4) s_pos[P.total + t] = R2{R(1e18), R(1e18)};
   }
+  // walls (static, sorted by cell): the spans of cell columns cx - 1 .. cx + 1 over the tile's
+  // cell rows [wcr0, wcr1]
+  const int wcr0 = P.r0 >> 1, nwr = ((P.r0 + P.nr - 1) >> 1) - wcr0 + 1;
+  int wq[3], wlo[3], wlen[3];
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int col = P.cx - 1 + a;
+    wq[a] = wlo[a] = wlen[a] = 0;
+    if ((P.sw & 1) && col >= 0 && col < cp.nx) {
+      wq[a] = cp.swp_off + col * (cp.ny + 1) + wcr0;
+      wlo[a] = d.w_lb[wq[a]];
+      wlen[a] = d.w_lb[wq[a] + nwr] - wlo[a];
+    }
+  }
+  const int nwall = wlen[0] + wlen[1] + wlen[2], wbase = P.total + 4;
+  const bool wstaged = staged && nwall > 0 && P.total + nwall <= CAP + WCAP && nwr <= WROWS;
+  if (wstaged) {
+    if (t <= nwr) {
+      int b = wbase;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        s_wlb[a][t] = wlen[a] > 0 ? d.w_lb[wq[a] + t] - wlo[a] + b : b;
+        b += wlen[a];
+      }
+    }
+    const R fdx = cp.cdx_r, fdy = cp.cdy_r, gs = cp.gscale;
+    for (int e = t; e < nwall; e += BLOCK) {
+      const int a = e < wlen[0] ? 0 : (e < wlen[0] + wlen[1] ? 1 : 2);
+      const int g = wlo[a] + e - (a > 0 ? wlen[0] : 0) - (a > 1 ? wlen[1] : 0);
+      const R wx = ld32(d.w_vx, g), wy = ld32(d.w_vy, g);
+      if constexpr (Pol::MIXED) {  // tile frame, scaled units
+        const int wk = ld32(d.w_cxy, g);
+        s_pos[wbase + e] = R2{(wx + (float)(cxy_x(wk) - P.cx) * fdx) * gs, (wy + (float)(cxy_y(wk) - P.cy_ref) * fdy) * gs};
+      } else {
+        s_pos[wbase + e] = R2{wx, wy};
+      }
+    }
+    if (t < 4) s_pos[wbase + nwall + t] = R2{R(1e18), R(1e18)};
+  }
   for (int o = 32; o > 0; o >>= 1) dmax = max(dmax, __shfl_xor(dmax, o, 64));
   if ((t & 63) == 0) s_dmax[t >> 6] = __int_as_float(dmax);
   __syncthreads();
@@ -1258,24 +1327,26 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
     ys = o.y;
   }
   const R rc2s = cp.rc2s;
+  // one staged candidate (fluid or wall); w = 0 outside the cut-off
+  auto staged_add = [&](int l, bool in) {
+    const R2 pj = s_pos[l];
+    const R dx = pj.x - xs, dy = pj.y - ys, r2 = dx * dx + dy * dy;
+    if constexpr (Pol::MIXED) {
+      const R e = __builtin_amdgcn_exp2f(-r2);
+      const R w = (in & (r2 < rc2s)) ? e : R(0);
+      gxs += dx * w;
+      gys += dy * w;
+    } else {
+      const R wk = K.w(r2);
+      const R w = in ? wk : R(0);
+      gx += dx * w;
+      gy += dy * w;
+    }
+  };
   for_windows(P, cp, sy, cy, lp, Rp, [&](int sc, int ra, int rb) {
     if (staged) {
       const int sa = sc - P.sc0;
-      for_range_masked<4>(s_lb[sa][ra - P.r0], s_lb[sa][rb + 1 - P.r0], [&](int l, bool in) {
-        const R2 pj = s_pos[l];
-        const R dx = pj.x - xs, dy = pj.y - ys, r2 = dx * dx + dy * dy;
-        if constexpr (Pol::MIXED) {
-          const R e = __builtin_amdgcn_exp2f(-r2);
-          const R w = (in & (r2 < rc2s)) ? e : R(0);
-          gxs += dx * w;
-          gys += dy * w;
-        } else {
-          const R wk = K.w(r2);
-          const R w = in ? wk : R(0);
-          gx += dx * w;
-          gy += dy * w;
-        }
-      });
+      for_range_masked<4>(s_lb[sa][ra - P.r0], s_lb[sa][rb + 1 - P.r0], staged_add);
     } else {  // global views, relative to their own cell: one run per cell row
       const R bx = xi - cell_shift_x<Pol>((sc >> 1) - cx, cp);
       for (int cr = ra >> 1; cr <= (rb >> 1); ++cr) {
@@ -1285,7 +1356,25 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
       }
     }
   });
+  // walls of the 3x3 cells (cell column by cell column, rows in order), then solids
+  if (wstaged) {
+    const int lr0 = max(cy - 1, 0) - wcr0, lr1 = min(cy + 1, cp.ny - 1) - wcr0;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) for_range_masked<4>(s_wlb[a][lr0], s_wlb[a][lr1 + 1], staged_add);
+  }
   for (int a = -1; a <= 1 && P.sw; ++a) {
+    const int jx = cx + a;
+    if (jx < 0 || jx >= cp.nx) continue;
+    const R bx = xi - cell_shift_x<Pol>(a, cp);
+    for (int b = -1; b <= 1 && (P.sw & 1) && !wstaged; ++b) {
+      const int jy = cy + b;
+      if (jy < 0 || jy >= cp.ny) continue;
+      const R by = yi - cell_shift_y<Pol>(b, cp);
+      const int2 rw = d.w_range[cp.cell_off + jx * cp.ny + jy];
+      for_range<4>(rw.x, rw.y, [&](int s) { add(ld32(d.w_vx, s) - bx, ld32(d.w_vy, s) - by); });
+    }
+  }
+  for (int a = -1; a <= 1 && (P.sw & 2); ++a) {
     const int jx = cx + a;
     if (jx < 0 || jx >= cp.nx) continue;
     const R bx = xi - cell_shift_x<Pol>(a, cp);
@@ -1293,14 +1382,11 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
       const int jy = cy + b;
       if (jy < 0 || jy >= cp.ny) continue;
       const R by = yi - cell_shift_y<Pol>(b, cp);
-      const int q = cp.cell_off + jx * cp.ny + jy;
-      const int2 rs = d.s_range[q];
+      const int2 rs = d.s_range[cp.cell_off + jx * cp.ny + jy];
       for_range<2>(rs.x, rs.y, [&](int k) {
         const int j = d.s_sorted[k];
         add(d.s_vx[j] - bx, d.s_vy[j] - by);
       });
-      const int2 rw = d.w_range[q];
-      for_range<4>(rw.x, rw.y, [&](int s) { add(ld32(d.w_vx, s) - bx, ld32(d.w_vy, s) - by); });
     }
   }
   if constexpr (Pol::MIXED) {

@@ -457,7 +457,7 @@ struct Engine : EngineBase {
     if ((rc = dalloc(&d.wx, nw)) || (rc = dalloc(&d.wy, nw)) || (rc = dalloc(&d.w_u, nw)) || (rc = dalloc(&d.w_v, nw)) ||
         (rc = dalloc(&d.w_p, nw)) || (rc = dalloc(&d.w_rho, nw)) || (rc = dalloc(&d.w_n, 2 * (size_t)nw)) ||
         (rc = dalloc(&d.w_phase, nw)) || (rc = dalloc(&d.w_bc, nw)) || (rc = dalloc(&d.w_orig, nw)) ||
-        (rc = dalloc(&d.w_cxy, nw)) || (rc = dalloc(&d.w_range, nc)))
+        (rc = dalloc(&d.w_cxy, nw)) || (rc = dalloc(&d.w_range, nc)) || (rc = dalloc(&d.w_lb, (size_t)swp_total)))
       return rc;
     if constexpr (Pol::MIXED) {
       if ((rc = dalloc(&d.w_vx, nw)) || (rc = dalloc(&d.w_vy, nw))) return rc;
@@ -505,6 +505,8 @@ struct Engine : EngineBase {
         k_clear_ranges<<<nblk(nc, 256), 256, 0, stream>>>(d.w_range, nc, nullptr, 0);
         LAUNCH_CHECK();
       }
+      k_wall_rows<Pol><<<nblk(G, 256), 256, 0, stream>>>(d, G, col_case);
+      LAUNCH_CHECK();
       HIPC(hipStreamSynchronize(stream));
     }
 

@@ -726,6 +726,51 @@ __device__ __forceinline__ void tile_stage(const TilePlan &P, const int *f_lb, i
   }
 }
 
+// The walls of a tile's 3x3 cell columns (cx - 1 .. cx + 1) over its cell rows
+// [wcr0, wcr0 + nwr): one span per column in the wall order (walls are sorted by cell once,
+// and w_lb gives each column's first wall of a cell row or later).
+struct WallSpans {
+  int wcr0, nwr, n;
+  int q[3], lo[3], len[3];
+};
+template <class Pol, class CP>
+__device__ __forceinline__ WallSpans wall_spans(const Dev<Pol> &d, const TilePlan &P, const CP &cp) {
+  WallSpans W;
+  W.wcr0 = P.r0 >> 1;
+  W.nwr = ((P.r0 + P.nr - 1) >> 1) - W.wcr0 + 1;
+  W.n = 0;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const int col = P.cx - 1 + a;
+    W.q[a] = W.lo[a] = W.len[a] = 0;
+    if ((P.sw & 1) && col >= 0 && col < cp.nx) {
+      W.q[a] = cp.swp_off + col * (cp.ny + 1) + W.wcr0;
+      W.lo[a] = d.w_lb[W.q[a]];
+      W.len[a] = d.w_lb[W.q[a] + W.nwr] - W.lo[a];
+    }
+    W.n += W.len[a];
+  }
+  return W;
+}
+// Copies the wall spans to LDS from index base on: the row table (wlb[a][k] = LDS index of the
+// first wall of cell row wcr0 + k or later in column a) and, through st(l, g), the walls.
+template <int BLOCK, int WROWS, class Pol, class St>
+__device__ __forceinline__ void wall_stage(const Dev<Pol> &d, const WallSpans &W, int base, int (*wlb)[WROWS + 1],
+                                           St &&st) {
+  const int t = threadIdx.x;
+  if (t <= W.nwr) {
+    int b = base;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+      wlb[a][t] = W.len[a] > 0 ? d.w_lb[W.q[a] + t] - W.lo[a] + b : b;
+      b += W.len[a];
+    }
+  }
+  const int e1 = W.len[0], e2 = W.len[0] + W.len[1];
+  for (int e = t; e < W.n; e += BLOCK)  // (selects, not indexing: the spans stay in registers)
+    st(base + e, e < e1 ? W.lo[0] + e : (e < e2 ? W.lo[1] + e - e1 : W.lo[2] + e - e2));
+}
+
 // A lane's position relative to its hashed sub-cell, in sub-cell units.
 struct LanePos {
   float fx, fy;
@@ -913,10 +958,15 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
   using R = typename Pol::R;
   using R2 = typename V2<R>::T;
   constexpr int CAP = PairCap<Pol, BLOCK>::value;
-  // + padding read by the masked tail of a window: far away, finite, no flags
-  __shared__ R2 s_pos[CAP + 4], s_vel[CAP + 4], s_rp[CAP + 4];
-  __shared__ int s_fl[CAP + 4];
+  // walls of the tile's 3x3 cell columns over its cell rows follow the fluid candidates
+  // (a tile's fluid candidates rarely fill CAP, so the walls also use what they leave)
+  constexpr int WCAP = BLOCK / 4, WROWS = TileRows<BLOCK>::value / 2 + 2;
+  // fluid candidates, padding, walls, padding (the padding is far away, finite, without flags:
+  // read by masked tails)
+  __shared__ R2 s_pos[CAP + WCAP + 8], s_vel[CAP + WCAP + 8], s_rp[CAP + WCAP + 8];
+  __shared__ int s_fl[CAP + WCAP + 8];
   __shared__ int s_lb[kTileCols][TileRows<BLOCK>::value + 1];
+  __shared__ int s_wlb[3][WROWS + 1];  // LDS index of the first wall of cell row wcr0 + k or later
   __shared__ unsigned long long s_red[BLOCK / 64];
   __shared__ float s_dmax[BLOCK / 64];
   const int nt = *ntiles, tile = xcd_tile(blockIdx.x, nt);
@@ -960,6 +1010,33 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
       s_vel[P.total + t] = R2{R(0), R(0)};
       s_rp[P.total + t] = R2{R(1), R(0)};
       s_fl[P.total + t] = 0;
+    }
+  }
+  // walls (fluidVirtCalculate: rho_j = rho0[wall phase]; a wall is never FREE_SURFACE)
+  const auto &cw = d.cases[c];
+  const WallSpans W = wall_spans(d, P, cw);
+  const int wbase = P.total + 4;
+  const bool wstaged = staged && W.n > 0 && P.total + W.n <= CAP + WCAP && W.nwr <= WROWS;
+  if (wstaged) {
+    const R fdx = cw.cdx_r, fdy = cw.cdy_r;
+    wall_stage<BLOCK, WROWS>(d, W, wbase, s_wlb, [&](int l, int g) {
+      const R wx = ld32(d.w_vx, g), wy = ld32(d.w_vy, g);
+      const int phj = ld32(d.w_phase, g);
+      if constexpr (Pol::MIXED) {  // tile frame
+        const int wk = ld32(d.w_cxy, g);
+        s_pos[l] = R2{wx + (float)(cxy_x(wk) - P.cx) * fdx, wy + (float)(cxy_y(wk) - P.cy_ref) * fdy};
+      } else {
+        s_pos[l] = R2{wx, wy};
+      }
+      s_vel[l] = R2{ld32(d.w_u, g), ld32(d.w_v, g)};
+      s_rp[l] = R2{cw.rho0[phj], ld32(d.w_p, g)};
+      s_fl[l] = phj;
+    });
+    if (t < 4) {
+      s_pos[wbase + W.n + t] = R2{R(1e18), R(1e18)};
+      s_vel[wbase + W.n + t] = R2{R(0), R(0)};
+      s_rp[wbase + W.n + t] = R2{R(1), R(0)};
+      s_fl[wbase + W.n + t] = 0;
     }
   }
   for (int o = 32; o > 0; o >>= 1) dmax = max(dmax, __shfl_xor(dmax, o, 64));
@@ -1036,17 +1113,11 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
       // the same non-zero terms in the same order as a pass over every candidate
       unsigned mk[kTileCols];
       int w0[kTileCols];
-#pragma unroll
-      for (int sa = 0; sa < kTileCols; ++sa) {
-        mk[sa] = 0u;
-        w0[sa] = 0;
-        int ra, rb;
-        if (!span_window(P, cp, sa, sy, cy, lp, Rp, ra, rb)) continue;
-        const int s0 = s_lb[sa][ra - P.r0], s1 = s_lb[sa][rb + 1 - P.r0];
-        w0[sa] = s0;
-        if (s1 - s0 > 32) {  // a long window: evaluate every candidate
+      // in-range mask of the run [s0, s1) (a run longer than 32 is evaluated right away)
+      auto mark = [&](int s0, int s1) -> unsigned {
+        if (s1 - s0 > 32) {
           for_range_masked<2>(s0, s1, [&](int l, bool in) { staged_j(Cut{}, l, in); });
-          continue;
+          return 0u;
         }
         unsigned m = 0u;
         for_range_masked<4>(s0, s1, [&](int l, bool in) {
@@ -1055,15 +1126,33 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
           const bool ok = in & (dx * dx + dy * dy < K.rc2);
           m |= (ok ? 1u : 0u) << ((l - s0) & 31);
         });
-        mk[sa] = m;
-      }
-#pragma unroll
-      for (int sa = 0; sa < kTileCols; ++sa) {
-        unsigned m = mk[sa];
+        return m;
+      };
+      auto eval = [&](unsigned m, int s0) {
         while (m) {
           const int b = __builtin_ctz(m);
           m &= m - 1u;
-          staged_j(Inside{}, w0[sa] + b, true);
+          staged_j(Inside{}, s0 + b, true);
+        }
+      };
+#pragma unroll
+      for (int sa = 0; sa < kTileCols; ++sa) {
+        mk[sa] = 0u;
+        w0[sa] = 0;
+        int ra, rb;
+        if (!span_window(P, cp, sa, sy, cy, lp, Rp, ra, rb)) continue;
+        const int s0 = s_lb[sa][ra - P.r0], s1 = s_lb[sa][rb + 1 - P.r0];
+        w0[sa] = s0;
+        mk[sa] = mark(s0, s1);
+      }
+#pragma unroll
+      for (int sa = 0; sa < kTileCols; ++sa) eval(mk[sa], w0[sa]);
+      if (wstaged) {  // walls of the 3x3 cells, cell column by cell column
+        const int lr0 = max(cy - 1, 0) - W.wcr0, lr1 = min(cy + 1, cp.ny - 1) - W.wcr0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+          const int s0 = s_wlb[a][lr0];
+          eval(mark(s0, s_wlb[a][lr1 + 1]), s0);
         }
       }
     }
@@ -1078,7 +1167,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
         rows(sc, ra, rb, bx, yi);
       }
     });
-    for (int a = -1; a <= 1 && P.sw; ++a) {
+    for (int a = -1; a <= 1 && (P.sw & 1) && !wstaged; ++a) {
       const int jx = cx + a;
       if (jx < 0 || jx >= cp.nx) continue;
       const R bx = xi - cell_shift_x<Pol>(a, cp);
@@ -1086,18 +1175,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
         const int jy = cy + b;
         if (jy < 0 || jy >= cp.ny) continue;
         const R by = yi - cell_shift_y<Pol>(b, cp);
-        const int q = cp.cell_off + jx * cp.ny + jy;
-        const int2 rs = d.s_range[q];
-        for_range<2>(rs.x, rs.y, [&](int k) {  // fluidSolidCalculate: rho_j = rho_i
-          const int j = d.s_sorted[k];
-          const R dx = d.s_vx[j] - bx, dy = d.s_vy[j] - by;
-          const R r2 = dx * dx + dy * dy;
-          if (!(r2 < K.rc2)) return;
-          const R w = K.w(r2);
-          riemann_pair<R, VISCID>(dx, dy, r2, w, ui, vi, (R)d.su[j], (R)d.sv[j], rhoi, rhoi_inv, rhoi, pi, d.s_p[j],
-                                  cs, csinv, mu8, eps_h2, S);
-        });
-        const int2 rw = d.w_range[q];
+        const int2 rw = d.w_range[cp.cell_off + jx * cp.ny + jy];
         for_range<4>(rw.x, rw.y, [&](int s) {  // fluidVirtCalculate: rho_j = rho0[wall phase]
           const R dx = ld32(d.w_vx, s) - bx, dy = ld32(d.w_vy, s) - by;
           const R r2 = dx * dx + dy * dy;
@@ -1107,6 +1185,26 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
           riemann_pair<R, VISCID>(dx, dy, r2, w, ui, vi, d.w_u[s], d.w_v[s], rhoi, rhoi_inv, cp.rho0[phj], pi,
                                   d.w_p[s], cs, csinv, mu8, eps_h2, S);
           if (!(liq && phj == 1)) S.col += r2 * w;
+        });
+      }
+    }
+    for (int a = -1; a <= 1 && (P.sw & 2); ++a) {
+      const int jx = cx + a;
+      if (jx < 0 || jx >= cp.nx) continue;
+      const R bx = xi - cell_shift_x<Pol>(a, cp);
+      for (int b = -1; b <= 1; ++b) {
+        const int jy = cy + b;
+        if (jy < 0 || jy >= cp.ny) continue;
+        const R by = yi - cell_shift_y<Pol>(b, cp);
+        const int2 rs = d.s_range[cp.cell_off + jx * cp.ny + jy];
+        for_range<2>(rs.x, rs.y, [&](int k) {  // fluidSolidCalculate: rho_j = rho_i
+          const int j = d.s_sorted[k];
+          const R dx = d.s_vx[j] - bx, dy = d.s_vy[j] - by;
+          const R r2 = dx * dx + dy * dy;
+          if (!(r2 < K.rc2)) return;
+          const R w = K.w(r2);
+          riemann_pair<R, VISCID>(dx, dy, r2, w, ui, vi, (R)d.su[j], (R)d.sv[j], rhoi, rhoi_inv, rhoi, pi, d.s_p[j],
+                                  cs, csinv, mu8, eps_h2, S);
         });
       }
     }
@@ -1189,7 +1287,8 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
   using R2 = typename V2<R>::T;
   constexpr int CAP = ShiftCap<BLOCK>::value;
   // walls of the tile's 3x3 cell columns over its cell rows follow the fluid candidates
-  constexpr int WCAP = Pol::MIXED ? 2 * BLOCK : BLOCK, WROWS = TileRows<BLOCK>::value / 2 + 2;
+  // (a tile's fluid candidates rarely fill CAP, so the walls also use what they leave)
+  constexpr int WCAP = BLOCK / 2, WROWS = TileRows<BLOCK>::value / 2 + 2;
   // fluid candidates, padding, walls, padding (the padding is far away: read by masked tails)
   __shared__ R2 s_pos[CAP + WCAP + 8];
   __shared__ int s_lb[kTileCols][TileRows<BLOCK>::value + 1];
@@ -1247,44 +1346,22 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
         });
     if (t < 4) s_pos[P.total + t] = R2{R(1e18), R(1e18)};
   }
-  // walls (static, sorted by cell): the spans of cell columns cx - 1 .. cx + 1 over the tile's
-  // cell rows [wcr0, wcr1]
-  const int wcr0 = P.r0 >> 1, nwr = ((P.r0 + P.nr - 1) >> 1) - wcr0 + 1;
-  int wq[3], wlo[3], wlen[3];
-#pragma unroll
-  for (int a = 0; a < 3; ++a) {
-    const int col = P.cx - 1 + a;
-    wq[a] = wlo[a] = wlen[a] = 0;
-    if ((P.sw & 1) && col >= 0 && col < cp.nx) {
-      wq[a] = cp.swp_off + col * (cp.ny + 1) + wcr0;
-      wlo[a] = d.w_lb[wq[a]];
-      wlen[a] = d.w_lb[wq[a] + nwr] - wlo[a];
-    }
-  }
-  const int nwall = wlen[0] + wlen[1] + wlen[2], wbase = P.total + 4;
-  const bool wstaged = staged && nwall > 0 && P.total + nwall <= CAP + WCAP && nwr <= WROWS;
+  // walls: staged after the fluid candidates' padding
+  const WallSpans W = wall_spans(d, P, cp);
+  const int wbase = P.total + 4;
+  const bool wstaged = staged && W.n > 0 && P.total + W.n <= CAP + WCAP && W.nwr <= WROWS;
   if (wstaged) {
-    if (t <= nwr) {
-      int b = wbase;
-#pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        s_wlb[a][t] = wlen[a] > 0 ? d.w_lb[wq[a] + t] - wlo[a] + b : b;
-        b += wlen[a];
-      }
-    }
     const R fdx = cp.cdx_r, fdy = cp.cdy_r, gs = cp.gscale;
-    for (int e = t; e < nwall; e += BLOCK) {
-      const int a = e < wlen[0] ? 0 : (e < wlen[0] + wlen[1] ? 1 : 2);
-      const int g = wlo[a] + e - (a > 0 ? wlen[0] : 0) - (a > 1 ? wlen[1] : 0);
+    wall_stage<BLOCK, WROWS>(d, W, wbase, s_wlb, [&](int l, int g) {
       const R wx = ld32(d.w_vx, g), wy = ld32(d.w_vy, g);
       if constexpr (Pol::MIXED) {  // tile frame, scaled units
         const int wk = ld32(d.w_cxy, g);
-        s_pos[wbase + e] = R2{(wx + (float)(cxy_x(wk) - P.cx) * fdx) * gs, (wy + (float)(cxy_y(wk) - P.cy_ref) * fdy) * gs};
+        s_pos[l] = R2{(wx + (float)(cxy_x(wk) - P.cx) * fdx) * gs, (wy + (float)(cxy_y(wk) - P.cy_ref) * fdy) * gs};
       } else {
-        s_pos[wbase + e] = R2{wx, wy};
+        s_pos[l] = R2{wx, wy};
       }
-    }
-    if (t < 4) s_pos[wbase + nwall + t] = R2{R(1e18), R(1e18)};
+    });
+    if (t < 4) s_pos[wbase + W.n + t] = R2{R(1e18), R(1e18)};
   }
   for (int o = 32; o > 0; o >>= 1) dmax = max(dmax, __shfl_xor(dmax, o, 64));
   if ((t & 63) == 0) s_dmax[t >> 6] = __int_as_float(dmax);
@@ -1358,7 +1435,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
   });
   // walls of the 3x3 cells (cell column by cell column, rows in order), then solids
   if (wstaged) {
-    const int lr0 = max(cy - 1, 0) - wcr0, lr1 = min(cy + 1, cp.ny - 1) - wcr0;
+    const int lr0 = max(cy - 1, 0) - W.wcr0, lr1 = min(cy + 1, cp.ny - 1) - W.wcr0;
 #pragma unroll
     for (int a = 0; a < 3; ++a) for_range_masked<4>(s_wlb[a][lr0], s_wlb[a][lr1 + 1], staged_add);
   }

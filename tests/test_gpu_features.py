@@ -22,9 +22,11 @@ def two_phase_small():
 @pytest.mark.parametrize("precision", ["p64", "m32"])
 def test_block_size_does_not_change_results(gpu, precision):
     """Every block size stages different tiles but performs the same per-particle sums in the
-    same order: P64 is bit-identical.  The M32 shift kernel re-bases staged positions on the
-    tile's frame, so its gradient differs by float rounding of a tile height (bounded here
-    at 1e-6 relative after 4 steps); the pair kernel and the classes stay identical."""
+    same order: P64 is bit-identical.  M32 re-bases the staged fluid and wall positions on the
+    tile's frame, so tiles of another size round separations differently, by float ulps of a
+    tile height.  In this nearly static start, where large pressure terms cancel to tiny
+    accelerations, that shows most in u and v (bounded here at 1e-5 relative after 4 steps,
+    well inside the 1e-4 M32 bar); the classes stay identical."""
     c = two_phase_small()
     ref = Simulation(c, precision)
     ref.step(4)
@@ -39,7 +41,7 @@ def test_block_size_does_not_change_results(gpu, precision):
             if precision == "p64" or k == "boundary_type":
                 assert np.array_equal(a["fluid"][k], b["fluid"][k]), (blk, k)
             else:
-                assert rel_l2(a["fluid"][k], b["fluid"][k]) <= 1e-6, (blk, k, rel_l2(a["fluid"][k], b["fluid"][k]))
+                assert rel_l2(a["fluid"][k], b["fluid"][k]) <= 1e-5, (blk, k, rel_l2(a["fluid"][k], b["fluid"][k]))
 
 
 def test_autotune_keeps_state(gpu):

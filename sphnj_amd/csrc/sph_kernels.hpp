@@ -959,12 +959,13 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
   using R2 = typename V2<R>::T;
   constexpr int CAP = PairCap<Pol, BLOCK>::value;
   // walls of the tile's 3x3 cell columns over its cell rows follow the fluid candidates
-  // (a tile's fluid candidates rarely fill CAP, so the walls also use what they leave)
-  constexpr int WCAP = BLOCK / 4, WROWS = TileRows<BLOCK>::value / 2 + 2;
+  // (a tile's fluid candidates rarely fill CAP: the walls use what they leave, so that the
+  // block stays within a fifth of the CU's LDS at 128 slots)
+  constexpr int WCAP = 0, WROWS = TileRows<BLOCK>::value / 2 + 2;
   // fluid candidates, padding, walls, padding (the padding is far away, finite, without flags:
   // read by masked tails)
   __shared__ R2 s_pos[CAP + WCAP + 8], s_vel[CAP + WCAP + 8], s_rp[CAP + WCAP + 8];
-  __shared__ int s_fl[CAP + WCAP + 8];
+  __shared__ uint8_t s_fl[CAP + WCAP + 8];  // phase | class << 4
   __shared__ int s_lb[kTileCols][TileRows<BLOCK>::value + 1];
   __shared__ int s_wlb[3][WROWS + 1];  // LDS index of the first wall of cell row wcr0 + k or later
   __shared__ unsigned long long s_red[BLOCK / 64];
@@ -991,7 +992,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
         P, d.f_lb, s_lb,
         [&](int g) {
           return Ld{R2{ld32(A.vx, g), ld32(A.vy, g)}, R2{ld32(A.vu, g), ld32(A.vv, g)},
-                    R2{ld32(A.vrho, g), ld32(A.p, g)}, (int)ld32(d.f_phase, g) | ((int)ld32(d.f_cls, g) << 8),
+                    R2{ld32(A.vrho, g), ld32(A.p, g)}, (int)ld32(d.f_phase, g) | ((int)ld32(d.f_cls, g) << 4),
                     ld32(reinterpret_cast<const int *>(A.cum), g), Pol::MIXED ? ld32(d.f_cxy, g) : 0};
         },
         [&](int l, const Ld &v, int) {
@@ -1002,7 +1003,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
             s_pos[l] = v.pos;
           s_vel[l] = v.vel;
           s_rp[l] = v.rp;
-          s_fl[l] = v.fl;
+          s_fl[l] = (uint8_t)v.fl;
           dmax = max(dmax, v.cum);
         });
     if (t < 4) {
@@ -1030,7 +1031,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
       }
       s_vel[l] = R2{ld32(d.w_u, g), ld32(d.w_v, g)};
       s_rp[l] = R2{cw.rho0[phj], ld32(d.w_p, g)};
-      s_fl[l] = phj;
+      s_fl[l] = (uint8_t)phj;
     });
     if (t < 4) {
       s_pos[wbase + W.n + t] = R2{R(1e18), R(1e18)};
@@ -1074,7 +1075,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
     auto fluid_j = [&](auto inside, bool self, R dx, R dy, R r2, R uj, R vj, R rhoj, R pj, int fl, bool in) {
       constexpr bool INSIDE = decltype(inside)::value;
       R w;
-      bool fs = liq & !self & ((fl >> 8) == 1);
+      bool fs = liq & !self & ((fl >> 4) == 1);
       if constexpr (INSIDE) {
         w = K.w_in(r2);
       } else {
@@ -1083,7 +1084,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
         fs = fs & in & (r2 < K.rc2);
       }
       riemann_pair<R, VISCID>(dx, dy, r2, w, ui, vi, uj, vj, rhoi, rhoi_inv, rhoj, pi, pj, cs, csinv, mu8, eps_h2, S);
-      if (!(liq && (fl & 0xff) == 1)) S.col += r2 * w;
+      if (!(liq && (fl & 0xf) == 1)) S.col += r2 * w;
       S.fsr = (fs && r2 < S.fsr) ? r2 : S.fsr;
     };
     using Cut = std::false_type;
@@ -1096,7 +1097,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
       for_range<2>(lbc[sc * ny21 + qa], lbc[sc * ny21 + qb + 1], [&](int s) {
         const R dx = A.vx[s] - bx, dy = A.vy[s] - by;
         fluid_j(Cut{}, s == i, dx, dy, dx * dx + dy * dy, A.vu[s], A.vv[s], A.vrho[s], A.p[s],
-                (int)d.f_phase[s] | ((int)d.f_cls[s] << 8), true);
+                (int)d.f_phase[s] | ((int)d.f_cls[s] << 4), true);
       });
     };
     const float Rp = prune_radius(cp, staged, staged_bound<BLOCK>(s_dmax));

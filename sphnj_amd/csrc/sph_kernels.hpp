@@ -789,27 +789,18 @@ __device__ __forceinline__ LanePos lane_pos_view(float xi, float yi, int cx, int
   return {fminf(fmaxf(tx, -8.f), 9.f), fminf(fmaxf(ty, -8.f), 9.f)};
 }
 
-// A lane in the right half of its sub-column has its longer windows on the right.  The pair
-// kernel evaluates such a lane's windows right to left and the others left to right, which
-// puts windows of about the same in-range count side by side in a wave (whose lanes
-// alternate between the halves on a lattice), so each window's loop runs about as long for
-// all lanes.
-__device__ __forceinline__ bool windows_reversed(LanePos lp) { return lp.fx > 0.5f; }
-
 // Visits the candidate row windows of a lane: f(sc, ra, rb) for each sub-column sc that meets
 // the lane's disc of radius R, with the sub-rows [ra, rb] that do, clipped to the 3x3 cells
-// of the lane's cell (the tile's cell column, row cy) and to the tile's candidate range; left
-// to right, or right to left if rev.
+// of the lane's cell (the tile's cell column, row cy) and to the tile's candidate range.
 template <class CP, class F>
 __device__ __forceinline__ void for_windows(const TilePlan &P, const CP &cp, int sy, int cy, LanePos lp, float R,
-                                            bool rev, F &&f) {
+                                            F &&f) {
   const int sx = P.sx;
   const float R2 = R * R;
   const int a_lo = max((int)floorf(lp.fx - R * cp.inv_ssx), P.sc0 - sx);
   const int a_hi = min((int)floorf(lp.fx + R * cp.inv_ssx), P.sc0 + P.nsc - 1 - sx);
   const int rmin = max(2 * cy - 2, P.r0), rmax = min(2 * cy + 3, P.r0 + P.nr - 1);
-  for (int k = 0; k <= a_hi - a_lo; ++k) {
-    const int a = rev ? a_hi - k : a_lo + k;
+  for (int a = a_lo; a <= a_hi; ++a) {
     const float g = fmaxf(fmaxf((float)a - lp.fx, lp.fx - (float)(a + 1)), 0.f) * cp.ssx;
     if (g >= R) continue;
     const float hr = fminf(__builtin_amdgcn_sqrtf(R2 - g * g) * cp.inv_ssy, 8.f);
@@ -1119,14 +1110,16 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
     };
     if (staged) {
       // staged windows (one run each): first mark the candidates inside the cut-off (32-bit
-      // mask per window), then evaluate the Riemann terms for those only, window by window in
-      // for_windows' order and in slot order within a window -- the same non-zero terms in
-      // the same order as the global path's pass over every candidate
+      // mask per window), then evaluate the Riemann terms for those only, in slot order --
+      // the same non-zero terms in the same order as a pass over every candidate
       unsigned mk[kTileCols];
-      int w0[kTileCols], w1[kTileCols];
-      // in-range mask of the run [s0, s1) (0 for a run longer than 32, evaluated whole by eval)
+      int w0[kTileCols];
+      // in-range mask of the run [s0, s1) (a run longer than 32 is evaluated right away)
       auto mark = [&](int s0, int s1) -> unsigned {
-        if (s1 - s0 > 32) return 0u;
+        if (s1 - s0 > 32) {
+          for_range_masked<2>(s0, s1, [&](int l, bool in) { staged_j(Cut{}, l, in); });
+          return 0u;
+        }
         unsigned m = 0u;
         for_range_masked<4>(s0, s1, [&](int l, bool in) {
           const R2 pj = s_pos[l];
@@ -1136,11 +1129,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
         });
         return m;
       };
-      auto eval = [&](unsigned m, int s0, int s1) {
-        if (s1 - s0 > 32) {  // a long run: every candidate, w = 0 outside the cut-off
-          for_range_masked<2>(s0, s1, [&](int l, bool in) { staged_j(Cut{}, l, in); });
-          return;
-        }
+      auto eval = [&](unsigned m, int s0) {
         while (m) {
           const int b = __builtin_ctz(m);
           m &= m - 1u;
@@ -1150,42 +1139,25 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
 #pragma unroll
       for (int sa = 0; sa < kTileCols; ++sa) {
         mk[sa] = 0u;
-        w0[sa] = w1[sa] = 0;
+        w0[sa] = 0;
         int ra, rb;
         if (!span_window(P, cp, sa, sy, cy, lp, Rp, ra, rb)) continue;
         const int s0 = s_lb[sa][ra - P.r0], s1 = s_lb[sa][rb + 1 - P.r0];
         w0[sa] = s0;
-        w1[sa] = s1;
         mk[sa] = mark(s0, s1);
       }
-      // the windows in for_windows' order: sub-column offsets -3 .. 3 from the lane's own,
-      // reversed for a lane in the right half (the masks are indexed by sub-column)
-      const int own_sa = P.sx - P.sc0;
-      const bool rev = windows_reversed(lp);
 #pragma unroll
-      for (int k = 0; k < 7; ++k) {
-        const int sa = own_sa + (rev ? 3 - k : k - 3);
-        unsigned m = 0u;
-        int b = 0, e = 0;
-#pragma unroll
-        for (int q = 0; q < kTileCols; ++q)
-          if (sa == q) {
-            m = mk[q];
-            b = w0[q];
-            e = w1[q];
-          }
-        eval(m, b, e);
-      }
+      for (int sa = 0; sa < kTileCols; ++sa) eval(mk[sa], w0[sa]);
       if (wstaged) {  // walls of the 3x3 cells, cell column by cell column
         const int lr0 = max(cy - 1, 0) - W.wcr0, lr1 = min(cy + 1, cp.ny - 1) - W.wcr0;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          const int s0 = s_wlb[a][lr0], s1 = s_wlb[a][lr1 + 1];
-          eval(mark(s0, s1), s0, s1);
+          const int s0 = s_wlb[a][lr0];
+          eval(mark(s0, s_wlb[a][lr1 + 1]), s0);
         }
       }
     }
-    for_windows(P, cp, sy, cy, lp, Rp, windows_reversed(lp), [&](int sc, int ra, int rb) {
+    for_windows(P, cp, sy, cy, lp, Rp, [&](int sc, int ra, int rb) {
       if (staged) return;
       const R bx = xi - cell_shift_x<Pol>((sc >> 1) - cx, cp);
       if constexpr (Pol::MIXED) {
@@ -1449,7 +1421,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
       gy += dy * w;
     }
   };
-  for_windows(P, cp, sy, cy, lp, Rp, false, [&](int sc, int ra, int rb) {
+  for_windows(P, cp, sy, cy, lp, Rp, [&](int sc, int ra, int rb) {
     if (staged) {
       const int sa = sc - P.sc0;
       for_range_masked<4>(s_lb[sa][ra - P.r0], s_lb[sa][rb + 1 - P.r0], staged_add);

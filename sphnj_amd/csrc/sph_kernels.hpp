@@ -162,8 +162,8 @@ __global__ void __launch_bounds__(256) k_hash_wall(Dev<Pol> d, const double *x, 
 }
 
 // Gather the fluid state into sorted order (the zip permute of Fluid_List::sort,
-// src/sort.cu:10-20 / include/particle.cuh:93-96), rebuild the views, reset the per-case
-// maximum speed slots and the path lengths of this step.
+// src/sort.cu:10-20 / include/particle.cuh:93-96), rebuild the views, write the sub-cell
+// ranges, reset the per-case maximum speed slots and the path lengths of this step.
 template <class Pol>
 __global__ void __launch_bounds__(256) k_permute_fluid(Dev<Pol> d, const int *sorted_vals) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -186,6 +186,10 @@ __global__ void __launch_bounds__(256) k_permute_fluid(Dev<Pol> d, const int *so
   d.f_phase2[t] = d.f_phase[s];
   d.f_cls2[t] = d.f_cls[s];
   d.f_orig[t] = o;
+  // [start, end) of the slot's sub-cell (k_cell_bounds' rule; the ranges were cleared)
+  const int k = d.f_key[t];
+  if (t == 0 || d.f_key[t - 1] != k) d.f_srange[k].x = t;
+  if (t == d.nf - 1 || d.f_key[t + 1] != k) d.f_srange[k].y = t + 1;
   const int c = case_of(t, d.f_case_off, d.ncase);
   const auto &cp = d.cases[c];
   unsigned fl = 0;
@@ -438,6 +442,29 @@ struct TilePlan {
   int gsz, gmul;         // staging: threads per span B / nsc; ceil(2^20 / gsz) (t / gsz = t gmul >> 20)
 };
 
+// The tiles of one block size: counts per sub-column, their exclusive scan (G + 1 entries, the
+// total last) and the plans.  The planning kernels take two jobs (the count in k_subcol_rows,
+// the others by blockIdx.y), so the tiles of the pair and the shift block sizes are planned
+// by the same launches.
+struct PlanJob {
+  int B;
+  int *cnt, *off;
+  TilePlan *plans;
+};
+struct PlanJobs {
+  PlanJob j[2];
+};
+
+// Tiles of block size B in the sub-column whose slots are [s0, s0 + n) (all lanes of a wave
+// call it; lane 0 gets the count).
+template <class Pol>
+__device__ __forceinline__ int count_tiles(const Dev<Pol> &d, int s0, int n, int B, int lane) {
+  const int nch = (n + B - 1) / B;
+  int t = 0;
+  for (int k = lane; k < nch; k += 64) t += chunk_pieces(d.f_cxy, s0 + k * B, min(B, n - k * B), B);
+  return wave_sum(t);
+}
+
 // Wall row table w_lb of every cell column (once, after the walls are sorted; a thread per
 // even global sub-column): the suffix minimum of the non-empty cells' starts, with the
 // column's end past its last non-empty cell.
@@ -466,10 +493,12 @@ __global__ void __launch_bounds__(256) k_wall_rows(Dev<Pol> d, int G, const int 
 
 // Row table of every sub-column (one wave per global sub-column): f_lb[k] for row k = the
 // suffix minimum of the non-empty rows' starts, with the sub-column's end past the last
-// non-empty row.  The waves of even sub-columns also build their cell column's f_swp (when
-// the batch has solids).
+// non-empty row; and the sub-column's tile counts for the njobs block sizes.  The waves of
+// even sub-columns also audit their cell column's cells against the reference's 64-entry cell
+// capacity (include/global.h:9) and build its f_swp (when the batch has solids).
 template <class Pol>
-__global__ void __launch_bounds__(256) k_subcol_rows(Dev<Pol> d, int G, const int *col_case) {
+__global__ void __launch_bounds__(256) k_subcol_rows(Dev<Pol> d, int G, const int *col_case, PlanJobs jobs,
+                                                     int njobs) {
   const int g = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (g >= G) return;
   const auto &cp = d.cases[col_case[g]];
@@ -510,6 +539,25 @@ __global__ void __launch_bounds__(256) k_subcol_rows(Dev<Pol> d, int G, const in
       }
   }
   if (lane == 0) lb[ny2] = end;
+  // the sub-column's tile counts (lane 0 ends with run = lb[0])
+  const int s0 = __shfl(run, 0, 64);
+  for (int j = 0; j < njobs; ++j) {
+    const int nt = count_tiles(d, s0, end - s0, jobs.j[j].B, lane);
+    if (lane == 0) jobs.j[j].cnt[g] = nt;
+  }
+  if ((sx & 1) == 0) {  // the wave of an even sub-column audits its cell column
+    const int qc = cp.cell_off + (sx >> 1) * cp.ny, kc = 4 * cp.cell_off + sx * ny2;
+    bool over = false;
+    for (int k = lane; k < cp.ny; k += 64) {
+      const int2 a = d.f_srange[kc + 2 * k], b = d.f_srange[kc + 2 * k + 1];
+      const int2 e = d.f_srange[kc + ny2 + 2 * k], f = d.f_srange[kc + ny2 + 2 * k + 1];
+      const int2 so = d.s_range[qc + k], w = d.w_range[qc + k];
+      const int n = max(a.y - a.x, 0) + max(b.y - b.x, 0) + max(e.y - e.x, 0) + max(f.y - f.x, 0) +
+                    (so.y - so.x) + (w.y - w.x);
+      over = over || n > 64;
+    }
+    if (__ballot(over ? 1 : 0) != 0ull && lane == 0) atomicOr(d.status, kStCellOver64);
+  }
   if ((sx & 1) == 0 && d.ns > 0) {  // the wave of an even sub-column also builds its column's f_swp
     const int qc = cp.cell_off + (sx >> 1) * cp.ny;
     int *swp = d.f_swp + cp.swp_off + (sx >> 1) * (cp.ny + 1);
@@ -537,18 +585,6 @@ __global__ void __launch_bounds__(256) k_subcol_rows(Dev<Pol> d, int G, const in
     if (lane == 0) swp[0] = 0;
   }
 }
-
-// The tiles of one block size: counts per sub-column, their exclusive scan (G + 1 entries, the
-// total last) and the plans.  The planning kernels take two jobs and pick one by blockIdx.y,
-// so the tiles of the pair and the shift block sizes are planned by the same launches.
-struct PlanJob {
-  int B;
-  int *cnt, *off;
-  TilePlan *plans;
-};
-struct PlanJobs {
-  PlanJob j[2];
-};
 
 // Tiles per sub-column (one wave per global sub-column, a lane per chunk).
 template <class Pol>

@@ -565,6 +565,10 @@ struct Engine : EngineBase {
 
   int stage_cells() {
     const int nf = d.nf, ns = d.ns, nc = d.total_cells;
+    // every species' cell ranges start empty (the permute writes the fluid's, k_cell_bounds the
+    // solids')
+    k_clear_ranges<<<nblk(4 * nc, 256), 256, 0, stream>>>(d.f_srange, 4 * nc, d.s_range, nc);
+    LAUNCH_CHECK();
     if (nf > 0) {
       k_hash_fluid<Pol><<<nblk(nf, blk_stream), blk_stream, 0, stream>>>(d);
       LAUNCH_CHECK();
@@ -582,28 +586,27 @@ struct Engine : EngineBase {
       LAUNCH_CHECK();
       HIPC(radix_sort_pairs(ssort, d.s_key, &s_sorted_keys, &d.s_sorted, stream));
     }
-    k_clear_ranges<<<nblk(4 * nc, 256), 256, 0, stream>>>(d.f_srange, 4 * nc, d.s_range, nc);
-    LAUNCH_CHECK();
-    if (nf > 0) k_cell_bounds<<<nblk(nf, 256), 256, 0, stream>>>(f_sorted_keys, nf, d.f_srange);
     if (ns > 0) k_cell_bounds<<<nblk(ns, 256), 256, 0, stream>>>(s_sorted_keys, ns, d.s_range);
-    k_cell_audit<Pol><<<nblk(nc, 256), 256, 0, stream>>>(d, d.status);
-    LAUNCH_CHECK();
     move_epoch = 0;
     plan_ok[0] = plan_ok[1] = plan_ok[2] = false;
-    if (nf > 0) {
-      k_subcol_rows<Pol><<<nblk(G, 4), 256, 0, stream>>>(d, G, col_case);
+    if (nf > 0) {  // row tables, the cell audit and the tile counts of both block sizes
+      PlanJobs jobs{};
+      const int nj = plan_jobs(blk_pair, blk_shift, jobs);
+      k_subcol_rows<Pol><<<nblk(G, 4), 256, 0, stream>>>(d, G, col_case, jobs, nj);
       LAUNCH_CHECK();
       int rc;
-      if ((rc = ensure_plans(blk_pair, blk_shift))) return rc;
+      if ((rc = ensure_plans(blk_pair, blk_shift, true))) return rc;
+    } else {
+      k_cell_audit<Pol><<<nblk(nc, 256), 256, 0, stream>>>(d, d.status);
+      LAUNCH_CHECK();
     }
     return SPHNJ_OK;
   }
 
   // Tile plans of size B for the current hash (the row table and tile counts are built by
   // the CELLS stage).
-  int ensure_plans(int B1, int B2 = 0) {
-    if (d.nf == 0) return SPHNJ_OK;
-    PlanJobs jobs{};
+  // The planning jobs of block sizes B1 and B2 (0 = none) whose plans are not current.
+  int plan_jobs(int B1, int B2, PlanJobs &jobs) const {
     int nj = 0;
     for (int B : {B1, B2}) {
       if (B == 0) continue;
@@ -611,8 +614,15 @@ struct Engine : EngineBase {
       if (plan_ok[b] || (nj == 1 && jobs.j[0].B == B)) continue;
       jobs.j[nj++] = PlanJob{B, tile_cnt[b], tile_off[b], plans[b]};
     }
+    return nj;
+  }
+  // counted: k_subcol_rows has just written these jobs' tile counts
+  int ensure_plans(int B1, int B2 = 0, bool counted = false) {
+    if (d.nf == 0) return SPHNJ_OK;
+    PlanJobs jobs{};
+    const int nj = plan_jobs(B1, B2, jobs);
     if (nj == 0) return SPHNJ_OK;
-    k_tile_count<Pol><<<dim3(nblk(G, 4), nj), 256, 0, stream>>>(d, G, col_case, jobs);
+    if (!counted) k_tile_count<Pol><<<dim3(nblk(G, 4), nj), 256, 0, stream>>>(d, G, col_case, jobs);
     k_tile_scan<<<nj, 1024, 0, stream>>>(jobs, G);
     k_tile_plan<Pol><<<dim3(nblk(G, 4), nj), 256, 0, stream>>>(d, G, col_case, jobs);
     LAUNCH_CHECK();

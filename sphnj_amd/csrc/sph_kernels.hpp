@@ -13,6 +13,7 @@
 #pragma once
 
 #include "device.hpp"
+#include "radix_sort.hpp"
 
 namespace sphnj {
 
@@ -102,26 +103,128 @@ __device__ __forceinline__ Mat2<R> mmul(const Mat2<R> &x, const Mat2<R> &y) {
 }
 
 // ======================================================================================
-// CELLS stage: hash, key sort (radix_sort.hpp), permute, cell ranges
+// CELLS stage: hash, key sort (a counting sort for the fluid, radix_sort.hpp for the solids),
+// permute, cell ranges
 // ======================================================================================
 
-// findPositionCell + fillFluidCellID (src/pair.cu:8-27) on the half-cell grid: sub-cell key
-// of every fluid slot, (sy + sx * 2 ny) after the case's 4 ncell sub-cells offset, scattered
-// to original-index order so that a stable sort over original order yields the
-// (sub-column, sub-row, original index) slot order.
+// findPositionCell + fillFluidCellID (src/pair.cu:8-27) on the half-cell grid, and the count
+// of a counting sort (replacing thrust::sort_by_key, src/sort.cu:10-20): every fluid slot's
+// sub-cell key, (sy + sx * 2 ny) after the case's 4 ncell sub-cells offset, and its rank among
+// its key's slots.  Slots sorted last step hold equal keys in runs, so each run of equal keys
+// among a wave's consecutive lanes takes one atomic; k_bucket_order later fixes the order
+// inside each key, which the atomics leave open.  Every lane of the block must reach the
+// ballots.
 template <class Pol>
 __global__ void __launch_bounds__(256) k_hash_fluid(Dev<Pol> d) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63;
+  const bool valid = s < d.nf;
+  int key = -1 - lane;  // distinct for the lanes past the end
+  if (valid) {
+    const int c = case_of(s, d.f_case_off, d.ncase);
+    const auto &cp = d.cases[c];
+    unsigned fl = 0;
+    const int sx = axis_sub(d.fa.x[s], cp.cxmin, cp.inv_cdx, cp.nx, &fl);
+    const int sy = axis_sub(d.fa.y[s], cp.cymin, cp.inv_cdy, cp.ny, &fl);
+    key = 4 * cp.cell_off + sy + sx * (2 * cp.ny);
+    if (fl) atomicOr(d.status, kStOutOfGrid);
+  }
+  // runs of equal keys: a run starts where the key differs from the previous lane's
+  const int prev = __shfl_up(key, 1, 64);
+  const unsigned long long heads = __ballot((lane == 0 || prev != key) ? 1 : 0);
+  const unsigned long long above = lane == 63 ? 0ull : heads >> (lane + 1);
+  const int head = 63 - __builtin_clzll(heads & ((2ull << lane) - 1ull));  // this lane's run start
+  int base = 0;
+  if (valid && head == lane) base = atomicAdd(&d.f_cnt[key], above ? __builtin_ctzll(above) + 1 : 64 - lane);
+  base = __shfl(base, head, 64);
+  if (valid) {
+    d.f_hkey[s] = key;
+    d.f_hrank[s] = base + lane - head;
+  }
+}
+
+// Counting sort, step 2: the sum of each tile of kScanTile key counts.
+constexpr int kScanItems = 8, kScanTile = kSortThreads * kScanItems;
+__global__ void __launch_bounds__(kSortThreads) k_count_tiles(const int *cnt, int nk, int *tsum) {
+  __shared__ int tmp[4];
+  const int k0 = blockIdx.x * kScanTile + (int)threadIdx.x * kScanItems;
+  int v = 0;
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q)
+    if (k0 + q < nk) v += cnt[k0 + q];
+  int tot;
+  block_exscan256(v, tmp, &tot);
+  if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
+}
+
+// Exclusive scan of n values in one block of 1024 threads; the total goes to out[n].
+__global__ void __launch_bounds__(1024) k_exscan_small(const int *in, int n, int *out) {
+  __shared__ int part[1024];
+  const int t = threadIdx.x, per = (n + 1023) / 1024, a = min(t * per, n), b = min(a + per, n);
+  int s = 0;
+  for (int k = a; k < b; ++k) s += in[k];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < 1024; o <<= 1) {  // Hillis-Steele inclusive scan
+    const int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  int run = part[t] - s;
+  for (int k = a; k < b; ++k) {
+    out[k] = run;
+    run += in[k];
+  }
+  if (t == 1023) out[n] = part[1023];
+}
+
+// Counting sort, step 3: each key's first slot and [start, end) range (empty: [0, 0)); the
+// counts are cleared for the next step.
+__global__ void __launch_bounds__(kSortThreads) k_count_apply(int *cnt, int nk, const int *toff, int *start,
+                                                              int2 *range) {
+  __shared__ int tmp[4];
+  const int k0 = blockIdx.x * kScanTile + (int)threadIdx.x * kScanItems;
+  int c[kScanItems], v = 0;
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q) {
+    c[q] = k0 + q < nk ? cnt[k0 + q] : 0;
+    v += c[q];
+  }
+  int tot;
+  int st = toff[blockIdx.x] + block_exscan256(v, tmp, &tot);
+#pragma unroll
+  for (int q = 0; q < kScanItems; ++q)
+    if (k0 + q < nk) {
+      start[k0 + q] = st;
+      range[k0 + q] = c[q] > 0 ? make_int2(st, st + c[q]) : make_int2(0, 0);
+      cnt[k0 + q] = 0;
+      st += c[q];
+    }
+}
+
+// Counting sort, step 4: every slot to its place in atomic order.
+template <class Pol>
+__global__ void __launch_bounds__(256) k_count_scatter(Dev<Pol> d) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= d.nf) return;
-  const int c = case_of(s, d.f_case_off, d.ncase);
-  const auto &cp = d.cases[c];
-  unsigned fl = 0;
-  const int sx = axis_sub(d.fa.x[s], cp.cxmin, cp.inv_cdx, cp.nx, &fl);
-  const int sy = axis_sub(d.fa.y[s], cp.cymin, cp.inv_cdy, cp.ny, &fl);
-  const int o = d.f_orig[s];
-  d.f_key_by_orig[o] = 4 * cp.cell_off + sy + sx * (2 * cp.ny);
-  d.f_slot_of_orig[o] = s;
-  if (fl) atomicOr(d.status, kStOutOfGrid);
+  d.f_perm[d.f_start[d.f_hkey[s]] + d.f_hrank[s]] = s;
+}
+
+// Counting sort, step 5: inside each key, slots in original-index order -- a slot's place is
+// its key's start plus the number of the key's slots with a smaller original index -- so the
+// order is (sub-column, sub-row, original index) whatever order the atomics took.
+template <class Pol>
+__global__ void __launch_bounds__(256) k_bucket_order(Dev<Pol> d) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= d.nf) return;
+  const int s = d.f_perm[t], k = d.f_hkey[s], o = d.f_orig[s];
+  const int2 r = d.f_srange[k];
+  int rank = 0;
+  for (int u = r.x; u < r.y; ++u) rank += d.f_orig[d.f_perm[u]] < o ? 1 : 0;
+  const int t2 = r.x + rank;
+  d.f_perm2[t2] = s;
+  d.f_orig2[t2] = o;
+  d.f_key[t2] = k;
 }
 
 // fillSolidCellID (src/pair.cu:29-36); solids stay in original order, the key array is the
@@ -162,10 +265,10 @@ __global__ void __launch_bounds__(256) k_hash_wall(Dev<Pol> d, const double *x, 
 }
 
 // Gather the fluid state into sorted order (the zip permute of Fluid_List::sort,
-// src/sort.cu:10-20 / include/particle.cuh:93-96), rebuild the views, write the sub-cell
-// ranges, reset the per-case maximum speed slots and the path lengths of this step.
+// src/sort.cu:10-20 / include/particle.cuh:93-96), rebuild the views, reset the per-case
+// maximum speed slots and the path lengths of this step.
 template <class Pol>
-__global__ void __launch_bounds__(256) k_permute_fluid(Dev<Pol> d, const int *sorted_vals) {
+__global__ void __launch_bounds__(256) k_permute_fluid(Dev<Pol> d) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t < d.ncase * kSpread) d.umax_slot[(size_t)t * kSlotStride] = 0ull;
   if (t < d.ncase) {
@@ -173,8 +276,8 @@ __global__ void __launch_bounds__(256) k_permute_fluid(Dev<Pol> d, const int *so
     d.umax[t] = 0;
   }
   if (t >= d.nf) return;
-  const int o = sorted_vals[t];
-  const int s = d.f_slot_of_orig[o];
+  const int o = d.f_orig2[t];
+  const int s = d.f_perm2[t];
   const double x = d.fa.x[s], y = d.fa.y[s], u = d.fa.u[s], v = d.fa.v[s], rho = d.fa.rho[s];
   d.fb.x[t] = x;
   d.fb.y[t] = y;
@@ -186,10 +289,6 @@ __global__ void __launch_bounds__(256) k_permute_fluid(Dev<Pol> d, const int *so
   d.f_phase2[t] = d.f_phase[s];
   d.f_cls2[t] = d.f_cls[s];
   d.f_orig[t] = o;
-  // [start, end) of the slot's sub-cell (k_cell_bounds' rule; the ranges were cleared)
-  const int k = d.f_key[t];
-  if (t == 0 || d.f_key[t - 1] != k) d.f_srange[k].x = t;
-  if (t == d.nf - 1 || d.f_key[t + 1] != k) d.f_srange[k].y = t + 1;
   const int c = case_of(t, d.f_case_off, d.ncase);
   const auto &cp = d.cases[c];
   unsigned fl = 0;

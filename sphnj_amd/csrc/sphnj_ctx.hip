@@ -126,8 +126,9 @@ struct Engine : EngineBase {
   int max_tiles(int B) const { return kMaxPieces * (nblk(d.nf, B) + G); }
   int grid_tiles(int B) const { return (max_tiles(B) + kXcds - 1) / kXcds * kXcds; }
   bool viscid = false;
-  RadixSorter fsort, ssort;
-  int *f_sorted_keys = nullptr, *s_sorted_keys = nullptr;
+  RadixSorter ssort;
+  int *s_sorted_keys = nullptr;
+  int *cnt_tsum = nullptr, *cnt_toff = nullptr;  // counting sort: tile sums and their scan
   std::vector<void *> allocs;
   struct GraphEntry {
     hipGraphExec_t exec;
@@ -347,7 +348,11 @@ struct Engine : EngineBase {
     }
     if ((rc = dalloc(&d.f_phase, nf)) || (rc = dalloc(&d.f_phase2, nf)) || (rc = dalloc(&d.f_cls, nf)) ||
         (rc = dalloc(&d.f_cls2, nf)) || (rc = dalloc(&d.f_orig, nf)) || (rc = dalloc(&d.f_cxy, nf)) ||
-        (rc = dalloc(&d.f_key_by_orig, nf)) || (rc = dalloc(&d.f_slot_of_orig, nf)) ||
+        (rc = dalloc(&d.f_hkey, nf)) || (rc = dalloc(&d.f_hrank, nf)) || (rc = dalloc(&d.f_perm, nf)) ||
+        (rc = dalloc(&d.f_perm2, nf)) || (rc = dalloc(&d.f_orig2, nf)) || (rc = dalloc(&d.f_key, nf)) ||
+        (rc = dalloc(&d.f_cnt, 4 * (size_t)nc + 1)) || (rc = dalloc(&d.f_start, 4 * (size_t)nc + 1)) ||
+        (rc = dalloc(&cnt_tsum, (size_t)nblk(4 * nc, kScanTile) + 1)) ||
+        (rc = dalloc(&cnt_toff, (size_t)nblk(4 * nc, kScanTile) + 1)) ||
         (rc = dalloc(&d.f_col, nf)) || (rc = dalloc(&d.f_near, nf)) || (rc = dalloc(&d.f_L, 4 * (size_t)nf)) ||
         (rc = dalloc(&d.f_sacc, 2 * (size_t)nf)) ||
         (rc = dalloc(&d.f_srange, 4 * (size_t)nc)) || (rc = dalloc(&d.umax2, ncase)) || (rc = dalloc(&d.umax, ncase)) ||
@@ -389,7 +394,8 @@ struct Engine : EngineBase {
         return rc;
       HIPC(hipStreamSynchronize(stream));
     }
-    if ((rc = sorter_alloc(fsort, nf, std::max(4 * nc - 1, 0)))) return rc;
+    HIPC(hipMemsetAsync(d.f_cnt, 0, (4 * (size_t)nc + 1) * sizeof(int), stream));  // zero between steps
+    HIPC(hipMemsetAsync(d.f_srange, 0, 4 * (size_t)nc * sizeof(int2), stream));
 
     // ---- solid
     if ((rc = dalloc(&d.sx, ns)) || (rc = dalloc(&d.sy, ns)) || (rc = dalloc(&d.su, ns)) || (rc = dalloc(&d.sv, ns)) ||
@@ -567,15 +573,21 @@ struct Engine : EngineBase {
     const int nf = d.nf, ns = d.ns, nc = d.total_cells;
     // every species' cell ranges start empty (the permute writes the fluid's, k_cell_bounds the
     // solids')
-    k_clear_ranges<<<nblk(4 * nc, 256), 256, 0, stream>>>(d.f_srange, 4 * nc, d.s_range, nc);
+    // The solids' cell ranges start empty (k_cell_bounds writes the non-empty ones); so do the
+    // fluid's when there is no fluid (otherwise the counting sort writes every one of them).
+    k_clear_ranges<<<nblk(nf > 0 ? nc : 4 * nc, 256), 256, 0, stream>>>(d.s_range, nc, nf > 0 ? nullptr : d.f_srange,
+                                                                       nf > 0 ? 0 : 4 * nc);
     LAUNCH_CHECK();
-    if (nf > 0) {
-      k_hash_fluid<Pol><<<nblk(nf, blk_stream), blk_stream, 0, stream>>>(d);
+    if (nf > 0) {  // counting sort on the sub-cell keys, then the permute
+      const int nk = 4 * nc, nkb = nblk(nk, kScanTile);
+      k_hash_fluid<Pol><<<nblk(nf, 256), 256, 0, stream>>>(d);
+      k_count_tiles<<<nkb, kSortThreads, 0, stream>>>(d.f_cnt, nk, cnt_tsum);
+      k_exscan_small<<<1, 1024, 0, stream>>>(cnt_tsum, nkb, cnt_toff);
+      k_count_apply<<<nkb, kSortThreads, 0, stream>>>(d.f_cnt, nk, cnt_toff, d.f_start, d.f_srange);
+      k_count_scatter<Pol><<<nblk(nf, 256), 256, 0, stream>>>(d);
+      k_bucket_order<Pol><<<nblk(nf, 256), 256, 0, stream>>>(d);
       LAUNCH_CHECK();
-      int *sv;
-      HIPC(radix_sort_pairs(fsort, d.f_key_by_orig, &f_sorted_keys, &sv, stream));
-      d.f_key = f_sorted_keys;
-      k_permute_fluid<Pol><<<nblk(std::max(nf, ncase * kSpread), blk_stream), blk_stream, 0, stream>>>(d, sv);
+      k_permute_fluid<Pol><<<nblk(std::max(nf, ncase * kSpread), blk_stream), blk_stream, 0, stream>>>(d);
       LAUNCH_CHECK();
       swap_fluid_all();
       std::swap(d.f_phase, d.f_phase2);
@@ -1032,7 +1044,7 @@ struct Engine : EngineBase {
       n = f_off[c + 1] - lo;
       k.resize(n);
       o.resize(n);
-      if (n && f_sorted_keys && (rc = d2h(k.data(), f_sorted_keys + lo, n))) return rc;
+      if (n && (rc = d2h(k.data(), d.f_key + lo, n))) return rc;
       if ((rc = d2h(o.data(), d.f_orig + lo, n)) || (rc = d2h(rg.data(), d.f_srange + 4 * (size_t)cp.cell_off, nb)))
         return rc;
     } else if (species == SPHNJ_SOLID) {

@@ -91,11 +91,13 @@ struct Dev {
   int *f_key;            // slot -> global sub-cell key (sorted keys)
   int *f_cxy;            // slot -> (sx << 16) | sy, sub-cell coordinates (cx = sx >> 1)
   // counting sort on the sub-cell keys: per key (4 total_cells + 1) the slots hashed there this
-  // step (zero between steps) and the first slot; per slot of the order being replaced its
-  // key and its rank among its key's slots in atomic order; the slot maps of the sort
-  int *f_cnt, *f_start;
+  // step (zero between steps); per slot of the order being replaced its key and its rank among
+  // its key's slots in atomic order; per slot in atomic order the old slot, original index and
+  // key; per sorted slot the old slot and original index
+  int *f_cnt;
   int *f_hkey, *f_hrank;
-  int *f_perm, *f_perm2, *f_orig2;
+  int *f_perm, *f_porig, *f_pkey;
+  int *f_perm2, *f_orig2;
   R *f_col;
   int8_t *f_near;
   R *f_L;  // 4 per slot

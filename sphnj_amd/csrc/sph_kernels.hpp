@@ -142,15 +142,16 @@ __global__ void __launch_bounds__(256) k_hash_fluid(Dev<Pol> d) {
   }
 }
 
-// Counting sort, step 2: the sum of each tile of kScanTile key counts.
+// Counting sort, step 2: the sum of each tile of kScanTile key counts (read in rounds of
+// kSortThreads consecutive keys).
 constexpr int kScanItems = 8, kScanTile = kSortThreads * kScanItems;
 __global__ void __launch_bounds__(kSortThreads) k_count_tiles(const int *cnt, int nk, int *tsum) {
   __shared__ int tmp[4];
-  const int k0 = blockIdx.x * kScanTile + (int)threadIdx.x * kScanItems;
+  const int k0 = blockIdx.x * kScanTile + (int)threadIdx.x;
   int v = 0;
 #pragma unroll
   for (int q = 0; q < kScanItems; ++q)
-    if (k0 + q < nk) v += cnt[k0 + q];
+    if (k0 + q * kSortThreads < nk) v += cnt[k0 + q * kSortThreads];
   int tot;
   block_exscan256(v, tmp, &tot);
   if (threadIdx.x == 0) tsum[blockIdx.x] = tot;
@@ -178,36 +179,34 @@ __global__ void __launch_bounds__(1024) k_exscan_small(const int *in, int n, int
   if (t == 1023) out[n] = part[1023];
 }
 
-// Counting sort, step 3: each key's first slot and [start, end) range (empty: [0, 0)); the
-// counts are cleared for the next step.
-__global__ void __launch_bounds__(kSortThreads) k_count_apply(int *cnt, int nk, const int *toff, int *start,
-                                                              int2 *range) {
+// Counting sort, step 3: each key's [start, end) slot range (empty: [0, 0)), in rounds of
+// kSortThreads consecutive keys; the counts are cleared for the next step.
+__global__ void __launch_bounds__(kSortThreads) k_count_apply(int *cnt, int nk, const int *toff, int2 *range) {
   __shared__ int tmp[4];
-  const int k0 = blockIdx.x * kScanTile + (int)threadIdx.x * kScanItems;
-  int c[kScanItems], v = 0;
-#pragma unroll
+  int run = toff[blockIdx.x];
   for (int q = 0; q < kScanItems; ++q) {
-    c[q] = k0 + q < nk ? cnt[k0 + q] : 0;
-    v += c[q];
-  }
-  int tot;
-  int st = toff[blockIdx.x] + block_exscan256(v, tmp, &tot);
-#pragma unroll
-  for (int q = 0; q < kScanItems; ++q)
-    if (k0 + q < nk) {
-      start[k0 + q] = st;
-      range[k0 + q] = c[q] > 0 ? make_int2(st, st + c[q]) : make_int2(0, 0);
-      cnt[k0 + q] = 0;
-      st += c[q];
+    const int k = blockIdx.x * kScanTile + q * kSortThreads + (int)threadIdx.x;
+    const int c = k < nk ? cnt[k] : 0;
+    int tot;
+    const int st = run + block_exscan256(c, tmp, &tot);
+    if (k < nk) {
+      range[k] = c > 0 ? make_int2(st, st + c) : make_int2(0, 0);
+      cnt[k] = 0;
     }
+    run += tot;
+  }
 }
 
-// Counting sort, step 4: every slot to its place in atomic order.
+// Counting sort, step 4: every slot to its place in atomic order, with its original index
+// and key.
 template <class Pol>
 __global__ void __launch_bounds__(256) k_count_scatter(Dev<Pol> d) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= d.nf) return;
-  d.f_perm[d.f_start[d.f_hkey[s]] + d.f_hrank[s]] = s;
+  const int k = d.f_hkey[s], p = d.f_srange[k].x + d.f_hrank[s];
+  d.f_perm[p] = s;
+  d.f_porig[p] = d.f_orig[s];
+  d.f_pkey[p] = k;
 }
 
 // Counting sort, step 5: inside each key, slots in original-index order -- a slot's place is
@@ -217,12 +216,12 @@ template <class Pol>
 __global__ void __launch_bounds__(256) k_bucket_order(Dev<Pol> d) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= d.nf) return;
-  const int s = d.f_perm[t], k = d.f_hkey[s], o = d.f_orig[s];
+  const int k = d.f_pkey[t], o = d.f_porig[t];
   const int2 r = d.f_srange[k];
   int rank = 0;
-  for (int u = r.x; u < r.y; ++u) rank += d.f_orig[d.f_perm[u]] < o ? 1 : 0;
+  for (int u = r.x; u < r.y; ++u) rank += d.f_porig[u] < o ? 1 : 0;
   const int t2 = r.x + rank;
-  d.f_perm2[t2] = s;
+  d.f_perm2[t2] = d.f_perm[t];
   d.f_orig2[t2] = o;
   d.f_key[t2] = k;
 }

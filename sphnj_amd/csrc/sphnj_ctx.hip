@@ -350,7 +350,7 @@ struct Engine : EngineBase {
         (rc = dalloc(&d.f_cls2, nf)) || (rc = dalloc(&d.f_orig, nf)) || (rc = dalloc(&d.f_cxy, nf)) ||
         (rc = dalloc(&d.f_hkey, nf)) || (rc = dalloc(&d.f_hrank, nf)) || (rc = dalloc(&d.f_perm, nf)) ||
         (rc = dalloc(&d.f_perm2, nf)) || (rc = dalloc(&d.f_orig2, nf)) || (rc = dalloc(&d.f_key, nf)) ||
-        (rc = dalloc(&d.f_cnt, 4 * (size_t)nc + 1)) || (rc = dalloc(&d.f_start, 4 * (size_t)nc + 1)) ||
+        (rc = dalloc(&d.f_cnt, 4 * (size_t)nc + 1)) || (rc = dalloc(&d.f_porig, nf)) || (rc = dalloc(&d.f_pkey, nf)) ||
         (rc = dalloc(&cnt_tsum, (size_t)nblk(4 * nc, kScanTile) + 1)) ||
         (rc = dalloc(&cnt_toff, (size_t)nblk(4 * nc, kScanTile) + 1)) ||
         (rc = dalloc(&d.f_col, nf)) || (rc = dalloc(&d.f_near, nf)) || (rc = dalloc(&d.f_L, 4 * (size_t)nf)) ||
@@ -583,7 +583,7 @@ struct Engine : EngineBase {
       k_hash_fluid<Pol><<<nblk(nf, 256), 256, 0, stream>>>(d);
       k_count_tiles<<<nkb, kSortThreads, 0, stream>>>(d.f_cnt, nk, cnt_tsum);
       k_exscan_small<<<1, 1024, 0, stream>>>(cnt_tsum, nkb, cnt_toff);
-      k_count_apply<<<nkb, kSortThreads, 0, stream>>>(d.f_cnt, nk, cnt_toff, d.f_start, d.f_srange);
+      k_count_apply<<<nkb, kSortThreads, 0, stream>>>(d.f_cnt, nk, cnt_toff, d.f_srange);
       k_count_scatter<Pol><<<nblk(nf, 256), 256, 0, stream>>>(d);
       k_bucket_order<Pol><<<nblk(nf, 256), 256, 0, stream>>>(d);
       LAUNCH_CHECK();

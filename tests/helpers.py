@@ -2,6 +2,7 @@
 import numpy as np
 
 from sphnj_amd import Case, default_params
+from sphnj_amd.sim import grid
 
 
 def params(**kw):
@@ -76,3 +77,20 @@ def lattice(nx, ny, dx, x0=0.0, y0=0.0):
 def case(p, f=None, s=None, w=None):
     return Case(p, f if f is not None else empty_fluid(), s if s is not None else empty_solid(),
                 w if w is not None else empty_wall())
+
+
+def half_cell_keys(case, x, y):
+    """Sub-cell keys (sy + sx 2 ny) of positions, by the hash's own arithmetic (axis_sub)."""
+    nx, ny, cdx, cdy = grid(case.params)
+    p = case.params
+
+    def axis(v, v0, d, n):
+        t = (np.asarray(v, np.float64) - v0) * (1.0 / d)
+        c = np.trunc(t)
+        h = np.clip(np.trunc(2.0 * t) - 2.0 * c, 0, 1)
+        out = (2 * c + h).astype(np.int64)
+        out[~(t > -1.0)] = 0
+        out[~(t < n)] = 2 * n - 1
+        return out
+
+    return axis(y, p.cell_ymin, cdy, ny) + axis(x, p.cell_xmin, cdx, nx) * 2 * ny

@@ -8,6 +8,7 @@ import numpy as np
 import pytest
 
 from conftest import rel_l2
+from helpers import half_cell_keys
 from sphnj_amd import Case, Simulation
 
 pytestmark = pytest.mark.gpu
@@ -55,6 +56,29 @@ def test_cells_bit_exact(gpu, oracle_mod, precision):
         assert np.array_equal(os_, oo), f"species {species}: order differs"
         assert np.array_equal(ss, so), f"species {species}: cell starts differ"
         assert np.all(np.diff(ks) >= 0)
+
+
+@pytest.mark.parametrize("precision", ["p64", "m32"])
+def test_sort_after_motion(gpu, precision):
+    """After particles have crossed sub-cells, the CELLS stage's order (a counting sort that
+    starts from last step's order) is still the (sub-column, sub-row, original index) order of
+    the current positions, with consistent starts."""
+    case = small_dam()
+    sim = Simulation(case, precision)
+    k0 = half_cell_keys(case, case.fluid["x"], case.fluid["y"])
+    sim.step(300)
+    sim.particle_cell_update()
+    ks, os_, ss = sim.sorted(0, 0)
+    st = sim.download()["fluid"]
+    key = half_cell_keys(case, st["x"], st["y"])
+    assert np.count_nonzero(key != k0) > 20, "the case must move particles across sub-cells"
+    assert np.array_equal(np.sort(os_), np.arange(len(os_)))
+    assert np.array_equal(ks, key[os_])
+    assert np.all(np.diff(ks) >= 0)
+    same = np.diff(ks) == 0
+    assert np.all(np.diff(os_)[same] > 0), "original-index order inside a sub-cell"
+    slots = np.arange(len(ks))
+    assert np.all(ss[ks] <= slots) and np.all(slots < ss[ks + 1])
 
 
 @pytest.mark.parametrize("precision", ["p64", "m32"])

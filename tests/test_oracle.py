@@ -13,7 +13,7 @@ import pytest
 from scipy import integrate
 
 from conftest import rel_l2
-from helpers import case, fluid, lattice, params, solid, wall
+from helpers import case, fluid, half_cell_keys, lattice, params, solid, wall
 from sphnj_amd import _abi as A
 
 
@@ -454,3 +454,27 @@ def test_hydrostatic_tank_stays_near_rest(oracle_mod):
     assert np.hypot(f["u"], f["v"]).max() < 1e-3 * c0
     assert abs(f["y"].max() - c.fluid["y"].max()) < 0.1 * 0.02
     assert o.status() == 0
+
+
+def test_sort_after_motion():
+    """The oracle's fluid order after particles have crossed sub-cells: keys recomputed from
+    the current positions by the hash's arithmetic, non-decreasing, original-index order inside
+    a sub-cell, and starts that bracket every slot."""
+    from oracle.oracle_py import Oracle
+    from sphnj_amd import Case
+
+    c = Case.generate("dam_break", tank_w=1.0, tank_h=0.8, col_w=0.4, col_h=0.6, dx=0.02, steps=100)
+    o = Oracle(c)
+    k0 = half_cell_keys(c, c.fluid["x"], c.fluid["y"])
+    o.step(300)
+    o.particle_cell_update()
+    ks, os_, ss = o.sorted(0, 0)
+    st = o.download()["fluid"]
+    key = half_cell_keys(c, st["x"], st["y"])
+    assert np.count_nonzero(key != k0) > 20
+    assert np.array_equal(ks, key[os_])
+    assert np.all(np.diff(ks) >= 0)
+    same = np.diff(ks) == 0
+    assert np.all(np.diff(os_)[same] > 0)
+    slots = np.arange(len(ks))
+    assert np.all(ss[ks] <= slots) and np.all(slots < ss[ks + 1])

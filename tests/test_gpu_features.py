@@ -149,3 +149,32 @@ def test_two_phase_with_walls_and_gas_matches_oracle(gpu, oracle_mod):
         assert rel_l2(a["fluid"][k], b["fluid"][k]) <= 1e-9, k
     assert np.array_equal(a["wall"]["phase"], b["wall"]["phase"])
     assert np.array_equal(a["fluid"]["boundary_type"], b["fluid"]["boundary_type"])
+
+
+def test_dense_tiles_read_global_candidates(gpu, oracle_mod):
+    """A block packed at a quarter of the particle spacing overflows every tile's LDS budget,
+    so its tiles read their candidates from global memory, in the pair kernel and in every
+    shift pass; beside it a block at the normal spacing is staged.  Both match the oracle, and
+    P64 is bit-identical at every block size (the two paths sum the same terms in the same
+    order)."""
+    p = params(shift_per_step=2)
+    xd, yd = lattice(40, 40, 0.005, 0.3, 0.3)
+    xn, yn = lattice(20, 20, 0.02, 0.6, 0.2)
+    x, y = np.r_[xd, xn], np.r_[yd, yn]
+    c = case(p, f=fluid(x, y, u=0.5 * (y - 0.4), v=0.3 * (x - 0.5), p=100.0 * x, rho=1000.0 + 0.5 * y))
+    o = oracle_mod.Oracle(c)
+    o.step(3)
+    ref = o.download()["fluid"]
+    out = {}
+    for prec, tol in (("p64", 1e-9), ("m32", 1e-4)):
+        for blk in (64, 128, 256):
+            s = Simulation(c, prec)
+            s.set_block(0, blk)
+            s.set_block(1, blk)
+            s.step(3)
+            out[prec, blk] = s.download()["fluid"]
+            for k in ("x", "y", "u", "v", "rho", "p"):
+                assert rel_l2(out[prec, blk][k], ref[k]) <= tol, (prec, blk, k)
+    for blk in (128, 256):
+        for k in ("x", "y", "u", "v", "rho", "p", "boundary_type"):
+            assert np.array_equal(out["p64", 64][k], out["p64", blk][k]), (blk, k)

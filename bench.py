@@ -73,6 +73,27 @@ def measured_traffic(config, stage):
     return None
 
 
+def case_probes(config, case, a, b):
+    """The configurations' own error figures, M32 (a) against P64 (b) at the compared step: the
+    elastic gate's tip displacement (mean horizontal displacement of the free bottom row of the
+    plate, which is clamped at the top), the wedge's largest surface pressure, and its mean solid
+    pressure (the figure src/output.cu:546 logs)."""
+    def rel(m, r):
+        return {"m32": float(f"{m:.6e}"), "p64": float(f"{r:.6e}"),
+                "rel_err": float(f"{abs(m - r) / abs(r):.3e}") if r else None}
+    if config == "gate":
+        Y = np.asarray(case.solid["Y"], np.float64)
+        row = np.isclose(Y, Y.min())
+        X = np.asarray(case.solid["X"], np.float64)[row]
+        return {"gate_tip_displacement_m": rel(float(np.mean(a["solid"]["x"][row] - X)),
+                                               float(np.mean(b["solid"]["x"][row] - X)))}
+    if config == "wedge":
+        pa, pb = a["solid"]["pressure"], b["solid"]["pressure"]
+        return {"wedge_max_surface_pressure_pa": rel(float(np.max(pa)), float(np.max(pb))),
+                "wedge_mean_solid_pressure_pa": rel(float(np.mean(pa)), float(np.mean(pb)))}
+    return {}
+
+
 def rel_l2(a, b):
     a = np.asarray(a, np.float64).ravel()
     b = np.asarray(b, np.float64).ravel()
@@ -217,10 +238,12 @@ def main() -> int:
         errs = {f: rel_l2(a["fluid"][f], b["fluid"][f]) for f in ("x", "y", "u", "v", "rho", "p")}
         if ns:
             errs.update({"solid_" + f: rel_l2(a["solid"][f], b["solid"][f]) for f in ("x", "y", "stress")})
+        probes = case_probes(args.config, case, a, b)
         extra = {"p64_value": round(p64_value, 3), "p64_ms_per_step": round(1e3 * tp / args.steps, 4),
                  "speedup_vs_p64": round(value / p64_value, 3),
                  "rel_l2_vs_p64": {k: float(f"{v:.3e}") for k, v in errs.items()},
                  "steps_compared": done,
+                 **probes,
                  "p64_stage_ms": {k: round(v["ms_per_step"], 4) for k, v in prof64.items()},
                  "p64_blocks": tune64 and {k: v["block"] for k, v in tune64.items()}}
         print(f"# p64: {1e3 * tp / args.steps:.3f} ms/step, {p64_value:.1f} M updates/s; "

@@ -94,3 +94,23 @@ def half_cell_keys(case, x, y):
         return out
 
     return axis(y, p.cell_ymin, cdy, ny) + axis(x, p.cell_xmin, cdx, nx) * 2 * ny
+
+
+GOLDEN = ("dam_liquid", "dam_two_phase", "gate_fsi")
+
+
+def golden(name):
+    """A fixture of tests/golden (written by tests/golden/make_golden.py): the Case rebuilt from
+    its stored parameters and inputs, and the fixture's arrays."""
+    import json
+    from pathlib import Path
+
+    import sphnj_amd._abi as A
+
+    z = dict(np.load(Path(__file__).resolve().parent / "golden" / f"{name}.npz"))
+    p = A.Params()
+    for k, v in json.loads(str(z["params"])).items():
+        setattr(p, k, v)
+    sp = {s: {k.split(".", 2)[2]: a for k, a in z.items() if k.startswith(f"in.{s}.")}
+          for s in ("fluid", "solid", "wall")}
+    return Case(p, sp["fluid"], sp["solid"], sp["wall"]), z

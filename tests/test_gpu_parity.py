@@ -178,3 +178,31 @@ def test_batch_equals_single(gpu):
         for sp in ("fluid", "solid", "wall"):
             for k in a[sp]:
                 assert np.array_equal(a[sp][k], b[sp][k]), (i, sp, k)
+
+
+@pytest.mark.parametrize("precision,tol", [("p64", 1e-9), ("m32", 1e-4)])
+@pytest.mark.parametrize("name", ["dam_liquid", "dam_two_phase", "gate_fsi"])
+def test_matches_golden(gpu, name, precision, tol):
+    """The library against the committed fixtures (tests/golden): the first hash's keys,
+    original-index order and cell starts bit for bit, then the state after the fixture's steps
+    to the precision's bar (M32: the solid stress is left to the P64 check)."""
+    from helpers import golden
+
+    c, z = golden(name)
+    sim = Simulation(c, precision)
+    sim.particle_cell_update()
+    for s in (0, 1, 2):
+        for part, a in zip(("keys", "order", "starts"), sim.sorted(0, s)):
+            assert np.array_equal(a, z[f"sorted.{s}.{part}"]), (s, part)
+    sim.close()
+    sim = Simulation(c, precision)
+    sim.step(int(z["steps"]))
+    assert sim.status() == 0
+    st = sim.download()
+    errs = {}
+    for k, ref in z.items():
+        if k.startswith("out.") and not (precision == "m32" and k == "out.solid.stress"):
+            _, sp, f = k.split(".")
+            errs[k] = rel_l2(st[sp][f], ref)
+    bad = {k: v for k, v in errs.items() if not v <= tol}
+    assert not bad, f"fields over tolerance {tol}: {bad}"

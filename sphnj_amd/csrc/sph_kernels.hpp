@@ -923,10 +923,16 @@ __device__ __forceinline__ LanePos lane_pos_view(float xi, float yi, int cx, int
   return {fminf(fmaxf(tx, -8.f), 9.f), fminf(fmaxf(ty, -8.f), 9.f)};
 }
 
+// Sub-column offset of the q-th window a lane visits: 0, -1, +1, -2, +2, -3, +3 (the 3x3
+// cells of a sub-column span offsets -3 .. +3).
+constexpr int kWindowOrder = 7;
+__device__ __forceinline__ int window_offset(int q) { return (q & 1) ? -((q + 1) >> 1) : (q >> 1); }
+
 // Visits the candidate row windows of a lane: f(sc, ra, rb) for each sub-column sc that meets
 // the lane's disc of radius R, with the sub-rows [ra, rb] that do, clipped to the 3x3 cells
-// of the lane's cell (the tile's cell column, row cy) and to the tile's candidate range.
-template <class CP, class F>
+// of the lane's cell (the tile's cell column, row cy) and to the tile's candidate range;
+// sub-columns in window_offset order (PAIRED) or from left to right.
+template <bool PAIRED, class CP, class F>
 __device__ __forceinline__ void for_windows(const TilePlan &P, const CP &cp, int sy, int cy, LanePos lp, float R,
                                             F &&f) {
   const int sx = P.sx;
@@ -934,7 +940,9 @@ __device__ __forceinline__ void for_windows(const TilePlan &P, const CP &cp, int
   const int a_lo = max((int)floorf(lp.fx - R * cp.inv_ssx), P.sc0 - sx);
   const int a_hi = min((int)floorf(lp.fx + R * cp.inv_ssx), P.sc0 + P.nsc - 1 - sx);
   const int rmin = max(2 * cy - 2, P.r0), rmax = min(2 * cy + 3, P.r0 + P.nr - 1);
-  for (int a = a_lo; a <= a_hi; ++a) {
+  for (int q = PAIRED ? 0 : a_lo; q < (PAIRED ? kWindowOrder : a_hi + 1); ++q) {
+    const int a = PAIRED ? window_offset(q) : q;
+    if (PAIRED && (a < a_lo || a > a_hi)) continue;
     const float g = fmaxf(fmaxf((float)a - lp.fx, lp.fx - (float)(a + 1)), 0.f) * cp.ssx;
     if (g >= R) continue;
     const float hr = fminf(__builtin_amdgcn_sqrtf(R2 - g * g) * cp.inv_ssy, 8.f);
@@ -949,7 +957,7 @@ __device__ __forceinline__ void for_windows(const TilePlan &P, const CP &cp, int
 template <class CP>
 __device__ __forceinline__ bool span_window(const TilePlan &P, const CP &cp, int sa, int sy, int cy, LanePos lp,
                                             float R, int &ra, int &rb) {
-  if (sa >= P.nsc) return false;
+  if (sa < 0 || sa >= P.nsc) return false;
   const int a = P.sc0 + sa - P.sx;
   const float R2 = R * R;
   if (a < (int)floorf(lp.fx - R * cp.inv_ssx) || a > (int)floorf(lp.fx + R * cp.inv_ssx)) return false;
@@ -1244,10 +1252,14 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
     };
     if (staged) {
       // staged windows (one run each): first mark the candidates inside the cut-off (32-bit
-      // mask per window), then evaluate the Riemann terms for those only, in slot order --
-      // the same non-zero terms in the same order as a pass over every candidate
-      unsigned mk[kTileCols];
-      int w0[kTileCols];
+      // mask per window), then evaluate the Riemann terms for those only, windows in
+      // for_windows order and slots in order -- the same non-zero terms in the same order as
+      // a pass over every candidate.  The windows at offsets -k and +k share one loop over a
+      // 64-bit mask: a lane near one side of its sub-column has more terms on that side and
+      // fewer on the other, so a wave runs the pair for its lane with the most terms in both
+      // together instead of each window for its own longest lane.
+      unsigned mk[kWindowOrder];
+      int w0[kWindowOrder];
       // in-range mask of the run [s0, s1) (a run longer than 32 is evaluated right away)
       auto mark = [&](int s0, int s1) -> unsigned {
         if (s1 - s0 > 32) {
@@ -1263,25 +1275,51 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
         });
         return m;
       };
-      auto eval = [&](unsigned m, int s0) {
-        while (m) {
-          const int b = __builtin_ctz(m);
-          m &= m - 1u;
-          staged_j(Inside{}, s0 + b, true);
-        }
+      // the marked candidates of window (ma, sa0) and then of window (mb, sb0), in slot order;
+      // the LDS reads of the next candidate are issued before the current one is summed, so
+      // they overlap its Riemann term instead of stalling the lane
+      struct Cand {
+        R2 p, v, r;
+        int fl, l;
       };
+      auto eval2 = [&](unsigned ma, int sa0, unsigned mb, int sb0) {
+        unsigned long long m = (unsigned long long)ma | ((unsigned long long)mb << 32);
+        if (m == 0ull) return;
+        const int sb = sb0 - 32;
+        auto next = [&]() {
+          const int b = __builtin_ctzll(m);
+          m &= m - 1ull;
+          const int l = (b < 32 ? sa0 : sb) + b;
+          return Cand{s_pos[l], s_vel[l], s_rp[l], (int)s_fl[l], l};
+        };
+        auto use = [&](const Cand &q) {
+          const R dx = q.p.x - xi, dy = q.p.y - yg;
+          fluid_j(Inside{}, q.l == own, dx, dy, dx * dx + dy * dy, q.v.x, q.v.y, q.r.x, q.r.y, q.fl, true);
+        };
+        Cand cur = next();
+        while (m) {
+          const Cand nxt = next();
+          use(cur);
+          cur = nxt;
+        }
+        use(cur);
+      };
+      auto eval = [&](unsigned m, int s0) { eval2(m, s0, 0u, 0); };
+      const int off = P.sx - P.sc0;  // tile sub-column of the lanes' own sub-column
 #pragma unroll
-      for (int sa = 0; sa < kTileCols; ++sa) {
-        mk[sa] = 0u;
-        w0[sa] = 0;
+      for (int q = 0; q < kWindowOrder; ++q) {
+        mk[q] = 0u;
+        w0[q] = 0;
+        const int sa = off + window_offset(q);
         int ra, rb;
         if (!span_window(P, cp, sa, sy, cy, lp, Rp, ra, rb)) continue;
         const int s0 = s_lb[sa][ra - P.r0], s1 = s_lb[sa][rb + 1 - P.r0];
-        w0[sa] = s0;
-        mk[sa] = mark(s0, s1);
+        w0[q] = s0;
+        mk[q] = mark(s0, s1);
       }
+      eval(mk[0], w0[0]);
 #pragma unroll
-      for (int sa = 0; sa < kTileCols; ++sa) eval(mk[sa], w0[sa]);
+      for (int q = 1; q < kWindowOrder; q += 2) eval2(mk[q], w0[q], mk[q + 1], w0[q + 1]);
       if (wstaged) {  // walls of the 3x3 cells, cell column by cell column
         const int lr0 = max(cy - 1, 0) - W.wcr0, lr1 = min(cy + 1, cp.ny - 1) - W.wcr0;
 #pragma unroll
@@ -1291,7 +1329,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
         }
       }
     }
-    for_windows(P, cp, sy, cy, lp, Rp, [&](int sc, int ra, int rb) {
+    for_windows<true>(P, cp, sy, cy, lp, Rp, [&](int sc, int ra, int rb) {
       if (staged) return;
       const R bx = xi - cell_shift_x<Pol>((sc >> 1) - cx, cp);
       if constexpr (Pol::MIXED) {
@@ -1555,7 +1593,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
       gy += dy * w;
     }
   };
-  for_windows(P, cp, sy, cy, lp, Rp, [&](int sc, int ra, int rb) {
+  for_windows<false>(P, cp, sy, cy, lp, Rp, [&](int sc, int ra, int rb) {
     if (staged) {
       const int sa = sc - P.sc0;
       for_range_masked<4>(s_lb[sa][ra - P.r0], s_lb[sa][rb + 1 - P.r0], staged_add);

@@ -1080,10 +1080,15 @@ __device__ __forceinline__ void riemann_pair(R dx, R dy, R r2, R w, R ui, R vi, 
   }
 }
 
-// LDS capacity (particles) of a tile of BLOCK slots: the planner's budget.
+// LDS capacity (particles) of a tile of BLOCK slots: the planner's budget.  The M32 pair kernel
+// at 256 slots (25 B of LDS per candidate) gives up 104 of them so that three blocks share a
+// CU's 160 KB (53.7 KB each, 12 waves instead of 8); a 256-slot tile of a dense sub-column
+// stages about 1,650 candidates, well inside the 1,944 left.  FP64 candidates take 49 B, so
+// the next block count per CU is out of reach for P64 at any size without falling below a
+// typical tile.
 template <class Pol, int BLOCK>
 struct PairCap {
-  static constexpr int value = plan_cap(BLOCK);
+  static constexpr int value = plan_cap(BLOCK) - ((Pol::MIXED && BLOCK == 256) ? 104 : 0);
 };
 template <int BLOCK>
 struct ShiftCap {

@@ -58,12 +58,12 @@ def configs():
     }
 
 
-def measured_traffic(config, stage):
-    """HBM bytes per launch of the stage's kernel as measured by the committed rocprofv3
-    FETCH_SIZE / WRITE_SIZE passes (profiles/pmc_traffic_<workload>.json, copy-calibrated);
-    None where no measurement of this workload is committed."""
-    kern = {"shift": "k_fluid_shift<sphnj::M32", "fluid": "k_fluid_pair_update<sphnj::M32",
-            "cells": None}[stage]
+def measured_traffic(config, stage, block):
+    """HBM bytes per launch of the stage's kernel at the block size in use, as measured by the
+    committed rocprofv3 FETCH_SIZE / WRITE_SIZE passes (profiles/pmc_traffic_<workload>.json,
+    copy-calibrated); None where no measurement of this workload and block size is committed."""
+    kern = {"shift": f"k_fluid_shift<sphnj::M32, {block}>",
+            "fluid": f"k_fluid_pair_update<sphnj::M32, false, {block}>", "cells": None}[stage]
     f = ROOT / "profiles" / {"two_phase": "pmc_traffic_two_phase_3M.json"}.get(config, "-")
     if kern is None or not f.exists():
         return None
@@ -255,11 +255,14 @@ def main() -> int:
     per_launch_ms = prof[dom]["ms_per_launch"]
     bytes_launch = BYTES[dom]["m32"] * nf
     achieved = bytes_launch / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms else 0.0
-    kernel_name = {"fluid": "k_fluid_pair_update<M32>", "shift": "k_fluid_shift<M32>",
-                   "cells": "cells stage (hash + radix sort + permute + ranges)"}[dom]
+    blocks = {"fluid": (tune or {}).get("pair", {}).get("block", 256),
+              "shift": (tune or {}).get("shift", {}).get("block", 256), "cells": 256}
+    kernel_name = {"fluid": f"k_fluid_pair_update<M32, {blocks['fluid']}>",
+                   "shift": f"k_fluid_shift<M32, {blocks['shift']}>",
+                   "cells": "cells stage (hash + counting sort + permute + ranges)"}[dom]
     roofline = {"bound": "hbm", "kernel": kernel_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(args.config, dom),
+                "traffic": measured_traffic(args.config, dom, blocks[dom]),
                 "bytes_per_particle": BYTES[dom]["m32"], "ms_per_launch": round(per_launch_ms, 5)}
 
     cpu = None

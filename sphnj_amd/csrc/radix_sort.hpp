@@ -162,6 +162,105 @@ __global__ void __launch_bounds__(kSortThreads) radix_downsweep(const int *__res
   }
 }
 
+// The same stable passes for n <= kSmallSortN items in one 1024-thread block: wave w holds
+// the w-th sixteenth of the items (16 rounds of 64, in index order) in registers, ranks them
+// as the downsweep does, and the block exchanges them through LDS between passes, so a pass
+// costs barriers instead of three launches (a few thousand solids: one launch instead of nine).
+constexpr int kSmallSortThreads = 1024;
+constexpr int kSmallSortItems = 16;
+constexpr int kSmallSortN = kSmallSortThreads * kSmallSortItems;
+
+__global__ void __launch_bounds__(kSmallSortThreads) radix_sort_small(const int *__restrict__ kin,
+                                                                      int *__restrict__ kout,
+                                                                      int *__restrict__ vout, int n, int passes) {
+  constexpr int kWaves = kSmallSortThreads / 64;
+  __shared__ int s_key[kSmallSortN];
+  __shared__ int s_val[kSmallSortN];
+  __shared__ int woff[kWaves][kRadix];  // per-wave digit counts, then the waves' offsets
+  __shared__ int tmp[kRadix / 64];
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const int first = w * (kSmallSortItems * 64);
+  const unsigned long long lt = lanemask_lt();
+  int key[kSmallSortItems], val[kSmallSortItems], loc[kSmallSortItems];
+#pragma unroll
+  for (int r = 0; r < kSmallSortItems; ++r) {
+    const int idx = first + r * 64 + lane;
+    key[r] = idx < n ? kin[idx] : 0;
+    val[r] = idx;
+  }
+  for (int p = 0; p < passes; ++p) {
+    const int shift = p * kRadixBits;
+    for (int k = t; k < kWaves * kRadix; k += kSmallSortThreads) woff[k / kRadix][k % kRadix] = 0;
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSmallSortItems; ++r) {
+      const bool valid = first + r * 64 + lane < n;
+      const unsigned d = (((unsigned)key[r]) >> shift) & (kRadix - 1);
+      const unsigned long long peers = match_digit(d, valid);
+      const int rank = (int)__popcll(peers & lt);
+      int run = 0;
+      if (valid && rank == 0) {  // the group's first lane owns the wave's count of digit d
+        run = woff[w][d];
+        woff[w][d] = run + (int)__popcll(peers);
+      }
+      run = __shfl(run, valid ? (int)__builtin_ctzll(peers) : lane, 64);
+      loc[r] = run + rank;
+    }
+    __syncthreads();
+    // digit totals, their exclusive scan over the digits (threads 0..255), then each wave's
+    // offset within its digit
+    int dtot = 0, inc = 0;
+    if (t < kRadix) {
+#pragma unroll
+      for (int k = 0; k < kWaves; ++k) dtot += woff[k][t];
+      inc = dtot;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int u = __shfl_up(inc, o, 64);
+        if (lane >= o) inc += u;
+      }
+      if (lane == 63) tmp[w] = inc;
+    }
+    __syncthreads();
+    if (t < kRadix) {
+      int b = inc - dtot;
+      for (int k = 0; k < w; ++k) b += tmp[k];
+#pragma unroll
+      for (int k = 0; k < kWaves; ++k) {
+        const int c = woff[k][t];
+        woff[k][t] = b;
+        b += c;
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSmallSortItems; ++r) {
+      if (first + r * 64 + lane < n) {
+        const int pos = woff[w][(((unsigned)key[r]) >> shift) & (kRadix - 1)] + loc[r];
+        s_key[pos] = key[r];
+        s_val[pos] = val[r];
+      }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < kSmallSortItems; ++r) {
+      const int idx = first + r * 64 + lane;
+      if (idx < n) {
+        key[r] = s_key[idx];
+        val[r] = s_val[idx];
+      }
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < kSmallSortItems; ++r) {
+    const int idx = first + r * 64 + lane;
+    if (idx < n) {
+      kout[idx] = key[r];
+      vout[idx] = val[r];
+    }
+  }
+}
+
 // Work buffers for one sorter instance (sized for n items).
 struct RadixSorter {
   int n = 0, nblocks = 0, passes = 0;
@@ -175,14 +274,21 @@ inline int radix_passes_for(int max_key) {
   return p < 1 ? 1 : p;
 }
 
-// Sort (keys[i], i) for i < n.  keys is overwritten on even pass counts.  On return *out_keys /
-// *out_vals point at the sorted keys and values (inside keys / the sorter's buffers).
+// Sort (keys[i], i) for i < n.  keys is overwritten on even pass counts of the multi-block
+// path.  On return *out_keys / *out_vals point at the sorted keys and values (inside keys / the
+// sorter's buffers).
 inline hipError_t radix_sort_pairs(const RadixSorter &S, int *keys, int **out_keys, int **out_vals,
                                    hipStream_t st) {
   if (S.n == 0) {
     *out_keys = keys;
     *out_vals = S.v0;
     return hipSuccess;
+  }
+  if (S.n <= kSmallSortN) {
+    radix_sort_small<<<1, kSmallSortThreads, 0, st>>>(keys, S.k1, S.v0, S.n, S.passes);
+    *out_keys = S.k1;
+    *out_vals = S.v0;
+    return hipGetLastError();
   }
   int *ka = keys, *kb = S.k1;
   int *va = nullptr, *vb = S.v1;

@@ -1704,6 +1704,24 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
 // ======================================================================================
 // SOLID
 // ======================================================================================
+// Run r (0 .. kCellRuns - 1) of the fluid in the 3x3 cells around cell (cx, cy): cell
+// (cx + r / 6 - 1, cy + (r / 2) % 3 - 1), sub-column r % 2, both of its sub-rows.  False when the
+// cell lies off the grid; else the run's slots and the lane's position relative to the run's
+// cell origin (M32 views are relative to their own cell).
+constexpr int kCellRuns = 18;
+template <class Pol, class CP, class R>
+__device__ __forceinline__ bool cell_run(const Dev<Pol> &d, const CP &cp, int cx, int cy, int r, R xi, R yi,
+                                         int2 &rr, R &bx, R &by) {
+  const int a = r / 6 - 1, b = (r >> 1) % 3 - 1, jx = cx + a, jy = cy + b;
+  if (jx < 0 || jx >= cp.nx || jy < 0 || jy >= cp.ny) return false;
+  const int sny = 2 * cp.ny;
+  const int k = 4 * cp.cell_off + (2 * jx + (r & 1)) * sny + 2 * jy;
+  rr = run_join(d.f_srange[k], d.f_srange[k + 1]);
+  bx = xi - cell_shift_x<Pol>(a, cp);
+  by = yi - cell_shift_y<Pol>(b, cp);
+  return true;
+}
+
 // Solid kernels give each particle a group of kSolidLanes lanes: the lanes split the
 // particle's candidates (cell ranges, reference list) and combine their partial sums with
 // shuffles, so a few thousand solids still fill the chip.
@@ -1741,30 +1759,28 @@ __global__ void __launch_bounds__(256) k_solid_couple(Dev<Pol> d) {
     const LoopConsts<R> K(cp);
     const R eR = cp.eR, k2c = cp.k2c, hinv2 = cp.hinv2;
     const auto &F = d.fa;
-    for (int a = -1; a <= 1; ++a) {
-      const int jx = cx + a;
-      if (jx < 0 || jx >= cp.nx) continue;
-      const R bx = xi - cell_shift_x<Pol>(a, cp);
-      for (int b = -1; b <= 1; ++b) {
-        const int jy = cy + b;
-        if (jy < 0 || jy >= cp.ny) continue;
-        const R by = yi - cell_shift_y<Pol>(b, cp);
-        const int q = cp.cell_off + jx * cp.ny + jy;
-        int2 run[2];
-        fluid_cell_runs(d, cp, jx, jy, run[0], run[1]);
-        for (int h = 0; h < 2; ++h) {
-          for (int s = run[h].x + l; s < run[h].y; s += kSolidLanes) {
-            const R dx = F.vx[s] - bx, dy = F.vy[s] - by;
-            const R r2 = dx * dx + dy * dy;
-            if (r2 < K.rc2) {
-              const R w = (kexp<R>(-(r2 * hinv2)) - eR) * k2c;
-              S += w;
-              P += F.p[s] * w;
-            }
-          }
+    // fluid: the 3x3 cells' 18 runs spread over the group's lanes (as in the force kernel)
+    for (int r = l; r < kCellRuns; r += kSolidLanes) {
+      int2 rr;
+      R bx, by;
+      if (!cell_run(d, cp, cx, cy, r, xi, yi, rr, bx, by)) continue;
+      for (int s = rr.x; s < rr.y; ++s) {
+        const R dx = F.vx[s] - bx, dy = F.vy[s] - by;
+        const R r2 = dx * dx + dy * dy;
+        if (r2 < K.rc2) {
+          const R w = (kexp<R>(-(r2 * hinv2)) - eR) * k2c;
+          S += w;
+          P += F.p[s] * w;
         }
-        const int2 rs = d.s_range[q];
-        for (int k = rs.x + l; k < rs.y; k += kSolidLanes) {
+      }
+    }
+    // solids: one cell of the 3x3 per lane
+    if (l < 9) {
+      const int a = l / 3 - 1, b = l % 3 - 1, jx = cx + a, jy = cy + b;
+      if (jx >= 0 && jx < cp.nx && jy >= 0 && jy < cp.ny) {
+        const R bx = xi - cell_shift_x<Pol>(a, cp), by = yi - cell_shift_y<Pol>(b, cp);
+        const int2 rs = d.s_range[cp.cell_off + jx * cp.ny + jy];
+        for (int k = rs.x; k < rs.y; ++k) {
           const int j = d.s_sorted[k];
           const R dx = d.s_vx[j] - bx, dy = d.s_vy[j] - by;
           const R w = K.w(dx * dx + dy * dy);
@@ -1947,32 +1963,26 @@ __global__ void __launch_bounds__(256) k_solid_force_update(Dev<Pol> d, int sub)
       const R uiv = (R)ui, viv = (R)vi, psi = d.s_p[i];
       const R c0 = cp.c[0], c1 = cp.c[1];
       const auto &F = d.fa;
-      for (int a = -1; a <= 1; ++a) {
-        const int jx = cx + a;
-        if (jx < 0 || jx >= cp.nx) continue;
-        const R bx = xiv - cell_shift_x<Pol>(a, cp);
-        for (int b = -1; b <= 1; ++b) {
-          const int jy = cy + b;
-          if (jy < 0 || jy >= cp.ny) continue;
-          const R by = yiv - cell_shift_y<Pol>(b, cp);
-          int2 run[2];
-          fluid_cell_runs(d, cp, jx, jy, run[0], run[1]);
-          for (int h = 0; h < 2; ++h) {
-            for (int s = run[h].x + l; s < run[h].y; s += kSolidLanes) {
-              const R dx = F.vx[s] - bx, dy = F.vy[s] - by;
-              const R r2 = dx * dx + dy * dy;
-              if (!(r2 < K.rc2)) continue;
-              const R w = K.w(r2);
-              const R inv_r = r2 > R(0) ? krsqrt<R>(r2) : R(0);
-              const R rhoj = F.vrho[s];
-              R ps = R(0.5) * (F.p[s] + psi);
-              const R du = ((uiv - F.vu[s]) * dx + (viv - F.vv[s]) * dy) * inv_r;
-              if (du >= R(0)) ps += R(0.5) * rhoj * du * fmin(R(3) * du, d.f_phase[s] ? c1 : c0);
-              const R t = R(-2) * ps * rr_inv * w;
-              cpx += t * dx;
-              cpy += t * dy;
-            }
-          }
+      // the 3x3 cells' 18 fluid runs (one per sub-column of a cell) go to the group's lanes,
+      // each lane summing its runs in slot order: the lanes read their runs' ranges together
+      // instead of the group walking the cells one after another
+      for (int r = l; r < kCellRuns; r += kSolidLanes) {
+        int2 rr;
+        R bx, by;
+        if (!cell_run(d, cp, cx, cy, r, xiv, yiv, rr, bx, by)) continue;
+        for (int s = rr.x; s < rr.y; ++s) {
+          const R dx = F.vx[s] - bx, dy = F.vy[s] - by;
+          const R r2 = dx * dx + dy * dy;
+          if (!(r2 < K.rc2)) continue;
+          const R w = K.w(r2);
+          const R inv_r = r2 > R(0) ? krsqrt<R>(r2) : R(0);
+          const R rhoj = F.vrho[s];
+          R ps = R(0.5) * (F.p[s] + psi);
+          const R du = ((uiv - F.vu[s]) * dx + (viv - F.vv[s]) * dy) * inv_r;
+          if (du >= R(0)) ps += R(0.5) * rhoj * du * fmin(R(3) * du, d.f_phase[s] ? c1 : c0);
+          const R t = R(-2) * ps * rr_inv * w;
+          cpx += t * dx;
+          cpy += t * dy;
         }
       }
       // solid -> solid over the reference-configuration list

@@ -166,6 +166,8 @@ __global__ void __launch_bounds__(kSortThreads) radix_downsweep(const int *__res
 // the w-th sixteenth of the items (16 rounds of 64, in index order) in registers, ranks them
 // as the downsweep does, and the block exchanges them through LDS between passes, so a pass
 // costs barriers instead of three launches (a few thousand solids: one launch instead of nine).
+// The digits are taken from key - min(key), so a set of keys spanning few values (the cells of
+// one solid body) needs fewer passes than the largest key would.
 constexpr int kSmallSortThreads = 1024;
 constexpr int kSmallSortItems = 16;
 constexpr int kSmallSortN = kSmallSortThreads * kSmallSortItems;
@@ -178,24 +180,47 @@ __global__ void __launch_bounds__(kSmallSortThreads) radix_sort_small(const int 
   __shared__ int s_val[kSmallSortN];
   __shared__ int woff[kWaves][kRadix];  // per-wave digit counts, then the waves' offsets
   __shared__ int tmp[kRadix / 64];
+  __shared__ int s_lo[kWaves], s_hi[kWaves];
   const int t = threadIdx.x, w = t >> 6, lane = t & 63;
   const int first = w * (kSmallSortItems * 64);
   const unsigned long long lt = lanemask_lt();
   int key[kSmallSortItems], val[kSmallSortItems], loc[kSmallSortItems];
+  int lo = 0x7fffffff, hi = 0;
 #pragma unroll
   for (int r = 0; r < kSmallSortItems; ++r) {
     const int idx = first + r * 64 + lane;
     key[r] = idx < n ? kin[idx] : 0;
     val[r] = idx;
+    if (idx < n) {
+      lo = min(lo, key[r]);
+      hi = max(hi, key[r]);
+    }
   }
-  for (int p = 0; p < passes; ++p) {
+  for (int o = 32; o > 0; o >>= 1) {
+    lo = min(lo, __shfl_xor(lo, o, 64));
+    hi = max(hi, __shfl_xor(hi, o, 64));
+  }
+  if (lane == 0) {
+    s_lo[w] = lo;
+    s_hi[w] = hi;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < kWaves; ++k) {
+    lo = min(lo, s_lo[k]);
+    hi = max(hi, s_hi[k]);
+  }
+  const unsigned span = (unsigned)(hi - lo);  // keys are non-negative, so hi - lo fits
+  int need = 0;
+  while (need < passes && (span >> (need * kRadixBits)) != 0u) ++need;
+  for (int p = 0; p < need; ++p) {
     const int shift = p * kRadixBits;
     for (int k = t; k < kWaves * kRadix; k += kSmallSortThreads) woff[k / kRadix][k % kRadix] = 0;
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < kSmallSortItems; ++r) {
       const bool valid = first + r * 64 + lane < n;
-      const unsigned d = (((unsigned)key[r]) >> shift) & (kRadix - 1);
+      const unsigned d = ((unsigned)(key[r] - lo) >> shift) & (kRadix - 1);
       const unsigned long long peers = match_digit(d, valid);
       const int rank = (int)__popcll(peers & lt);
       int run = 0;
@@ -236,7 +261,7 @@ __global__ void __launch_bounds__(kSmallSortThreads) radix_sort_small(const int 
 #pragma unroll
     for (int r = 0; r < kSmallSortItems; ++r) {
       if (first + r * 64 + lane < n) {
-        const int pos = woff[w][(((unsigned)key[r]) >> shift) & (kRadix - 1)] + loc[r];
+        const int pos = woff[w][((unsigned)(key[r] - lo) >> shift) & (kRadix - 1)] + loc[r];
         s_key[pos] = key[r];
         s_val[pos] = val[r];
       }

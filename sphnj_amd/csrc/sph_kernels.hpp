@@ -78,6 +78,32 @@ __device__ __forceinline__ void fluid_cell_runs(const Dev<Pol> &d, const CP &cp,
   r1 = run_join(d.f_srange[k + sny], d.f_srange[k + sny + 1]);
 }
 
+// Run r (0 .. kCellRuns - 1) of the fluid in the 3x3 cells around cell (cx, cy): cell
+// (cx + r / 6 - 1, cy + (r / 2) % 3 - 1), sub-column r % 2, both of its sub-rows.  False when the
+// cell lies off the grid; else the run's slots and the lane's position relative to the run's
+// cell origin (M32 views are relative to their own cell).
+constexpr int kCellRuns = 18;
+template <class Pol, class CP, class R>
+__device__ __forceinline__ bool cell_run(const Dev<Pol> &d, const CP &cp, int cx, int cy, int r, R xi, R yi,
+                                         int2 &rr, R &bx, R &by) {
+  const int a = r / 6 - 1, b = (r >> 1) % 3 - 1, jx = cx + a, jy = cy + b;
+  if (jx < 0 || jx >= cp.nx || jy < 0 || jy >= cp.ny) return false;
+  const int sny = 2 * cp.ny;
+  const int k = 4 * cp.cell_off + (2 * jx + (r & 1)) * sny + 2 * jy;
+  rr = run_join(d.f_srange[k], d.f_srange[k + 1]);
+  bx = xi - cell_shift_x<Pol>(a, cp);
+  by = yi - cell_shift_y<Pol>(b, cp);
+  return true;
+}
+
+// Sum over the aligned groups of W lanes (W a power of two dividing 64).
+template <int W, class T>
+__device__ __forceinline__ T lanes_sum(T v) {
+#pragma unroll
+  for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
+  return v;
+}
+
 // Case owning global cell i (cases hold consecutive cell ranges).
 template <class Pol>
 __device__ __forceinline__ int case_of_cell(int i, const Dev<Pol> &d) {
@@ -407,48 +433,63 @@ __global__ void __launch_bounds__(256) k_wall_normal(Dev<Pol> d) {
 // getVirtData + virtDataCalculate (src/virt.cu:61-149): Shepard-interpolated velocity and
 // hydrostatically corrected pressure from the fluid, slip / no-slip mirroring, phase of the
 // nearest fluid particle (first strict minimum in iteration order), density from the EOS.
+// Each wall particle has a group of kWallLanes lanes that split its fluid runs.
+constexpr int kWallLanes = 8;
 template <class Pol>
 __global__ void __launch_bounds__(256) k_wall_update(Dev<Pol> d) {
   using R = typename Pol::R;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= d.nw) return;
-  const int c = case_of(i, d.w_case_off, d.ncase);
-  const auto &cp = d.cases[c];
-  const int pk = d.w_cxy[i], cx = cxy_x(pk), cy = cxy_y(pk);
-  const R xi = d.w_vx[i], yi = d.w_vy[i];
-  const auto &F = d.fa;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int i = gid / kWallLanes, l = gid % kWallLanes;
+  const bool active = i < d.nw;
   R S = 0, U = 0, V = 0, P = 0, nearest = 0;
-  int phase = 0;
-  bool found = false;
-  for (int a = -1; a <= 1; ++a) {
-    const int jx = cx + a;
-    if (jx < 0 || jx >= cp.nx) continue;
-    const R bx = xi - cell_shift_x<Pol>(a, cp);
-    for (int b = -1; b <= 1; ++b) {
-      const int jy = cy + b;
-      if (jy < 0 || jy >= cp.ny) continue;
-      const R by = yi - cell_shift_y<Pol>(b, cp);
-      int2 run[2];
-      fluid_cell_runs(d, cp, jx, jy, run[0], run[1]);
-      for (int h = 0; h < 2; ++h) {
-        for (int s = run[h].x; s < run[h].y; ++s) {
-          const R dx = F.vx[s] - bx, dy = F.vy[s] - by;
-          const R r2 = dx * dx + dy * dy;
-          if (!(r2 < cp.rc2)) continue;
-          const R w = (kexp<R>(-(r2 * cp.hinv2)) - cp.eR) * cp.k2c;
-          S += w;
-          U += F.vu[s] * w;
-          V += F.vv[s] * w;
-          P += (F.p[s] - F.vrho[s] * (cp.gx * dx + cp.gy * dy)) * w;
-          if (!found || r2 < nearest) {
-            found = true;
-            nearest = r2;
-            phase = d.f_phase[s];
-          }
+  int phase = 0, near_run = kCellRuns;  // run holding the lane's nearest fluid (none yet)
+  int c = 0;
+  if (active) {
+    c = case_of(i, d.w_case_off, d.ncase);
+    const auto &cp = d.cases[c];
+    const int pk = d.w_cxy[i], cx = cxy_x(pk), cy = cxy_y(pk);
+    const R xi = d.w_vx[i], yi = d.w_vy[i];
+    const auto &F = d.fa;
+    // the 3x3 cells' 18 fluid runs over the wall particle's lanes, each lane in run order
+    for (int r = l; r < kCellRuns; r += kWallLanes) {
+      int2 rr;
+      R bx, by;
+      if (!cell_run(d, cp, cx, cy, r, xi, yi, rr, bx, by)) continue;
+      for (int s = rr.x; s < rr.y; ++s) {
+        const R dx = F.vx[s] - bx, dy = F.vy[s] - by;
+        const R r2 = dx * dx + dy * dy;
+        if (!(r2 < cp.rc2)) continue;
+        const R w = (kexp<R>(-(r2 * cp.hinv2)) - cp.eR) * cp.k2c;
+        S += w;
+        U += F.vu[s] * w;
+        V += F.vv[s] * w;
+        P += (F.p[s] - F.vrho[s] * (cp.gx * dx + cp.gy * dy)) * w;
+        if (near_run == kCellRuns || r2 < nearest) {
+          near_run = r;
+          nearest = r2;
+          phase = d.f_phase[s];
         }
       }
     }
   }
+  S = lanes_sum<kWallLanes>(S);
+  U = lanes_sum<kWallLanes>(U);
+  V = lanes_sum<kWallLanes>(V);
+  P = lanes_sum<kWallLanes>(P);
+  // the nearest fluid of the whole scan: smallest r^2, ties to the earlier run (the scan order
+  // of the cells, so the first strict minimum of a serial scan)
+#pragma unroll
+  for (int o = kWallLanes / 2; o > 0; o >>= 1) {
+    const R n2 = __shfl_xor(nearest, o, kWallLanes);
+    const int q2 = __shfl_xor(near_run, o, kWallLanes), p2 = __shfl_xor(phase, o, kWallLanes);
+    if (q2 != kCellRuns && (near_run == kCellRuns || n2 < nearest || (n2 == nearest && q2 < near_run))) {
+      nearest = n2;
+      near_run = q2;
+      phase = p2;
+    }
+  }
+  if (!active || l != 0) return;
+  const auto &cp = d.cases[c];
   R u = 0, v = 0, p = 0;
   if (S != R(0)) {
     u = U / S;
@@ -1704,24 +1745,6 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
 // ======================================================================================
 // SOLID
 // ======================================================================================
-// Run r (0 .. kCellRuns - 1) of the fluid in the 3x3 cells around cell (cx, cy): cell
-// (cx + r / 6 - 1, cy + (r / 2) % 3 - 1), sub-column r % 2, both of its sub-rows.  False when the
-// cell lies off the grid; else the run's slots and the lane's position relative to the run's
-// cell origin (M32 views are relative to their own cell).
-constexpr int kCellRuns = 18;
-template <class Pol, class CP, class R>
-__device__ __forceinline__ bool cell_run(const Dev<Pol> &d, const CP &cp, int cx, int cy, int r, R xi, R yi,
-                                         int2 &rr, R &bx, R &by) {
-  const int a = r / 6 - 1, b = (r >> 1) % 3 - 1, jx = cx + a, jy = cy + b;
-  if (jx < 0 || jx >= cp.nx || jy < 0 || jy >= cp.ny) return false;
-  const int sny = 2 * cp.ny;
-  const int k = 4 * cp.cell_off + (2 * jx + (r & 1)) * sny + 2 * jy;
-  rr = run_join(d.f_srange[k], d.f_srange[k + 1]);
-  bx = xi - cell_shift_x<Pol>(a, cp);
-  by = yi - cell_shift_y<Pol>(b, cp);
-  return true;
-}
-
 // Solid kernels give each particle a group of kSolidLanes lanes: the lanes split the
 // particle's candidates (cell ranges, reference list) and combine their partial sums with
 // shuffles, so a few thousand solids still fill the chip.

@@ -763,7 +763,7 @@ struct Engine : EngineBase {
         return SPHNJ_OK;
       case SPHNJ_STAGE_WALL:
         if (nw > 0) {
-          k_wall_update<Pol><<<nblk(nw, blk_stream), blk_stream, 0, stream>>>(d);
+          k_wall_update<Pol><<<nblk(nw * kWallLanes, 256), 256, 0, stream>>>(d);
           LAUNCH_CHECK();
         }
         return SPHNJ_OK;

@@ -104,6 +104,38 @@ __device__ __forceinline__ T lanes_sum(T v) {
   return v;
 }
 
+// A fluid neighbour's fields as the solid loops read them.
+template <class R>
+struct FluidNb {
+  R x, y, u, v, rho, p;
+  int phase;
+  template <class FA>
+  __device__ __forceinline__ static FluidNb load(const FA &F, const int8_t *phase, int s) {
+    return FluidNb{F.vx[s], F.vy[s], F.vu[s], F.vv[s], F.vrho[s], F.p[s], (int)phase[s]};
+  }
+};
+
+// Position and pressure of a fluid neighbour.
+template <class R>
+struct R2c {
+  R x, y, p;
+};
+
+// Calls use(s, v) for s in [lo, hi) in order, with v = load(s) read U slots at a time: the
+// loads of U slots are issued together (indices clamped into the range), so a lane waits once
+// per U slots instead of once per slot.
+template <int U, class Ld, class F>
+__device__ __forceinline__ void for_run(int lo, int hi, Ld &&load, F &&use) {
+  for (int b = lo; b < hi; b += U) {
+    decltype(load(0)) v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = load(min(b + u, hi - 1));
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (b + u < hi) use(b + u, v[u]);
+  }
+}
+
 // Case owning global cell i (cases hold consecutive cell ranges).
 template <class Pol>
 __device__ __forceinline__ int case_of_cell(int i, const Dev<Pol> &d) {
@@ -1762,6 +1794,24 @@ __device__ __forceinline__ int group_or(int v) {
   return v;
 }
 
+// A lane's entries e = e0, e0 + kSolidLanes, ... < e1 of a reference list, U at a time: the U
+// neighbour indices are read together, then the U neighbours' fields through them, then
+// use(e, j, v) runs in entry order -- two waits per U entries instead of two per entry.
+template <int U, class Ld, class F>
+__device__ __forceinline__ void for_refs(const int *ref_j, int e0, int e1, Ld &&load, F &&use) {
+  for (int b = e0; b < e1; b += U * kSolidLanes) {
+    int jj[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) jj[u] = ref_j[min(b + u * kSolidLanes, e1 - 1)];
+    decltype(load(0, 0)) v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = load(min(b + u * kSolidLanes, e1 - 1), jj[u]);
+#pragma unroll
+    for (int u = 0; u < U; ++u)
+      if (b + u * kSolidLanes < e1) use(b + u * kSolidLanes, jj[u], v[u]);
+  }
+}
+
 // solidCoupleUpdate (src/solid.cu:355-444): Shepard pressure from the fluid, normal from the
 // solid neighbours in the current cells; and the check that every reference-list neighbour
 // still lies within the 3x3 current cells.
@@ -1787,15 +1837,15 @@ __global__ void __launch_bounds__(256) k_solid_couple(Dev<Pol> d) {
       int2 rr;
       R bx, by;
       if (!cell_run(d, cp, cx, cy, r, xi, yi, rr, bx, by)) continue;
-      for (int s = rr.x; s < rr.y; ++s) {
-        const R dx = F.vx[s] - bx, dy = F.vy[s] - by;
+      for_run<4>(rr.x, rr.y, [&](int s) { return R2c<R>{F.vx[s], F.vy[s], F.p[s]}; }, [&](int, const R2c<R> &f) {
+        const R dx = f.x - bx, dy = f.y - by;
         const R r2 = dx * dx + dy * dy;
         if (r2 < K.rc2) {
           const R w = (kexp<R>(-(r2 * hinv2)) - eR) * k2c;
           S += w;
-          P += F.p[s] * w;
+          P += f.p * w;
         }
-      }
+      });
     }
     // solids: one cell of the 3x3 per lane
     if (l < 9) {
@@ -1867,15 +1917,20 @@ __global__ void __launch_bounds__(256) k_solid_deform(Dev<Pol> d, int sub) {
     c = case_of(i, d.s_case_off, d.ncase);
     if (sub < d.cases[c].S) {
       const double dix = d.sx[i] - d.sX[i], diy = d.sy[i] - d.sY[i];
-      for (int e = d.s_ref_off[i] + l; e < d.s_ref_off[i + 1]; e += kSolidLanes) {
-        const int j = d.s_ref_j[e];
-        const R gX = d.s_ref_g[2 * e], gY = d.s_ref_g[2 * e + 1];
-        const R ux = (R)((d.sx[j] - d.sX[j]) - dix), uy = (R)((d.sy[j] - d.sY[j]) - diy);
-        h00 += ux * gX;
-        h01 += ux * gY;
-        h10 += uy * gX;
-        h11 += uy * gY;
-      }
+      struct Nb {
+        R gX, gY;
+        double x, X, y, Y;
+      };
+      for_refs<4>(
+          d.s_ref_j, d.s_ref_off[i] + l, d.s_ref_off[i + 1],
+          [&](int e, int j) { return Nb{d.s_ref_g[2 * e], d.s_ref_g[2 * e + 1], d.sx[j], d.sX[j], d.sy[j], d.sY[j]}; },
+          [&](int, int, const Nb &n) {
+            const R ux = (R)((n.x - n.X) - dix), uy = (R)((n.y - n.Y) - diy);
+            h00 += ux * n.gX;
+            h01 += ux * n.gY;
+            h10 += uy * n.gX;
+            h11 += uy * n.gY;
+          });
     }
   }
   h00 = group_sum(h00);
@@ -1993,39 +2048,46 @@ __global__ void __launch_bounds__(256) k_solid_force_update(Dev<Pol> d, int sub)
         int2 rr;
         R bx, by;
         if (!cell_run(d, cp, cx, cy, r, xiv, yiv, rr, bx, by)) continue;
-        for (int s = rr.x; s < rr.y; ++s) {
-          const R dx = F.vx[s] - bx, dy = F.vy[s] - by;
+        for_run<4>(rr.x, rr.y, [&](int s) { return FluidNb<R>::load(F, d.f_phase, s); }, [&](int, const FluidNb<R> &f) {
+          const R dx = f.x - bx, dy = f.y - by;
           const R r2 = dx * dx + dy * dy;
-          if (!(r2 < K.rc2)) continue;
+          if (!(r2 < K.rc2)) return;
           const R w = K.w(r2);
           const R inv_r = r2 > R(0) ? krsqrt<R>(r2) : R(0);
-          const R rhoj = F.vrho[s];
-          R ps = R(0.5) * (F.p[s] + psi);
-          const R du = ((uiv - F.vu[s]) * dx + (viv - F.vv[s]) * dy) * inv_r;
-          if (du >= R(0)) ps += R(0.5) * rhoj * du * fmin(R(3) * du, d.f_phase[s] ? c1 : c0);
+          R ps = R(0.5) * (f.p + psi);
+          const R du = ((uiv - f.u) * dx + (viv - f.v) * dy) * inv_r;
+          if (du >= R(0)) ps += R(0.5) * f.rho * du * fmin(R(3) * du, f.phase ? c1 : c0);
           const R t = R(-2) * ps * rr_inv * w;
           cpx += t * dx;
           cpy += t * dy;
-        }
+        });
       }
       // solid -> solid over the reference-configuration list
       const R *Fi = d.s_F + 4 * i, *Li = d.s_L + 4 * i, *PLi = d.s_PL + 4 * i;
       const Mat2<R> FL = mmul(Mat2<R>{Fi[0], Fi[1], Fi[2], Fi[3]}, Mat2<R>{Li[0], Li[1], Li[2], Li[3]});
       const R pl0 = PLi[0], pl1 = PLi[1], pl2 = PLi[2], pl3 = PLi[3];
       const R h = cp.h, eps_h2 = cp.eps_h2, alpha = cp.alpha, beta = cp.beta, cs = cp.c_s;
-      for (int e = d.s_ref_off[i] + l; e < d.s_ref_off[i + 1]; e += kSolidLanes) {
-        const int j = d.s_ref_j[e];
-        const R gX = d.s_ref_g[2 * e], gY = d.s_ref_g[2 * e + 1];
-        const R *PLj = d.s_PL + 4 * j;
-        ax += ((PLj[0] + pl0) * gX + (PLj[1] + pl1) * gY) * rr_inv;
-        ay += ((PLj[2] + pl2) * gX + (PLj[3] + pl3) * gY) * rr_inv;
-        const R dx = (R)(d.sx[j] - xi), dy = (R)(d.sy[j] - yi);
-        const R du = (R)(d.su[j] - ui), dv = (R)(d.sv[j] - vi);
-        const R nu = h * (du * dx + dv * dy) / (dx * dx + dy * dy + eps_h2);
-        const R vis = alpha * nu * cs - beta * nu * nu;
-        ax += vis * (FL.a * gX + FL.b * gY);
-        ay += vis * (FL.c * gX + FL.d * gY);
-      }
+      struct Nb {
+        R gX, gY, p0, p1, p2, p3;
+        double x, y, u, v;
+      };
+      for_refs<4>(
+          d.s_ref_j, d.s_ref_off[i] + l, d.s_ref_off[i + 1],
+          [&](int e, int j) {
+            const R *PLj = d.s_PL + 4 * j;
+            return Nb{d.s_ref_g[2 * e], d.s_ref_g[2 * e + 1], PLj[0], PLj[1], PLj[2], PLj[3],
+                      d.sx[j], d.sy[j], d.su[j], d.sv[j]};
+          },
+          [&](int, int, const Nb &n) {
+            ax += ((n.p0 + pl0) * n.gX + (n.p1 + pl1) * n.gY) * rr_inv;
+            ay += ((n.p2 + pl2) * n.gX + (n.p3 + pl3) * n.gY) * rr_inv;
+            const R dx = (R)(n.x - xi), dy = (R)(n.y - yi);
+            const R du = (R)(n.u - ui), dv = (R)(n.v - vi);
+            const R nu = h * (du * dx + dv * dy) / (dx * dx + dy * dy + eps_h2);
+            const R vis = alpha * nu * cs - beta * nu * nu;
+            ax += vis * (FL.a * n.gX + FL.b * n.gY);
+            ay += vis * (FL.c * n.gX + FL.d * n.gY);
+          });
     }
   }
   cpx = group_sum(cpx);

@@ -670,66 +670,87 @@ __global__ void __launch_bounds__(256) k_wall_rows(Dev<Pol> d, int G, const int 
 template <class Pol>
 __global__ void __launch_bounds__(256) k_subcol_rows(Dev<Pol> d, int G, const int *col_case, PlanJobs jobs,
                                                      int njobs) {
-  const int g = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  // one block per sub-column: the four waves build its row table together, then wave 0
+  // counts its tiles while waves 1 and 2 audit and index its cell column
+  const int g = blockIdx.x, t = threadIdx.x, w = t >> 6, lane = t & 63;
   if (g >= G) return;
+  __shared__ int s_min[4], s_end[4], s_s0;
   const auto &cp = d.cases[col_case[g]];
   const int sx = g - cp.col_off, ny2 = 2 * cp.ny;
   const int kb = 4 * cp.cell_off + sx * ny2;
   int *lb = d.f_lb + cp.lb_off + sx * (ny2 + 1);
-  // each lane takes a contiguous run of rows, [k0, k1), read kU at a time so the loads overlap
-  constexpr int kU = 8;
-  const int per = (ny2 + 63) >> 6, k0 = min(lane * per, ny2), k1 = min(k0 + per, ny2);
+  // each thread takes a contiguous run of rows [k0, k1), held in registers when it has at most
+  // kR of them (up to 2048 sub-rows), else read again in the backward pass
+  constexpr int kR = 8;
+  const int per = (ny2 + 255) >> 8, k0 = min(t * per, ny2), k1 = min(k0 + per, ny2);
   int lmin = 0x7fffffff, end = 0;
-  for (int ka = k0; ka < k1; ka += kU) {
-    int2 r[kU];
+  int2 r[kR];
+  for (int ka = k0; ka < k1; ka += kR) {
 #pragma unroll
-    for (int u = 0; u < kU; ++u) r[u] = ka + u < k1 ? d.f_srange[kb + ka + u] : make_int2(0, 0);
+    for (int u = 0; u < kR; ++u) r[u] = ka + u < k1 ? d.f_srange[kb + ka + u] : make_int2(0, 0);
 #pragma unroll
-    for (int u = 0; u < kU; ++u)
+    for (int u = 0; u < kR; ++u)
       if (r[u].y > r[u].x) {
         lmin = min(lmin, r[u].x);
         end = max(end, r[u].y);
       }
   }
   for (int o = 32; o > 0; o >>= 1) end = max(end, __shfl_xor(end, o, 64));
-  for (int o = 1; o < 64; o <<= 1) {  // suffix minimum over the lanes
+  for (int o = 1; o < 64; o <<= 1) {  // suffix minimum over the wave's lanes
     const int u = __shfl_down(lmin, o, 64);
     if (lane + o < 64) lmin = min(lmin, u);
   }
+  if (lane == 0) {
+    s_min[w] = lmin;
+    s_end[w] = end;
+  }
+  __syncthreads();
+  end = max(max(s_end[0], s_end[1]), max(s_end[2], s_end[3]));
   int run = __shfl_down(lmin, 1, 64);
-  run = min(lane == 63 ? 0x7fffffff : run, end);  // rows past the last non-empty one: the end
-  for (int ka = k1 - 1; ka >= k0; ka -= kU) {
-    int2 r[kU];
+  if (lane == 63) run = 0x7fffffff;
+  for (int k = w + 1; k < 4; ++k) run = min(run, s_min[k]);
+  run = min(run, end);  // rows past the last non-empty one: the end
+  if (per <= kR) {
 #pragma unroll
-    for (int u = 0; u < kU; ++u) r[u] = ka - u >= k0 ? d.f_srange[kb + ka - u] : make_int2(0, 0);
-#pragma unroll
-    for (int u = 0; u < kU; ++u)
-      if (ka - u >= k0) {
+    for (int u = kR - 1; u >= 0; --u)
+      if (k0 + u < k1) {
         if (r[u].y > r[u].x) run = min(run, r[u].x);
-        lb[ka - u] = run;
+        lb[k0 + u] = run;
       }
+  } else {
+    for (int ka = k1 - 1; ka >= k0; --ka) {
+      const int2 q = d.f_srange[kb + ka];
+      if (q.y > q.x) run = min(run, q.x);
+      lb[ka] = run;
+    }
   }
-  if (lane == 0) lb[ny2] = end;
-  // the sub-column's tile counts (lane 0 ends with run = lb[0])
-  const int s0 = __shfl(run, 0, 64);
-  for (int j = 0; j < njobs; ++j) {
-    const int nt = count_tiles(d, s0, end - s0, jobs.j[j].B, lane);
-    if (lane == 0) jobs.j[j].cnt[g] = nt;
+  if (t == 0) {
+    lb[ny2] = end;
+    s_s0 = run;  // = lb[0]
   }
-  if ((sx & 1) == 0) {  // the wave of an even sub-column audits its cell column
+  __syncthreads();
+  if (w == 0) {  // the sub-column's tile counts
+    const int s0 = s_s0;
+    for (int j = 0; j < njobs; ++j) {
+      const int nt = count_tiles(d, s0, end - s0, jobs.j[j].B, lane);
+      if (lane == 0) jobs.j[j].cnt[g] = nt;
+    }
+  }
+  if (w == 1 && (sx & 1) == 0) {  // the block of an even sub-column audits its cell column
     const int qc = cp.cell_off + (sx >> 1) * cp.ny, kc = 4 * cp.cell_off + sx * ny2;
     bool over = false;
     for (int k = lane; k < cp.ny; k += 64) {
       const int2 a = d.f_srange[kc + 2 * k], b = d.f_srange[kc + 2 * k + 1];
       const int2 e = d.f_srange[kc + ny2 + 2 * k], f = d.f_srange[kc + ny2 + 2 * k + 1];
-      const int2 so = d.s_range[qc + k], w = d.w_range[qc + k];
+      const int2 so = d.s_range[qc + k], wr = d.w_range[qc + k];
       const int n = max(a.y - a.x, 0) + max(b.y - b.x, 0) + max(e.y - e.x, 0) + max(f.y - f.x, 0) +
-                    (so.y - so.x) + (w.y - w.x);
+                    (so.y - so.x) + (wr.y - wr.x);
       over = over || n > 64;
     }
     if (__ballot(over ? 1 : 0) != 0ull && lane == 0) atomicOr(d.status, kStCellOver64);
   }
-  if ((sx & 1) == 0 && d.ns > 0) {  // the wave of an even sub-column also builds its column's f_swp
+  if (w == 2 && (sx & 1) == 0 && d.ns > 0) {  // ... and builds its column's f_swp
+    constexpr int kU = 8;
     const int qc = cp.cell_off + (sx >> 1) * cp.ny;
     int *swp = d.f_swp + cp.swp_off + (sx >> 1) * (cp.ny + 1);
     const int pc = (cp.ny + 63) >> 6, c0 = min(lane * pc, cp.ny), c1 = min(c0 + pc, cp.ny);

@@ -97,7 +97,6 @@ struct Dev {
   int *f_cnt;
   int *f_hkey, *f_hrank;
   int *f_perm, *f_porig, *f_pkey;
-  int *f_perm2, *f_orig2;
   R *f_col;
   int8_t *f_near;
   R *f_L;  // 4 per slot

@@ -169,7 +169,7 @@ __device__ __forceinline__ Mat2<R> mmul(const Mat2<R> &x, const Mat2<R> &y) {
 // of a counting sort (replacing thrust::sort_by_key, src/sort.cu:10-20): every fluid slot's
 // sub-cell key, (sy + sx * 2 ny) after the case's 4 ncell sub-cells offset, and its rank among
 // its key's slots.  Slots sorted last step hold equal keys in runs, so each run of equal keys
-// among a wave's consecutive lanes takes one atomic; k_bucket_order later fixes the order
+// among a wave's consecutive lanes takes one atomic; the permute later fixes the order
 // inside each key, which the atomics leave open.  Every lane of the block must reach the
 // ballots.
 template <class Pol>
@@ -267,23 +267,6 @@ __global__ void __launch_bounds__(256) k_count_scatter(Dev<Pol> d) {
   d.f_pkey[p] = k;
 }
 
-// Counting sort, step 5: inside each key, slots in original-index order -- a slot's place is
-// its key's start plus the number of the key's slots with a smaller original index -- so the
-// order is (sub-column, sub-row, original index) whatever order the atomics took.
-template <class Pol>
-__global__ void __launch_bounds__(256) k_bucket_order(Dev<Pol> d) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= d.nf) return;
-  const int k = d.f_pkey[t], o = d.f_porig[t];
-  const int2 r = d.f_srange[k];
-  int rank = 0;
-  for (int u = r.x; u < r.y; ++u) rank += d.f_porig[u] < o ? 1 : 0;
-  const int t2 = r.x + rank;
-  d.f_perm2[t2] = d.f_perm[t];
-  d.f_orig2[t2] = o;
-  d.f_key[t2] = k;
-}
-
 // fillSolidCellID (src/pair.cu:29-36); solids stay in original order, the key array is the
 // sort input directly.  Also refreshes the M32 cell-relative position view.
 template <class Pol>
@@ -321,20 +304,29 @@ __global__ void __launch_bounds__(256) k_hash_wall(Dev<Pol> d, const double *x, 
   if (fl) atomicOr(d.status, kStOutOfGrid);
 }
 
-// Gather the fluid state into sorted order (the zip permute of Fluid_List::sort,
-// src/sort.cu:10-20 / include/particle.cuh:93-96), rebuild the views, reset the per-case
-// maximum speed slots and the path lengths of this step.
+// The counting sort's last step fused with the gather of the fluid state into sorted order
+// (the zip permute of Fluid_List::sort, src/sort.cu:10-20 / include/particle.cuh:93-96): a
+// thread per scattered slot finds the slot's final place in its key and writes the particle
+// there; also rebuilds the views and resets the per-case maximum speed slots and the path
+// lengths of this step.
 template <class Pol>
 __global__ void __launch_bounds__(256) k_permute_fluid(Dev<Pol> d) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t < d.ncase * kSpread) d.umax_slot[(size_t)t * kSlotStride] = 0ull;
-  if (t < d.ncase) {
-    d.umax2[t] = 0ull;
-    d.umax[t] = 0;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < d.ncase * kSpread) d.umax_slot[(size_t)i * kSlotStride] = 0ull;
+  if (i < d.ncase) {
+    d.umax2[i] = 0ull;
+    d.umax[i] = 0;
   }
-  if (t >= d.nf) return;
-  const int o = d.f_orig2[t];
-  const int s = d.f_perm2[t];
+  if (i >= d.nf) return;
+  // counting sort, step 5: inside each key, slots in original-index order -- a slot's place is
+  // its key's start plus the number of the key's slots with a smaller original index -- so the
+  // order is (sub-column, sub-row, original index) whatever order the atomics took
+  const int k = d.f_pkey[i], o = d.f_porig[i], s = d.f_perm[i];
+  const int2 r = d.f_srange[k];
+  int rank = 0;
+  for (int u = r.x; u < r.y; ++u) rank += d.f_porig[u] < o ? 1 : 0;
+  const int t = r.x + rank;
+  d.f_key[t] = k;
   const double x = d.fa.x[s], y = d.fa.y[s], u = d.fa.u[s], v = d.fa.v[s], rho = d.fa.rho[s];
   d.fb.x[t] = x;
   d.fb.y[t] = y;

@@ -349,7 +349,7 @@ struct Engine : EngineBase {
     if ((rc = dalloc(&d.f_phase, nf)) || (rc = dalloc(&d.f_phase2, nf)) || (rc = dalloc(&d.f_cls, nf)) ||
         (rc = dalloc(&d.f_cls2, nf)) || (rc = dalloc(&d.f_orig, nf)) || (rc = dalloc(&d.f_cxy, nf)) ||
         (rc = dalloc(&d.f_hkey, nf)) || (rc = dalloc(&d.f_hrank, nf)) || (rc = dalloc(&d.f_perm, nf)) ||
-        (rc = dalloc(&d.f_perm2, nf)) || (rc = dalloc(&d.f_orig2, nf)) || (rc = dalloc(&d.f_key, nf)) ||
+        (rc = dalloc(&d.f_key, nf)) ||
         (rc = dalloc(&d.f_cnt, 4 * (size_t)nc + 1)) || (rc = dalloc(&d.f_porig, nf)) || (rc = dalloc(&d.f_pkey, nf)) ||
         (rc = dalloc(&cnt_tsum, (size_t)nblk(4 * nc, kScanTile) + 1)) ||
         (rc = dalloc(&cnt_toff, (size_t)nblk(4 * nc, kScanTile) + 1)) ||
@@ -585,7 +585,6 @@ struct Engine : EngineBase {
       k_exscan_small<<<1, 1024, 0, stream>>>(cnt_tsum, nkb, cnt_toff);
       k_count_apply<<<nkb, kSortThreads, 0, stream>>>(d.f_cnt, nk, cnt_toff, d.f_srange);
       k_count_scatter<Pol><<<nblk(nf, 256), 256, 0, stream>>>(d);
-      k_bucket_order<Pol><<<nblk(nf, 256), 256, 0, stream>>>(d);
       LAUNCH_CHECK();
       k_permute_fluid<Pol><<<nblk(std::max(nf, ncase * kSpread), blk_stream), blk_stream, 0, stream>>>(d);
       LAUNCH_CHECK();

@@ -787,38 +787,28 @@ __global__ void __launch_bounds__(256) k_tile_count(Dev<Pol> d, int G, const int
   if (lane == 0) cnt[g] = t;
 }
 
-// Exclusive scan of the per-sub-column tile counts (one block of 1024 threads per job); the
-// total goes to off[G].
-__global__ void __launch_bounds__(1024) k_tile_scan(PlanJobs jobs, int G) {
-  __shared__ int part[1024];
-  const int *cnt = jobs.j[blockIdx.x].cnt;
-  int *off = jobs.j[blockIdx.x].off;
-  const int t = threadIdx.x, per = (G + 1023) / 1024, a = min(t * per, G), b = min(a + per, G);
-  int s = 0;
-  for (int k = a; k < b; ++k) s += cnt[k];
-  part[t] = s;
-  __syncthreads();
-  for (int o = 1; o < 1024; o <<= 1) {  // Hillis-Steele inclusive scan
-    const int v = t >= o ? part[t - o] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
-  }
-  int run = part[t] - s;
-  for (int k = a; k < b; ++k) {
-    off[k] = run;
-    run += cnt[k];
-  }
-  if (t == 1023) off[G] = part[1023];
-}
-
 // Plans of the tiles of size B (one wave per global sub-column, a lane per chunk).
 template <class Pol>
 __global__ void __launch_bounds__(256) k_tile_plan(Dev<Pol> d, int G, const int *col_case, PlanJobs jobs) {
-  const int g = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int w = threadIdx.x >> 6, g0 = blockIdx.x * 4, g = g0 + w, lane = threadIdx.x & 63;
+  const int *cnt = jobs.j[blockIdx.y].cnt;
+  int *off = jobs.j[blockIdx.y].off;
+  // the sub-column's first tile: the block sums the counts of the sub-columns before its own
+  // four (a few thousand at most), each wave adds those of the block's earlier ones
+  __shared__ int s_part[4];
+  int before = 0;
+  for (int k = threadIdx.x; k < g0; k += 256) before += cnt[k];
+  before = wave_sum(before);
+  if (lane == 0) s_part[w] = before;
+  __syncthreads();
+  int tile = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+  for (int q = g0; q < g && q < G; ++q) tile += cnt[q];
   if (g >= G) return;
+  if (lane == 0) {
+    off[g] = tile;
+    if (g == G - 1) off[G] = tile + cnt[g];  // the tile count
+  }
   const int B = jobs.j[blockIdx.y].B;
-  const int *off = jobs.j[blockIdx.y].off;
   TilePlan *plans = jobs.j[blockIdx.y].plans;
   const int c = col_case[g];
   const auto &cp = d.cases[c];
@@ -828,7 +818,6 @@ __global__ void __launch_bounds__(256) k_tile_plan(Dev<Pol> d, int G, const int 
   const int sc0 = max(2 * cx - 2, 0), sc1 = min(2 * cx + 3, 2 * cp.nx - 1), nsc = sc1 - sc0 + 1;
   const int wc0 = max(cx - 1, 0), wc1 = min(cx + 1, cp.nx - 1);
   const int *swp = d.f_swp + cp.swp_off, *wlb = d.w_lb + cp.swp_off;
-  int tile = off[g];
   for (int k0 = 0; k0 < nch; k0 += 64) {
     const int k = k0 + lane, cstart = s0 + k * B, ccount = k < nch ? min(B, n - k * B) : 0;
     const int np = k < nch ? chunk_pieces(d.f_cxy, cstart, ccount, B) : 0;

@@ -634,7 +634,6 @@ struct Engine : EngineBase {
     const int nj = plan_jobs(B1, B2, jobs);
     if (nj == 0) return SPHNJ_OK;
     if (!counted) k_tile_count<Pol><<<dim3(nblk(G, 4), nj), 256, 0, stream>>>(d, G, col_case, jobs);
-    k_tile_scan<<<nj, 1024, 0, stream>>>(jobs, G);
     k_tile_plan<Pol><<<dim3(nblk(G, 4), nj), 256, 0, stream>>>(d, G, col_case, jobs);
     LAUNCH_CHECK();
     for (int k = 0; k < nj; ++k) plan_ok[bidx(jobs.j[k].B)] = true;

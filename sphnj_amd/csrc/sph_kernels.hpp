@@ -175,6 +175,8 @@ __device__ __forceinline__ Mat2<R> mmul(const Mat2<R> &x, const Mat2<R> &y) {
 template <class Pol>
 __global__ void __launch_bounds__(256) k_hash_fluid(Dev<Pol> d) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63;
+  if (d.ns > 0)  // the solids' cell ranges start each step empty (their sort writes the rest)
+    for (int q = s; q < d.total_cells; q += gridDim.x * blockDim.x) d.s_range[q] = make_int2(0, 0);
   const bool valid = s < d.nf;
   int key = -1 - lane;  // distinct for the lanes past the end
   if (valid) {

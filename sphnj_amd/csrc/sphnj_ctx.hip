@@ -410,6 +410,7 @@ struct Engine : EngineBase {
         (rc = dalloc(&d.s_cp, 2 * (size_t)ns)) || (rc = dalloc(&d.s_key, ns)) || (rc = dalloc(&d.s_cxy, ns)) ||
         (rc = dalloc(&d.s_range, nc)) || (rc = dalloc(&d.s_ref_off, ns + 1)))
       return rc;
+    HIPC(hipMemsetAsync(d.s_range, 0, (size_t)nc * sizeof(int2), stream));  // no solid: stays empty
     if constexpr (Pol::MIXED) {
       if ((rc = dalloc(&d.s_vx, ns)) || (rc = dalloc(&d.s_vy, ns)) || (rc = dalloc(&d.s_vx2, ns)) ||
           (rc = dalloc(&d.s_vy2, ns)))
@@ -573,11 +574,13 @@ struct Engine : EngineBase {
     const int nf = d.nf, ns = d.ns, nc = d.total_cells;
     // every species' cell ranges start empty (the permute writes the fluid's, k_cell_bounds the
     // solids')
-    // The solids' cell ranges start empty (k_cell_bounds writes the non-empty ones); so do the
-    // fluid's when there is no fluid (otherwise the counting sort writes every one of them).
-    k_clear_ranges<<<nblk(nf > 0 ? nc : 4 * nc, 256), 256, 0, stream>>>(d.s_range, nc, nf > 0 ? nullptr : d.f_srange,
-                                                                       nf > 0 ? 0 : 4 * nc);
-    LAUNCH_CHECK();
+    // The solids' cell ranges start empty (k_cell_bounds writes the non-empty ones): the fluid
+    // hash clears them; with no fluid, a clear launch does, with the fluid's ranges (otherwise
+    // the counting sort writes every one of those).
+    if (nf == 0) {
+      k_clear_ranges<<<nblk(4 * nc, 256), 256, 0, stream>>>(d.s_range, nc, d.f_srange, 4 * nc);
+      LAUNCH_CHECK();
+    }
     if (nf > 0) {  // counting sort on the sub-cell keys, then the permute
       const int nk = 4 * nc, nkb = nblk(nk, kScanTile);
       k_hash_fluid<Pol><<<nblk(nf, 256), 256, 0, stream>>>(d);

@@ -459,9 +459,9 @@ __global__ void __launch_bounds__(256) k_wall_normal(Dev<Pol> d) {
 // getVirtData + virtDataCalculate (src/virt.cu:61-149): Shepard-interpolated velocity and
 // hydrostatically corrected pressure from the fluid, slip / no-slip mirroring, phase of the
 // nearest fluid particle (first strict minimum in iteration order), density from the EOS.
-// Each wall particle has a group of kWallLanes lanes that split its fluid runs.
-constexpr int kWallLanes = 8;
-template <class Pol>
+// Each wall particle has a group of LANES lanes that split its fluid runs: 8 when the walls
+// are few (tens of thousands, which alone would not fill the chip), fewer as they grow.
+template <class Pol, int kWallLanes>
 __global__ void __launch_bounds__(256) k_wall_update(Dev<Pol> d) {
   using R = typename Pol::R;
   const int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -789,26 +789,77 @@ __global__ void __launch_bounds__(256) k_tile_count(Dev<Pol> d, int G, const int
   if (lane == 0) cnt[g] = t;
 }
 
+// Exclusive scan of the per-sub-column tile counts (one block of 1024 threads per job); the
+// total goes to off[G].  Used when there are more than kPlanPrefixMax sub-columns; with fewer,
+// k_tile_plan sums the counts before each of its blocks itself and this launch is skipped.
+constexpr int kPlanPrefixMax = 4096;
+__global__ void __launch_bounds__(1024) k_tile_scan(PlanJobs jobs, int G) {
+  // wave w scans the contiguous segment [w seg, (w + 1) seg) in chunks of 64 (coalesced, four
+  // chunks' loads in flight), after a block scan of the segments' totals
+  constexpr int kU = 4;
+  __shared__ int part[16];
+  const int *cnt = jobs.j[blockIdx.x].cnt;
+  int *off = jobs.j[blockIdx.x].off;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const int seg = (G + 15) / 16, a = min(w * seg, G), b = min(a + seg, G);
+  int s = 0;
+  for (int k = a + lane; k < b; k += 64 * kU) {
+    int v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) v[u] = k + 64 * u < b ? cnt[k + 64 * u] : 0;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) s += v[u];
+  }
+  s = wave_sum(s);
+  if (lane == 0) part[w] = s;
+  __syncthreads();
+  int run = 0, total = 0;
+#pragma unroll
+  for (int k = 0; k < 16; ++k) {
+    if (k < w) run += part[k];
+    total += part[k];
+  }
+  for (int k = a; k < b; k += 64 * kU) {
+    int v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) v[u] = k + 64 * u + lane < b ? cnt[k + 64 * u + lane] : 0;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int inc = wave_incl_sum(v[u], lane);
+      if (k + 64 * u + lane < b) off[k + 64 * u + lane] = run + inc - v[u];
+      run += __shfl(inc, 63, 64);
+    }
+  }
+  if (t == 0) off[G] = total;
+}
+
 // Plans of the tiles of size B (one wave per global sub-column, a lane per chunk).
 template <class Pol>
-__global__ void __launch_bounds__(256) k_tile_plan(Dev<Pol> d, int G, const int *col_case, PlanJobs jobs) {
+__global__ void __launch_bounds__(256) k_tile_plan(Dev<Pol> d, int G, const int *col_case, PlanJobs jobs,
+                                                   int scanned) {
   const int w = threadIdx.x >> 6, g0 = blockIdx.x * 4, g = g0 + w, lane = threadIdx.x & 63;
   const int *cnt = jobs.j[blockIdx.y].cnt;
   int *off = jobs.j[blockIdx.y].off;
-  // the sub-column's first tile: the block sums the counts of the sub-columns before its own
-  // four (a few thousand at most), each wave adds those of the block's earlier ones
-  __shared__ int s_part[4];
-  int before = 0;
-  for (int k = threadIdx.x; k < g0; k += 256) before += cnt[k];
-  before = wave_sum(before);
-  if (lane == 0) s_part[w] = before;
-  __syncthreads();
-  int tile = s_part[0] + s_part[1] + s_part[2] + s_part[3];
-  for (int q = g0; q < g && q < G; ++q) tile += cnt[q];
-  if (g >= G) return;
-  if (lane == 0) {
-    off[g] = tile;
-    if (g == G - 1) off[G] = tile + cnt[g];  // the tile count
+  int tile;
+  if (scanned) {  // off holds the scan of the counts (many sub-columns)
+    if (g >= G) return;
+    tile = off[g];
+  } else {
+    // the sub-column's first tile: the block sums the counts of the sub-columns before its own
+    // four (at most kPlanPrefixMax), each wave adds those of the block's earlier ones
+    __shared__ int s_part[4];
+    int before = 0;
+    for (int k = threadIdx.x; k < g0; k += 256) before += cnt[k];
+    before = wave_sum(before);
+    if (lane == 0) s_part[w] = before;
+    __syncthreads();
+    tile = s_part[0] + s_part[1] + s_part[2] + s_part[3];
+    for (int q = g0; q < g && q < G; ++q) tile += cnt[q];
+    if (g >= G) return;
+    if (lane == 0) {
+      off[g] = tile;
+      if (g == G - 1) off[G] = tile + cnt[g];  // the tile count
+    }
   }
   const int B = jobs.j[blockIdx.y].B;
   TilePlan *plans = jobs.j[blockIdx.y].plans;

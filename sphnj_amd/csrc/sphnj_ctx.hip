@@ -637,7 +637,9 @@ struct Engine : EngineBase {
     const int nj = plan_jobs(B1, B2, jobs);
     if (nj == 0) return SPHNJ_OK;
     if (!counted) k_tile_count<Pol><<<dim3(nblk(G, 4), nj), 256, 0, stream>>>(d, G, col_case, jobs);
-    k_tile_plan<Pol><<<dim3(nblk(G, 4), nj), 256, 0, stream>>>(d, G, col_case, jobs);
+    const int scanned = G > kPlanPrefixMax;
+    if (scanned) k_tile_scan<<<nj, 1024, 0, stream>>>(jobs, G);
+    k_tile_plan<Pol><<<dim3(nblk(G, 4), nj), 256, 0, stream>>>(d, G, col_case, jobs, scanned);
     LAUNCH_CHECK();
     for (int k = 0; k < nj; ++k) plan_ok[bidx(jobs.j[k].B)] = true;
     return SPHNJ_OK;
@@ -764,7 +766,15 @@ struct Engine : EngineBase {
         return SPHNJ_OK;
       case SPHNJ_STAGE_WALL:
         if (nw > 0) {
-          k_wall_update<Pol><<<nblk(nw * kWallLanes, 256), 256, 0, stream>>>(d);
+          // lanes per wall particle: about half a million threads in all, 1 to 8 per particle
+          if (nw <= (1 << 16))
+            k_wall_update<Pol, 8><<<nblk(nw * 8, 256), 256, 0, stream>>>(d);
+          else if (nw <= (1 << 17))
+            k_wall_update<Pol, 4><<<nblk(nw * 4, 256), 256, 0, stream>>>(d);
+          else if (nw <= (1 << 18))
+            k_wall_update<Pol, 2><<<nblk(nw * 2, 256), 256, 0, stream>>>(d);
+          else
+            k_wall_update<Pol, 1><<<nblk(nw, 256), 256, 0, stream>>>(d);
           LAUNCH_CHECK();
         }
         return SPHNJ_OK;

@@ -656,19 +656,21 @@ __global__ void __launch_bounds__(256) k_wall_rows(Dev<Pol> d, int G, const int 
   lb[cp.ny] = end;
 }
 
-// Row table of every sub-column (one wave per global sub-column): f_lb[k] for row k = the
-// suffix minimum of the non-empty rows' starts, with the sub-column's end past the last
-// non-empty row; and the sub-column's tile counts for the njobs block sizes.  The waves of
+// Row table of every sub-column (one block of NW waves per global sub-column): f_lb[k] for row
+// k = the suffix minimum of the non-empty rows' starts, with the sub-column's end past the last
+// non-empty row; and the sub-column's tile counts for the njobs block sizes.  The blocks of
 // even sub-columns also audit their cell column's cells against the reference's 64-entry cell
-// capacity (include/global.h:9) and build its f_swp (when the batch has solids).
-template <class Pol>
-__global__ void __launch_bounds__(256) k_subcol_rows(Dev<Pol> d, int G, const int *col_case, PlanJobs jobs,
-                                                     int njobs) {
-  // one block per sub-column: the four waves build its row table together, then wave 0
-  // counts its tiles while waves 1 and 2 audit and index its cell column
+// capacity (include/global.h:9) and build its f_swp (when the batch has solids).  With four
+// waves (a few thousand sub-columns) the waves build the row table together, then wave 0 counts
+// the tiles while waves 1 and 2 audit and index the cell column; with one (many small
+// sub-columns, a batch of cases) the wave does all of it.
+template <class Pol, int NW>
+__global__ void __launch_bounds__(64 * NW) k_subcol_rows(Dev<Pol> d, int G, const int *col_case, PlanJobs jobs,
+                                                         int njobs) {
+  constexpr int NT = 64 * NW;
   const int g = blockIdx.x, t = threadIdx.x, w = t >> 6, lane = t & 63;
   if (g >= G) return;
-  __shared__ int s_min[4], s_end[4], s_s0;
+  __shared__ int s_min[NW], s_end[NW], s_s0;
   const auto &cp = d.cases[col_case[g]];
   const int sx = g - cp.col_off, ny2 = 2 * cp.ny;
   const int kb = 4 * cp.cell_off + sx * ny2;
@@ -676,7 +678,7 @@ __global__ void __launch_bounds__(256) k_subcol_rows(Dev<Pol> d, int G, const in
   // each thread takes a contiguous run of rows [k0, k1), held in registers when it has at most
   // kR of them (up to 2048 sub-rows), else read again in the backward pass
   constexpr int kR = 8;
-  const int per = (ny2 + 255) >> 8, k0 = min(t * per, ny2), k1 = min(k0 + per, ny2);
+  const int per = (ny2 + NT - 1) / NT, k0 = min(t * per, ny2), k1 = min(k0 + per, ny2);
   int lmin = 0x7fffffff, end = 0;
   int2 r[kR];
   for (int ka = k0; ka < k1; ka += kR) {
@@ -699,10 +701,11 @@ __global__ void __launch_bounds__(256) k_subcol_rows(Dev<Pol> d, int G, const in
     s_end[w] = end;
   }
   __syncthreads();
-  end = max(max(s_end[0], s_end[1]), max(s_end[2], s_end[3]));
+#pragma unroll
+  for (int k = 0; k < NW; ++k) end = max(end, s_end[k]);
   int run = __shfl_down(lmin, 1, 64);
   if (lane == 63) run = 0x7fffffff;
-  for (int k = w + 1; k < 4; ++k) run = min(run, s_min[k]);
+  for (int k = w + 1; k < NW; ++k) run = min(run, s_min[k]);
   run = min(run, end);  // rows past the last non-empty one: the end
   if (per <= kR) {
 #pragma unroll
@@ -730,7 +733,7 @@ __global__ void __launch_bounds__(256) k_subcol_rows(Dev<Pol> d, int G, const in
       if (lane == 0) jobs.j[j].cnt[g] = nt;
     }
   }
-  if (w == 1 && (sx & 1) == 0) {  // the block of an even sub-column audits its cell column
+  if (w == 1 % NW && (sx & 1) == 0) {  // the block of an even sub-column audits its cell column
     const int qc = cp.cell_off + (sx >> 1) * cp.ny, kc = 4 * cp.cell_off + sx * ny2;
     bool over = false;
     for (int k = lane; k < cp.ny; k += 64) {
@@ -743,7 +746,7 @@ __global__ void __launch_bounds__(256) k_subcol_rows(Dev<Pol> d, int G, const in
     }
     if (__ballot(over ? 1 : 0) != 0ull && lane == 0) atomicOr(d.status, kStCellOver64);
   }
-  if (w == 2 && (sx & 1) == 0 && d.ns > 0) {  // ... and builds its column's f_swp
+  if (w == 2 % NW && (sx & 1) == 0 && d.ns > 0) {  // ... and builds its column's f_swp
     constexpr int kU = 8;
     const int qc = cp.cell_off + (sx >> 1) * cp.ny;
     int *swp = d.f_swp + cp.swp_off + (sx >> 1) * (cp.ny + 1);

@@ -606,7 +606,10 @@ struct Engine : EngineBase {
     if (nf > 0) {  // row tables, the cell audit and the tile counts of both block sizes
       PlanJobs jobs{};
       const int nj = plan_jobs(blk_pair, blk_shift, jobs);
-      k_subcol_rows<Pol><<<G, 256, 0, stream>>>(d, G, col_case, jobs, nj);
+      if (G <= kPlanPrefixMax)
+        k_subcol_rows<Pol, 4><<<G, 256, 0, stream>>>(d, G, col_case, jobs, nj);
+      else
+        k_subcol_rows<Pol, 1><<<G, 64, 0, stream>>>(d, G, col_case, jobs, nj);
       LAUNCH_CHECK();
       int rc;
       if ((rc = ensure_plans(blk_pair, blk_shift, true))) return rc;

@@ -1143,6 +1143,26 @@ __device__ __forceinline__ void for_range_masked(int lo, int hi, F &&f) {
   }
 }
 
+// Calls f(s, true) for s in [lo, hi): four at a time, then the lane's last 0 to 3 by a pair and
+// a single under branches, so no call is masked (a wave runs the pair if any lane has two left,
+// the single if any has one).
+template <class F>
+__device__ __forceinline__ void for_range_4_2_1(int lo, int hi, F &&f) {
+  int s = lo;
+  for (const int last = hi - 3; s < last; s += 4) {
+    f(s, true);
+    f(s + 1, true);
+    f(s + 2, true);
+    f(s + 3, true);
+  }
+  if (s + 1 < hi) {
+    f(s, true);
+    f(s + 1, true);
+    s += 2;
+  }
+  if (s < hi) f(s, true);
+}
+
 // Per-case constants a neighbour loop needs, held in registers.
 template <class R>
 struct LoopConsts {
@@ -1732,7 +1752,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
   for_windows<false>(P, cp, sy, cy, lp, Rp, [&](int sc, int ra, int rb) {
     if (staged) {
       const int sa = sc - P.sc0;
-      for_range_masked<4>(s_lb[sa][ra - P.r0], s_lb[sa][rb + 1 - P.r0], staged_add);
+      for_range_4_2_1(s_lb[sa][ra - P.r0], s_lb[sa][rb + 1 - P.r0], staged_add);
     } else {  // global views, relative to their own cell: one run per cell row
       const R bx = xi - cell_shift_x<Pol>((sc >> 1) - cx, cp);
       for (int cr = ra >> 1; cr <= (rb >> 1); ++cr) {
@@ -1746,7 +1766,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
   if (wstaged) {
     const int lr0 = max(cy - 1, 0) - W.wcr0, lr1 = min(cy + 1, cp.ny - 1) - W.wcr0;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) for_range_masked<4>(s_wlb[a][lr0], s_wlb[a][lr1 + 1], staged_add);
+    for (int a = 0; a < 3; ++a) for_range_4_2_1(s_wlb[a][lr0], s_wlb[a][lr1 + 1], staged_add);
   }
   for (int a = -1; a <= 1 && P.sw; ++a) {
     const int jx = cx + a;

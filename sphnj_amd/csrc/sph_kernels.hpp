@@ -1423,11 +1423,10 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
           return 0u;
         }
         unsigned m = 0u;
-        for_range_masked<4>(s0, s1, [&](int l, bool in) {
+        for_range_4_2_1(s0, s1, [&](int l, bool) {
           const R2 pj = s_pos[l];
           const R dx = pj.x - xi, dy = pj.y - yg;
-          const bool ok = in & (dx * dx + dy * dy < K.rc2);
-          m |= (ok ? 1u : 0u) << ((l - s0) & 31);
+          m |= (dx * dx + dy * dy < K.rc2 ? 1u : 0u) << ((l - s0) & 31);
         });
         return m;
       };

@@ -1209,15 +1209,18 @@ __device__ __forceinline__ void riemann_pair(R dx, R dy, R r2, R w, R ui, R vi, 
   const R rn = r2 * inv_r;  // |dx|
   const R du_ = ui - uj, dv_ = vi - vj;
   const R q = du_ * dx + dv_ * dy;
+  // the density weights rho_i / (rho_i + rho_j) and rho_j / (rho_i + rho_j); the factors that
+  // depend on particle i alone (-2 / rho_i, 2 rho_i, p_i / c) are formed once per lane
   const R isum = krcp<R>(rhoi + rhoj);
-  R ps = (rhoi * pj + rhoj * pi) * isum;
+  const R ai = rhoi * isum, aj = rhoj * isum;
+  R ps = ai * pj + aj * pi;
   const R du = q * inv_r;
-  if (du >= R(0)) ps += rhoi * rhoj * du * isum * fmin(R(3) * du, cs);
-  const R t = (pi - pj) * isum * csinv;
-  const R s = R(-2) * ps * rhoi_inv * w;
+  if (du >= R(0)) ps += (rhoi * aj) * du * fmin(R(3) * du, cs);
+  const R t = (pi * csinv - pj * csinv) * isum;
+  const R s = ps * ((R(-2) * rhoi_inv) * w);
   A.ax += s * dx;
   A.ay += s * dy;
-  A.drho += R(2) * rhoi * w * (rhoj * isum * q - t * rn);
+  A.drho += ((R(2) * rhoi) * w) * (aj * q - t * rn);
   const R wdx = w * dx, wdy = w * dy;
   A.L00 -= du_ * wdx;
   A.L01 -= du_ * wdy;

@@ -73,6 +73,21 @@ def measured_traffic(config, stage, block):
     return None
 
 
+def measured_issue(config, stage, block):
+    """Vector-ALU instructions per CU-cycle of the stage's kernel at the block size in use, from
+    the committed rocprofv3 SQ_INSTS_VALU / GRBM_GUI_ACTIVE pass (profiles/pmc_valu_<workload>.json;
+    a CU issues at most about one wave64 VALU instruction per cycle); None where not measured."""
+    kern = {"shift": f"k_fluid_shift<sphnj::M32, {block}>",
+            "fluid": f"k_fluid_pair_update<sphnj::M32, false, {block}>", "cells": None}[stage]
+    f = ROOT / "profiles" / {"two_phase": "pmc_valu_two_phase_3M.json"}.get(config, "-")
+    if kern is None or not f.exists():
+        return None
+    for k, v in json.loads(f.read_text())["kernels"].items():
+        if kern in k:
+            return v["valu_per_cu_cycle"]
+    return None
+
+
 def case_probes(config, case, a, b):
     """The configurations' own error figures, M32 (a) against P64 (b) at the compared step: the
     elastic gate's tip displacement (mean horizontal displacement of the free bottom row of the
@@ -263,6 +278,7 @@ def main() -> int:
     roofline = {"bound": "hbm", "kernel": kernel_name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
                 "traffic": measured_traffic(args.config, dom, blocks[dom]),
+                "valu_per_cu_cycle": measured_issue(args.config, dom, blocks[dom]),
                 "bytes_per_particle": BYTES[dom]["m32"], "ms_per_launch": round(per_launch_ms, 5)}
 
     cpu = None

@@ -111,6 +111,10 @@ struct Dev {
   // max |u|^2 per case as the bit pattern of a non-negative double: blocks reduce into
   // kSpread slots per case (slot s at umax_slot[(c kSpread + s) kSlotStride], one cache line
   // each, so no address takes more than a few hundred atomics); umax2[c] is their maximum
+  // non-zero when a solid's cell key changed since the last sort of the solids (set by the
+  // solid hash, cleared after the step's cell stage): the solids' sort, cell ranges and their
+  // clearing run only then (a body moves by far less than a cell per step)
+  int *s_moved;
   unsigned long long *umax_slot, *umax2;
   R *umax;  // per case: min(sqrt(umax2), umax_cap), the speed the shift scales with
   // solid (original order, never permuted)

@@ -174,7 +174,9 @@ constexpr int kSmallSortN = kSmallSortThreads * kSmallSortItems;
 
 __global__ void __launch_bounds__(kSmallSortThreads) radix_sort_small(const int *__restrict__ kin,
                                                                       int *__restrict__ kout,
-                                                                      int *__restrict__ vout, int n, int passes) {
+                                                                      int *__restrict__ vout, int n, int passes,
+                                                                      const int *only_if) {
+  if (only_if != nullptr && *only_if == 0) return;  // the outputs of the last sort still hold
   constexpr int kWaves = kSmallSortThreads / 64;
   __shared__ int s_key[kSmallSortN];
   __shared__ int s_val[kSmallSortN];
@@ -301,16 +303,17 @@ inline int radix_passes_for(int max_key) {
 
 // Sort (keys[i], i) for i < n.  keys is overwritten on even pass counts of the multi-block
 // path.  On return *out_keys / *out_vals point at the sorted keys and values (inside keys / the
-// sorter's buffers).
+// sorter's buffers).  With only_if (n <= kSmallSortN only), the sort runs when *only_if is
+// non-zero at that point of the stream, and otherwise leaves the last sort's outputs in place.
 inline hipError_t radix_sort_pairs(const RadixSorter &S, int *keys, int **out_keys, int **out_vals,
-                                   hipStream_t st) {
+                                   hipStream_t st, const int *only_if = nullptr) {
   if (S.n == 0) {
     *out_keys = keys;
     *out_vals = S.v0;
     return hipSuccess;
   }
   if (S.n <= kSmallSortN) {
-    radix_sort_small<<<1, kSmallSortThreads, 0, st>>>(keys, S.k1, S.v0, S.n, S.passes);
+    radix_sort_small<<<1, kSmallSortThreads, 0, st>>>(keys, S.k1, S.v0, S.n, S.passes, only_if);
     *out_keys = S.k1;
     *out_vals = S.v0;
     return hipGetLastError();

@@ -173,9 +173,11 @@ __device__ __forceinline__ Mat2<R> mmul(const Mat2<R> &x, const Mat2<R> &y) {
 // inside each key, which the atomics leave open.  Every lane of the block must reach the
 // ballots.
 template <class Pol>
-__global__ void __launch_bounds__(256) k_hash_fluid(Dev<Pol> d) {
+__global__ void __launch_bounds__(256) k_hash_fluid(Dev<Pol> d, const int *solids_moved) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63;
-  if (d.ns > 0)  // the solids' cell ranges start each step empty (their sort writes the rest)
+  // the solids' cell ranges start empty when the solids are sorted again (their bounds write the
+  // rest); otherwise they stay as they are
+  if (d.ns > 0 && (solids_moved == nullptr || *solids_moved != 0))
     for (int q = s; q < d.total_cells; q += gridDim.x * blockDim.x) d.s_range[q] = make_int2(0, 0);
   const bool valid = s < d.nf;
   int key = -1 - lane;  // distinct for the lanes past the end
@@ -273,7 +275,7 @@ __global__ void __launch_bounds__(256) k_count_scatter(Dev<Pol> d) {
 // sort input directly.  Also refreshes the M32 cell-relative position view.
 template <class Pol>
 __global__ void __launch_bounds__(256) k_hash_solid(Dev<Pol> d, const double *x, const double *y, int *keys,
-                                                    int *cxy_out, int update_view) {
+                                                    int *cxy_out, int update_view, int *moved) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.ns) return;
   const int c = case_of(i, d.s_case_off, d.ncase);
@@ -281,7 +283,12 @@ __global__ void __launch_bounds__(256) k_hash_solid(Dev<Pol> d, const double *x,
   unsigned fl = 0;
   const int cx = axis_cell(x[i], cp.cxmin, cp.inv_cdx, cp.nx, &fl);
   const int cy = axis_cell(y[i], cp.cymin, cp.inv_cdy, cp.ny, &fl);
-  keys[i] = cp.cell_off + cy + cx * cp.ny;
+  const int key = cp.cell_off + cy + cx * cp.ny;
+  if (moved) {  // keys holds last step's keys: one atomic per wave with a change
+    const unsigned long long ch = __ballot(keys[i] != key ? 1 : 0);
+    if (ch != 0ull && (int)(threadIdx.x & 63) == __builtin_ctzll(__ballot(1))) atomicOr(moved, 1);
+  }
+  keys[i] = key;
   if (cxy_out) cxy_out[i] = cxy_pack(cx, cy);
   if (update_view) {
     if constexpr (Pol::MIXED) {
@@ -386,9 +393,9 @@ __global__ void __launch_bounds__(256) k_clear_ranges(int2 *r0, int n0, int2 *r1
 
 // Cell [start, end) from the sorted keys (replaces reduce_by_key + exclusive_scan +
 // updateCell*, src/sort.cu:16-19, 59-137).  Empty cells keep [0, 0).
-__global__ void __launch_bounds__(256) k_cell_bounds(const int *keys, int n, int2 *range) {
+__global__ void __launch_bounds__(256) k_cell_bounds(const int *keys, int n, int2 *range, const int *only_if) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || (only_if != nullptr && *only_if == 0)) return;
   const int k = keys[i];
   if (i == 0 || keys[i - 1] != k) range[k].x = i;
   if (i == n - 1 || keys[i + 1] != k) range[k].y = i + 1;
@@ -666,10 +673,11 @@ __global__ void __launch_bounds__(256) k_wall_rows(Dev<Pol> d, int G, const int 
 // sub-columns, a batch of cases) the wave does all of it.
 template <class Pol, int NW>
 __global__ void __launch_bounds__(64 * NW) k_subcol_rows(Dev<Pol> d, int G, const int *col_case, PlanJobs jobs,
-                                                         int njobs) {
+                                                         int njobs, int *reset_moved) {
   constexpr int NT = 64 * NW;
   const int g = blockIdx.x, t = threadIdx.x, w = t >> 6, lane = t & 63;
   if (g >= G) return;
+  if (reset_moved != nullptr && g == 0 && t == 0) *reset_moved = 0;  // read by the earlier kernels only
   __shared__ int s_min[NW], s_end[NW], s_s0;
   const auto &cp = d.cases[col_case[g]];
   const int sx = g - cp.col_off, ny2 = 2 * cp.ny;

@@ -357,6 +357,7 @@ struct Engine : EngineBase {
         (rc = dalloc(&d.f_sacc, 2 * (size_t)nf)) ||
         (rc = dalloc(&d.f_srange, 4 * (size_t)nc)) || (rc = dalloc(&d.umax2, ncase)) || (rc = dalloc(&d.umax, ncase)) ||
         (rc = dalloc(&d.umax_slot, (size_t)ncase * kSpread * kSlotStride)) || (rc = dalloc(&d.status, 1)) ||
+        (rc = dalloc(&d.s_moved, 1)) ||
         (rc = dalloc(&d.f_lb, (size_t)lb_total)) || (rc = dalloc(&d.f_swp, (size_t)swp_total)) ||
         (rc = dalloc(&col_case, G)))
       return rc;
@@ -411,6 +412,9 @@ struct Engine : EngineBase {
         (rc = dalloc(&d.s_range, nc)) || (rc = dalloc(&d.s_ref_off, ns + 1)))
       return rc;
     HIPC(hipMemsetAsync(d.s_range, 0, (size_t)nc * sizeof(int2), stream));  // no solid: stays empty
+    // the first cell stage sorts the solids: every key differs from -1, and the flag starts raised
+    HIPC(hipMemsetAsync(d.s_key, 0xff, std::max(ns, 1) * sizeof(int), stream));
+    HIPC(hipMemsetAsync(d.s_moved, 1, sizeof(int), stream));
     if constexpr (Pol::MIXED) {
       if ((rc = dalloc(&d.s_vx, ns)) || (rc = dalloc(&d.s_vy, ns)) || (rc = dalloc(&d.s_vx2, ns)) ||
           (rc = dalloc(&d.s_vy2, ns)))
@@ -505,7 +509,7 @@ struct Engine : EngineBase {
         k_permute_wall<Pol><<<nblk(nw, 256), 256, 0, stream>>>(d, sv, x0, y0, bc0);
         LAUNCH_CHECK();
         k_clear_ranges<<<nblk(nc, 256), 256, 0, stream>>>(d.w_range, nc, nullptr, 0);
-        k_cell_bounds<<<nblk(nw, 256), 256, 0, stream>>>(sk, nw, d.w_range);
+        k_cell_bounds<<<nblk(nw, 256), 256, 0, stream>>>(sk, nw, d.w_range, nullptr);
         k_wall_normal<Pol><<<nblk(nw, 256), 256, 0, stream>>>(d);
         LAUNCH_CHECK();
       } else {
@@ -537,12 +541,12 @@ struct Engine : EngineBase {
         }
         if ((rc = h2d(kdc, hk.data(), ncase)) || (rc = h2d(hinv2, hh.data(), ncase)) || (rc = h2d(rc2, hr.data(), ncase)))
           return rc;
-        k_hash_solid<Pol><<<nblk(ns, 256), 256, 0, stream>>>(d, d.sX, d.sY, keys, cxy, 0);
+        k_hash_solid<Pol><<<nblk(ns, 256), 256, 0, stream>>>(d, d.sX, d.sY, keys, cxy, 0, nullptr);
         LAUNCH_CHECK();
         int *sk, *sv;
         HIPC(radix_sort_pairs(rs, keys, &sk, &sv, stream));
         k_clear_ranges<<<nblk(nc, 256), 256, 0, stream>>>(range, nc, nullptr, 0);
-        k_cell_bounds<<<nblk(ns, 256), 256, 0, stream>>>(sk, ns, range);
+        k_cell_bounds<<<nblk(ns, 256), 256, 0, stream>>>(sk, ns, range, nullptr);
         k_ref_count<Pol><<<nblk(ns, 256), 256, 0, stream>>>(d, cxy, range, sv, rc2, cnt);
         LAUNCH_CHECK();
         std::vector<int> hcnt(ns);
@@ -577,13 +581,22 @@ struct Engine : EngineBase {
     // The solids' cell ranges start empty (k_cell_bounds writes the non-empty ones): the fluid
     // hash clears them; with no fluid, a clear launch does, with the fluid's ranges (otherwise
     // the counting sort writes every one of those).
+    // With fluid and at most kSmallSortN solids, the solids are sorted again (and their ranges
+    // cleared and rebuilt) only when a solid's cell changed: the solid hash runs first and
+    // raises d.s_moved, which k_subcol_rows lowers at the end of the stage.
+    const bool track = nf > 0 && ns > 0 && ns <= kSmallSortN;
+    int *moved = track ? d.s_moved : nullptr;
+    if (ns > 0) {
+      k_hash_solid<Pol><<<nblk(ns, blk_stream), blk_stream, 0, stream>>>(d, d.sx, d.sy, d.s_key, d.s_cxy, 1, moved);
+      LAUNCH_CHECK();
+    }
     if (nf == 0) {
       k_clear_ranges<<<nblk(4 * nc, 256), 256, 0, stream>>>(d.s_range, nc, d.f_srange, 4 * nc);
       LAUNCH_CHECK();
     }
     if (nf > 0) {  // counting sort on the sub-cell keys, then the permute
       const int nk = 4 * nc, nkb = nblk(nk, kScanTile);
-      k_hash_fluid<Pol><<<nblk(nf, 256), 256, 0, stream>>>(d);
+      k_hash_fluid<Pol><<<nblk(nf, 256), 256, 0, stream>>>(d, moved);
       k_count_tiles<<<nkb, kSortThreads, 0, stream>>>(d.f_cnt, nk, cnt_tsum);
       k_exscan_small<<<1, 1024, 0, stream>>>(cnt_tsum, nkb, cnt_toff);
       k_count_apply<<<nkb, kSortThreads, 0, stream>>>(d.f_cnt, nk, cnt_toff, d.f_srange);
@@ -596,20 +609,18 @@ struct Engine : EngineBase {
       std::swap(d.f_cls, d.f_cls2);
     }
     if (ns > 0) {
-      k_hash_solid<Pol><<<nblk(ns, blk_stream), blk_stream, 0, stream>>>(d, d.sx, d.sy, d.s_key, d.s_cxy, 1);
-      LAUNCH_CHECK();
-      HIPC(radix_sort_pairs(ssort, d.s_key, &s_sorted_keys, &d.s_sorted, stream));
+      HIPC(radix_sort_pairs(ssort, d.s_key, &s_sorted_keys, &d.s_sorted, stream, moved));
+      k_cell_bounds<<<nblk(ns, 256), 256, 0, stream>>>(s_sorted_keys, ns, d.s_range, moved);
     }
-    if (ns > 0) k_cell_bounds<<<nblk(ns, 256), 256, 0, stream>>>(s_sorted_keys, ns, d.s_range);
     move_epoch = 0;
     plan_ok[0] = plan_ok[1] = plan_ok[2] = false;
     if (nf > 0) {  // row tables, the cell audit and the tile counts of both block sizes
       PlanJobs jobs{};
       const int nj = plan_jobs(blk_pair, blk_shift, jobs);
       if (G <= kPlanPrefixMax)
-        k_subcol_rows<Pol, 4><<<G, 256, 0, stream>>>(d, G, col_case, jobs, nj);
+        k_subcol_rows<Pol, 4><<<G, 256, 0, stream>>>(d, G, col_case, jobs, nj, moved);
       else
-        k_subcol_rows<Pol, 1><<<G, 64, 0, stream>>>(d, G, col_case, jobs, nj);
+        k_subcol_rows<Pol, 1><<<G, 64, 0, stream>>>(d, G, col_case, jobs, nj, moved);
       LAUNCH_CHECK();
       int rc;
       if ((rc = ensure_plans(blk_pair, blk_shift, true))) return rc;

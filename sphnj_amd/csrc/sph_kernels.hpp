@@ -469,9 +469,9 @@ __global__ void __launch_bounds__(256) k_wall_normal(Dev<Pol> d) {
 // Each wall particle has a group of LANES lanes that split its fluid runs: 8 when the walls
 // are few (tens of thousands, which alone would not fill the chip), fewer as they grow.
 template <class Pol, int kWallLanes>
-__global__ void __launch_bounds__(256) k_wall_update(Dev<Pol> d) {
+__device__ __forceinline__ void wall_update_block(const Dev<Pol> &d, int bid) {
   using R = typename Pol::R;
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gid = bid * 256 + (int)threadIdx.x;
   const int i = gid / kWallLanes, l = gid % kWallLanes;
   const bool active = i < d.nw;
   R S = 0, U = 0, V = 0, P = 0, nearest = 0;
@@ -545,6 +545,10 @@ __global__ void __launch_bounds__(256) k_wall_update(Dev<Pol> d) {
   d.w_phase[i] = (int8_t)phase;
   // pressureToDensity (src/fluid.cu:6-13)
   d.w_rho[i] = (R)(cp.rho0_d[phase] * pow(((double)p - cp.p_bg_d) * cp.p2rho_d[phase] + 1.0, cp.ginv_d[phase]));
+}
+template <class Pol, int kWallLanes>
+__global__ void __launch_bounds__(256) k_wall_update(Dev<Pol> d) {
+  wall_update_block<Pol, kWallLanes>(d, (int)blockIdx.x);
 }
 
 // ======================================================================================
@@ -1904,9 +1908,9 @@ __device__ __forceinline__ void for_refs(const int *ref_j, int e0, int e1, Ld &&
 // solid neighbours in the current cells; and the check that every reference-list neighbour
 // still lies within the 3x3 current cells.
 template <class Pol>
-__global__ void __launch_bounds__(256) k_solid_couple(Dev<Pol> d) {
+__device__ __forceinline__ void solid_couple_block(const Dev<Pol> &d, int bid) {
   using R = typename Pol::R;
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  const int gid = bid * 256 + (int)threadIdx.x;
   const int i = gid / kSolidLanes, l = gid % kSolidLanes;
   const bool active = i < d.ns;
   R nx = 0, ny = 0, P = 0, S = 0;
@@ -1972,6 +1976,19 @@ __global__ void __launch_bounds__(256) k_solid_couple(Dev<Pol> d) {
   }
   d.s_p[i] = S != R(0) ? fmax(P / S, cp.p_min) : R(0);
   if (left) atomicOr(d.status, kStRefListLeft);
+}
+template <class Pol>
+__global__ void __launch_bounds__(256) k_solid_couple(Dev<Pol> d) {
+  solid_couple_block<Pol>(d, (int)blockIdx.x);
+}
+// The solid coupling and the wall update in one launch (they read the fluid only): blocks below
+// wall_blocks update walls, the rest couple solids.
+template <class Pol, int kWallLanes>
+__global__ void __launch_bounds__(256) k_couple_wall(Dev<Pol> d, int wall_blocks) {
+  if ((int)blockIdx.x < wall_blocks)
+    wall_update_block<Pol, kWallLanes>(d, (int)blockIdx.x);
+  else
+    solid_couple_block<Pol>(d, (int)blockIdx.x - wall_blocks);
 }
 
 // plasticElasticFunction (src/solid.cu:80-108)

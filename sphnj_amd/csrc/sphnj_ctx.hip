@@ -780,15 +780,7 @@ struct Engine : EngineBase {
         return SPHNJ_OK;
       case SPHNJ_STAGE_WALL:
         if (nw > 0) {
-          // lanes per wall particle: about half a million threads in all, 1 to 8 per particle
-          if (nw <= (1 << 16))
-            k_wall_update<Pol, 8><<<nblk(nw * 8, 256), 256, 0, stream>>>(d);
-          else if (nw <= (1 << 17))
-            k_wall_update<Pol, 4><<<nblk(nw * 4, 256), 256, 0, stream>>>(d);
-          else if (nw <= (1 << 18))
-            k_wall_update<Pol, 2><<<nblk(nw * 2, 256), 256, 0, stream>>>(d);
-          else
-            k_wall_update<Pol, 1><<<nblk(nw, 256), 256, 0, stream>>>(d);
+          launch_wall(false);
           LAUNCH_CHECK();
         }
         return SPHNJ_OK;
@@ -847,8 +839,37 @@ struct Engine : EngineBase {
     return 1;
   }
 
+  // The wall update (lanes per wall particle: about half a million threads in all, 1 to 8 per
+  // particle); with_couple puts the solid coupling's blocks after the walls' in the same launch.
+  template <int WL>
+  void launch_wall_at(bool with_couple) {
+    const int wb = nblk(d.nw * WL, 256);
+    if (with_couple)
+      k_couple_wall<Pol, WL><<<wb + nblk(d.ns * kSolidLanes, 256), 256, 0, stream>>>(d, wb);
+    else
+      k_wall_update<Pol, WL><<<wb, 256, 0, stream>>>(d);
+  }
+  void launch_wall(bool with_couple) {
+    if (d.nw <= (1 << 16))
+      launch_wall_at<8>(with_couple);
+    else if (d.nw <= (1 << 17))
+      launch_wall_at<4>(with_couple);
+    else if (d.nw <= (1 << 18))
+      launch_wall_at<2>(with_couple);
+    else
+      launch_wall_at<1>(with_couple);
+  }
+
   int enqueue_step() {
+    // untimed steps run the coupling and the wall update as one launch (both read the fluid only)
+    const bool fuse = !timing && d.nf > 0 && d.ns > 0 && d.nw > 0;
     for (int st = SPHNJ_STAGE_CELLS; st <= SPHNJ_STAGE_SHIFT; ++st) {
+      if (fuse && st == SPHNJ_STAGE_COUPLE) {
+        launch_wall(true);
+        LAUNCH_CHECK();
+        continue;
+      }
+      if (fuse && st == SPHNJ_STAGE_WALL) continue;
       hipEvent_t a = nullptr, b = nullptr;
       if (timing) {
         a = take_event();

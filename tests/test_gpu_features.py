@@ -206,3 +206,27 @@ def test_solid_sort_follows_cell_changes(gpu, oracle_mod):
     for k in ("x", "y", "u", "v", "rho", "p"):
         assert rel_l2(a["fluid"][k], b["fluid"][k]) <= 1e-9, k
     assert np.array_equal(a["solid"]["x"], b["solid"]["x"]) or rel_l2(a["solid"]["x"], b["solid"]["x"]) <= 1e-12
+
+
+@pytest.mark.parametrize("what", ["empty", "walls_only", "one_fluid", "solids_only"])
+def test_degenerate_inputs(gpu, oracle_mod, what):
+    """Degenerate inputs step like the oracle: no particle at all, walls alone, a single fluid
+    particle (ballistic under gravity), a solid block alone (the fluid stages are skipped)."""
+    p = params()
+    c = {
+        "empty": lambda: case(p),
+        "walls_only": lambda: case(p, w=wall(*lattice(5, 3, 0.02, 0.1, 0.0))),
+        "one_fluid": lambda: case(p, f=fluid([0.5], [0.5])),
+        "solids_only": lambda: case(p, s=solid(*lattice(4, 4, 0.02, 0.4, 0.4))),
+    }[what]()
+    sim = Simulation(c, "p64")
+    ora = oracle_mod.Oracle(c)
+    sim.step(3)
+    ora.step(3)
+    assert sim.status() == ora.status() == 0
+    a, b = sim.download(), ora.download()
+    for sp in ("fluid", "solid", "wall"):
+        assert len(a[sp]["x"]) == len(b[sp]["x"]) == len(getattr(c, sp)["x"]), sp
+        for k in ("x", "y"):
+            if len(b[sp]["x"]):
+                assert rel_l2(a[sp][k], b[sp][k]) <= 1e-12, (sp, k)

@@ -83,6 +83,7 @@ struct Dev {
   int ncase;
   int total_cells;
   const int *f_case_off, *s_case_off, *w_case_off;  // ncase + 1 each
+  const int *f_case_blk, *s_case_blk, *w_case_blk;  // case of slot 256 k, per species
   int nf, ns, nw;
   // fluid
   FluidBuf<Pol> fa, fb;  // cur (a) / alt (b)
@@ -159,6 +160,16 @@ __device__ __forceinline__ int case_of(int i, const int *off, int ncase) {
       hi = mid;
   }
   return lo;
+}
+
+// The same from a table of the case of every 256th slot (blk[k] = case_of(256 k)): one load
+// and a short forward walk instead of a binary search (cases hold consecutive slot ranges, so
+// the walk passes only cases ending inside the 256 slots, empty ones included).
+__device__ __forceinline__ int case_of(int i, const int *off, const int *blk, int ncase) {
+  if (ncase == 1) return 0;
+  int c = blk[i >> 8];
+  while (off[c + 1] <= i) ++c;
+  return c;
 }
 
 constexpr int kSpread = 64, kSlotStride = 16;

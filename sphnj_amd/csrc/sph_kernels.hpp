@@ -182,7 +182,7 @@ __global__ void __launch_bounds__(256) k_hash_fluid(Dev<Pol> d, const int *solid
   const bool valid = s < d.nf;
   int key = -1 - lane;  // distinct for the lanes past the end
   if (valid) {
-    const int c = case_of(s, d.f_case_off, d.ncase);
+    const int c = case_of(s, d.f_case_off, d.f_case_blk, d.ncase);
     const auto &cp = d.cases[c];
     unsigned fl = 0;
     const int sx = axis_sub(d.fa.x[s], cp.cxmin, cp.inv_cdx, cp.nx, &fl);
@@ -278,7 +278,7 @@ __global__ void __launch_bounds__(256) k_hash_solid(Dev<Pol> d, const double *x,
                                                     int *cxy_out, int update_view, int *moved) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.ns) return;
-  const int c = case_of(i, d.s_case_off, d.ncase);
+  const int c = case_of(i, d.s_case_off, d.s_case_blk, d.ncase);
   const auto &cp = d.cases[c];
   unsigned fl = 0;
   const int cx = axis_cell(x[i], cp.cxmin, cp.inv_cdx, cp.nx, &fl);
@@ -304,7 +304,7 @@ template <class Pol>
 __global__ void __launch_bounds__(256) k_hash_wall(Dev<Pol> d, const double *x, const double *y, int *keys) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.nw) return;
-  const int c = case_of(i, d.w_case_off, d.ncase);
+  const int c = case_of(i, d.w_case_off, d.w_case_blk, d.ncase);
   const auto &cp = d.cases[c];
   unsigned fl = 0;
   const int cx = axis_cell(x[i], cp.cxmin, cp.inv_cdx, cp.nx, &fl);
@@ -347,7 +347,7 @@ __global__ void __launch_bounds__(256) k_permute_fluid(Dev<Pol> d) {
   d.f_phase2[t] = d.f_phase[s];
   d.f_cls2[t] = d.f_cls[s];
   d.f_orig[t] = o;
-  const int c = case_of(t, d.f_case_off, d.ncase);
+  const int c = case_of(t, d.f_case_off, d.f_case_blk, d.ncase);
   const auto &cp = d.cases[c];
   unsigned fl = 0;
   const int sx = axis_sub(x, cp.cxmin, cp.inv_cdx, cp.nx, &fl);
@@ -375,7 +375,7 @@ __global__ void __launch_bounds__(256) k_permute_wall(Dev<Pol> d, const int *sor
   d.wy[t] = y;
   d.w_bc[t] = bc0[o];
   d.w_orig[t] = o;
-  const int c = case_of(t, d.w_case_off, d.ncase);
+  const int c = case_of(t, d.w_case_off, d.w_case_blk, d.ncase);
   const auto &cp = d.cases[c];
   unsigned fl = 0;
   const int cx = axis_cell(x, cp.cxmin, cp.inv_cdx, cp.nx, &fl);
@@ -428,7 +428,7 @@ __global__ void __launch_bounds__(256) k_wall_normal(Dev<Pol> d) {
   using R = typename Pol::R;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.nw) return;
-  const int c = case_of(i, d.w_case_off, d.ncase);
+  const int c = case_of(i, d.w_case_off, d.w_case_blk, d.ncase);
   const auto &cp = d.cases[c];
   const int pk = d.w_cxy[i], cx = cxy_x(pk), cy = cxy_y(pk);
   const R xi = d.w_vx[i], yi = d.w_vy[i];
@@ -478,7 +478,7 @@ __device__ __forceinline__ void wall_update_block(const Dev<Pol> &d, int bid) {
   int phase = 0, near_run = kCellRuns;  // run holding the lane's nearest fluid (none yet)
   int c = 0;
   if (active) {
-    c = case_of(i, d.w_case_off, d.ncase);
+    c = case_of(i, d.w_case_off, d.w_case_blk, d.ncase);
     const auto &cp = d.cases[c];
     const int pk = d.w_cxy[i], cx = cxy_x(pk), cy = cxy_y(pk);
     const R xi = d.w_vx[i], yi = d.w_vy[i];
@@ -1917,7 +1917,7 @@ __device__ __forceinline__ void solid_couple_block(const Dev<Pol> &d, int bid) {
   int left = 0;
   int c = 0;
   if (active) {
-    c = case_of(i, d.s_case_off, d.ncase);
+    c = case_of(i, d.s_case_off, d.s_case_blk, d.ncase);
     const auto &cp = d.cases[c];
     const int pk = d.s_cxy[i], cx = cxy_x(pk), cy = cxy_y(pk);
     const R xi = d.s_vx[i], yi = d.s_vy[i];
@@ -2019,7 +2019,7 @@ __global__ void __launch_bounds__(256) k_solid_deform(Dev<Pol> d, int sub) {
   int c = 0;
   R h00 = 0, h01 = 0, h10 = 0, h11 = 0;
   if (active) {
-    c = case_of(i, d.s_case_off, d.ncase);
+    c = case_of(i, d.s_case_off, d.s_case_blk, d.ncase);
     if (sub < d.cases[c].S) {
       const double dix = d.sx[i] - d.sX[i], diy = d.sy[i] - d.sY[i];
       struct Nb {
@@ -2132,7 +2132,7 @@ __global__ void __launch_bounds__(256) k_solid_force_update(Dev<Pol> d, int sub)
   bool run = false;
   R cpx = 0, cpy = 0, ax = 0, ay = 0;
   if (active) {
-    c = case_of(i, d.s_case_off, d.ncase);
+    c = case_of(i, d.s_case_off, d.s_case_blk, d.ncase);
     const auto &cp = d.cases[c];
     run = sub < cp.S;
     if (run) {
@@ -2256,7 +2256,7 @@ __global__ void __launch_bounds__(256) k_ref_count(Dev<Pol> d, const int *cxy, c
                                                    const int *sorted, const double *rc2_d, int *count) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.ns) return;
-  const int c = case_of(i, d.s_case_off, d.ncase);
+  const int c = case_of(i, d.s_case_off, d.s_case_blk, d.ncase);
   const auto &cp = d.cases[c];
   const int cx = cxy_x(cxy[i]), cy = cxy_y(cxy[i]);
   const double Xi = d.sX[i], Yi = d.sY[i], rc2 = rc2_d[c];
@@ -2285,7 +2285,7 @@ __global__ void __launch_bounds__(256) k_ref_fill(Dev<Pol> d, const int *cxy, co
   using R = typename Pol::R;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= d.ns) return;
-  const int c = case_of(i, d.s_case_off, d.ncase);
+  const int c = case_of(i, d.s_case_off, d.s_case_blk, d.ncase);
   const auto &cp = d.cases[c];
   const double kdc = kdc_d[c], hinv2 = hinv2_d[c], rc2 = rc2_d[c];
   const int cx = cxy_x(cxy[i]), cy = cxy_y(cxy[i]);
@@ -2346,7 +2346,7 @@ __global__ void __launch_bounds__(256) k_neighbors(Dev<Pol> d, int lo, int hi, c
   using R = typename Pol::R;
   const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= hi) return;
-  const int c = case_of(i, d.f_case_off, d.ncase);
+  const int c = case_of(i, d.f_case_off, d.f_case_blk, d.ncase);
   const auto &cp = d.cases[c];
   const auto &A = d.fa;
   const int pk = d.f_cxy[i], cx = cxy_x(pk) >> 1, cy = cxy_y(pk) >> 1;

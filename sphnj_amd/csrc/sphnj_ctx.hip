@@ -327,6 +327,31 @@ struct Engine : EngineBase {
     d.f_case_off = fco;
     d.s_case_off = sco;
     d.w_case_off = wco;
+    {  // case of every 256th slot, per species (the binary search of case_of, on the host)
+      auto table = [&](const std::vector<int> &off, int n, const int **out) -> int {
+        std::vector<int> t(std::max(nblk(n, 256), 1), 0);
+        for (size_t k = 0; k < t.size(); ++k) {
+          const int i = (int)k * 256;
+          int lo = 0, hi = ncase;
+          while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (off[mid] <= i)
+              lo = mid;
+            else
+              hi = mid;
+          }
+          t[k] = lo;
+        }
+        int *dev = nullptr, r;
+        if ((r = dalloc(&dev, t.size())) || (r = h2d(dev, t.data(), t.size()))) return r;
+        HIPC(hipStreamSynchronize(stream));  // t is freed on return
+        *out = dev;
+        return SPHNJ_OK;
+      };
+      if ((rc = table(f_off, nf, &d.f_case_blk)) || (rc = table(s_off, ns, &d.s_case_blk)) ||
+          (rc = table(w_off, nw, &d.w_case_blk)))
+        return rc;
+    }
 
     // ---- fluid
     for (FluidBuf<Pol> *B : {&d.fa, &d.fb}) {

@@ -688,8 +688,9 @@ __global__ void __launch_bounds__(64 * NW) k_subcol_rows(Dev<Pol> d, int G, cons
   const int kb = 4 * cp.cell_off + sx * ny2;
   int *lb = d.f_lb + cp.lb_off + sx * (ny2 + 1);
   // each thread takes a contiguous run of rows [k0, k1), held in registers when it has at most
-  // kR of them (up to 2048 sub-rows), else read again in the backward pass
-  constexpr int kR = 8;
+  // kR of them (up to 2048 sub-rows with four waves, 1024 with one), else read again in the
+  // backward pass
+  constexpr int kR = NW == 1 ? 16 : 8;
   const int per = (ny2 + NT - 1) / NT, k0 = min(t * per, ny2), k1 = min(k0 + per, ny2);
   int lmin = 0x7fffffff, end = 0;
   int2 r[kR];

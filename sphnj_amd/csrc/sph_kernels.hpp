@@ -242,10 +242,25 @@ __global__ void __launch_bounds__(1024) k_exscan_small(const int *in, int n, int
 }
 
 // Counting sort, step 3: each key's [start, end) slot range (empty: [0, 0)), in rounds of
-// kSortThreads consecutive keys; the counts are cleared for the next step.
-__global__ void __launch_bounds__(kSortThreads) k_count_apply(int *cnt, int nk, const int *toff, int2 *range) {
+// kSortThreads consecutive keys; the counts are cleared for the next step.  The tile's first
+// slot comes from toff (the scanned tile sums), or, with up to kApplyPrefixMax tiles and
+// toff == nullptr, from the block's own sum of the tile sums before it (no scan launch).
+constexpr int kApplyPrefixMax = 4096;
+__global__ void __launch_bounds__(kSortThreads) k_count_apply(int *cnt, int nk, const int *tsum, const int *toff,
+                                                               int2 *range) {
   __shared__ int tmp[4];
-  int run = toff[blockIdx.x];
+  int run;
+  if (toff != nullptr) {
+    run = toff[blockIdx.x];
+  } else {
+    int before = 0;
+    for (int k = (int)threadIdx.x; k < (int)blockIdx.x; k += kSortThreads) before += tsum[k];
+    before = wave_sum(before);
+    if ((threadIdx.x & 63) == 0) tmp[threadIdx.x >> 6] = before;
+    __syncthreads();
+    run = tmp[0] + tmp[1] + tmp[2] + tmp[3];
+    __syncthreads();  // tmp is reused by the scans below
+  }
   for (int q = 0; q < kScanItems; ++q) {
     const int k = blockIdx.x * kScanTile + q * kSortThreads + (int)threadIdx.x;
     const int c = k < nk ? cnt[k] : 0;

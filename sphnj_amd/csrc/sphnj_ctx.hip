@@ -623,8 +623,12 @@ struct Engine : EngineBase {
       const int nk = 4 * nc, nkb = nblk(nk, kScanTile);
       k_hash_fluid<Pol><<<nblk(nf, 256), 256, 0, stream>>>(d, moved);
       k_count_tiles<<<nkb, kSortThreads, 0, stream>>>(d.f_cnt, nk, cnt_tsum);
-      k_exscan_small<<<1, 1024, 0, stream>>>(cnt_tsum, nkb, cnt_toff);
-      k_count_apply<<<nkb, kSortThreads, 0, stream>>>(d.f_cnt, nk, cnt_toff, d.f_srange);
+      if (nkb > kApplyPrefixMax) {
+        k_exscan_small<<<1, 1024, 0, stream>>>(cnt_tsum, nkb, cnt_toff);
+        k_count_apply<<<nkb, kSortThreads, 0, stream>>>(d.f_cnt, nk, cnt_tsum, cnt_toff, d.f_srange);
+      } else {
+        k_count_apply<<<nkb, kSortThreads, 0, stream>>>(d.f_cnt, nk, cnt_tsum, nullptr, d.f_srange);
+      }
       k_count_scatter<Pol><<<nblk(nf, 256), 256, 0, stream>>>(d);
       LAUNCH_CHECK();
       k_permute_fluid<Pol><<<nblk(std::max(nf, ncase * kSpread), blk_stream), blk_stream, 0, stream>>>(d);

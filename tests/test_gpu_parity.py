@@ -167,12 +167,15 @@ def test_graph_equals_eager(gpu):
                 assert np.array_equal(da[sp][k], db[sp][k]), (sp, k, n)
 
 
-def test_batch_equals_single(gpu):
+@pytest.mark.parametrize("precision", ["p64", "m32"])
+def test_batch_equals_single(gpu, precision):
+    """A batch of uneven cases (two dam breaks and an FSI gate) steps each case bit for bit as
+    it steps alone."""
     cases = [small_dam(), small_dam(col_w=0.3, dx=0.025), small_gate()]
-    batch = Simulation(cases, "p64")
+    batch = Simulation(cases, precision)
     batch.step(5)
     for i, c in enumerate(cases):
-        one = Simulation(c, "p64")
+        one = Simulation(c, precision)
         one.step(5)
         a, b = batch.download(i), one.download()
         for sp in ("fluid", "solid", "wall"):

@@ -43,6 +43,12 @@ BYTES = {
 WALL_BYTES = {"m32": 38, "p64": 66}  # wall Shepard, per wall particle
 
 
+def shift_step_bytes_m32(K):
+    """M32 shift bytes per fluid particle per step at K passes: 61 B a pass, plus the last
+    pass's extra 56 B (66 B a pass on average at K = 10)."""
+    return 61 * K + 56
+
+
 def configs():
     from sphnj_amd import Case
 
@@ -251,10 +257,13 @@ def main() -> int:
             done = args.accuracy_steps
         a, b = sim.download(), ref.download()
         errs = {f: rel_l2(a["fluid"][f], b["fluid"][f]) for f in ("x", "y", "u", "v", "rho", "p")}
+        # velocity as one vector field: |u_a - u_b| over |u_b|, summed over particles
+        errs["vel"] = rel_l2(np.stack([a["fluid"]["u"], a["fluid"]["v"]]), np.stack([b["fluid"]["u"], b["fluid"]["v"]]))
         if ns:
             errs.update({"solid_" + f: rel_l2(a["solid"][f], b["solid"][f]) for f in ("x", "y", "stress")})
         probes = case_probes(args.config, case, a, b)
-        extra = {"p64_value": round(p64_value, 3), "p64_ms_per_step": round(1e3 * tp / args.steps, 4),
+        extra = {"p64_value": round(p64_value, 3), "p64_fluid_value": round(nf * args.steps / tp / 1e6, 3),
+                 "p64_ms_per_step": round(1e3 * tp / args.steps, 4),
                  "speedup_vs_p64": round(value / p64_value, 3),
                  "rel_l2_vs_p64": {k: float(f"{v:.3e}") for k, v in errs.items()},
                  "steps_compared": done,
@@ -281,6 +290,16 @@ def main() -> int:
                 "valu_per_cu_cycle": measured_issue(args.config, dom, blocks[dom]),
                 "bytes_per_particle": BYTES[dom]["m32"], "ms_per_launch": round(per_launch_ms, 5)}
 
+    # the whole step against HBM: algorithmic bytes of every fluid and wall stage (solids, which
+    # are at most 0.3% of the particles of these workloads, are not counted) over the step time
+    K = case.params.shift_per_step
+    step_bytes = nf * (BYTES["cells"]["m32"] + BYTES["fluid"]["m32"] + shift_step_bytes_m32(K)) \
+        + nw * WALL_BYTES["m32"]
+    step_gbs = step_bytes * args.steps / t / 1e9
+    step_hbm = {"achieved": round(step_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(step_gbs / HBM_PEAK_GBS, 4), "bytes_per_step": int(step_bytes),
+                "species": "fluid + wall"}
+
     cpu = None
     if args.cpu_steps > 0:
         from oracle.oracle_py import Oracle, set_threads
@@ -302,7 +321,8 @@ def main() -> int:
             "config": {"workload": desc, "fluid": nf, "solid": ns, "wall": nw, "form": "M32 (FP32-mixed)",
                        "dt": case.params.dt, "shift_per_step": case.params.shift_per_step,
                        "solid_sub_step": case.params.solid_sub_step},
-            "roofline": roofline, "cpu_baseline": cpu,
+            "roofline": roofline, "step_hbm": step_hbm, "cpu_baseline": cpu,
+            "fluid_value": round(nf * args.steps / t / 1e6, 3),
             "stage_ms": {k: round(v["ms_per_step"], 4) for k, v in prof.items()},
             "autotune": tune}
     line.update(extra)

@@ -483,13 +483,21 @@ __global__ void __launch_bounds__(256) k_wall_normal(Dev<Pol> d) {
 // nearest fluid particle (first strict minimum in iteration order), density from the EOS.
 // Each wall particle has a group of LANES lanes that split its fluid runs: 8 when the walls
 // are few (tens of thousands, which alone would not fill the chip), fewer as they grow.
+// Whatever the group size, the sums are formed in one order -- eight partial sums over the
+// runs r = k, k + 8, k + 16 (k = 0..7), combined as an 8-lane butterfly -- so a wall's state
+// does not depend on how many walls share the launch (e.g. a case alone or in a batch).
+constexpr int kWallPartials = 8;
 template <class Pol, int kWallLanes>
 __device__ __forceinline__ void wall_update_block(const Dev<Pol> &d, int bid) {
   using R = typename Pol::R;
+  constexpr int NP = kWallPartials / kWallLanes;  // partial sums per lane
+  static_assert(NP * kWallLanes == kWallPartials, "group size divides the partial count");
   const int gid = bid * 256 + (int)threadIdx.x;
   const int i = gid / kWallLanes, l = gid % kWallLanes;
   const bool active = i < d.nw;
-  R S = 0, U = 0, V = 0, P = 0, nearest = 0;
+  R S[NP], U[NP], V[NP], P[NP], nearest = 0;
+#pragma unroll
+  for (int j = 0; j < NP; ++j) S[j] = U[j] = V[j] = P[j] = 0;
   int phase = 0, near_run = kCellRuns;  // run holding the lane's nearest fluid (none yet)
   int c = 0;
   if (active) {
@@ -498,32 +506,45 @@ __device__ __forceinline__ void wall_update_block(const Dev<Pol> &d, int bid) {
     const int pk = d.w_cxy[i], cx = cxy_x(pk), cy = cxy_y(pk);
     const R xi = d.w_vx[i], yi = d.w_vy[i];
     const auto &F = d.fa;
-    // the 3x3 cells' 18 fluid runs over the wall particle's lanes, each lane in run order
-    for (int r = l; r < kCellRuns; r += kWallLanes) {
-      int2 rr;
-      R bx, by;
-      if (!cell_run(d, cp, cx, cy, r, xi, yi, rr, bx, by)) continue;
-      for (int s = rr.x; s < rr.y; ++s) {
-        const R dx = F.vx[s] - bx, dy = F.vy[s] - by;
-        const R r2 = dx * dx + dy * dy;
-        if (!(r2 < cp.rc2)) continue;
-        const R w = (kexp<R>(-(r2 * cp.hinv2)) - cp.eR) * cp.k2c;
-        S += w;
-        U += F.vu[s] * w;
-        V += F.vv[s] * w;
-        P += (F.p[s] - F.vrho[s] * (cp.gx * dx + cp.gy * dy)) * w;
-        if (near_run == kCellRuns || r2 < nearest) {
-          near_run = r;
-          nearest = r2;
-          phase = d.f_phase[s];
+    // the 3x3 cells' 18 fluid runs: partial sum k = l + j * LANES takes runs k, k + 8, ...
+#pragma unroll
+    for (int j = 0; j < NP; ++j) {
+      for (int r = l + j * kWallLanes; r < kCellRuns; r += kWallPartials) {
+        int2 rr;
+        R bx, by;
+        if (!cell_run(d, cp, cx, cy, r, xi, yi, rr, bx, by)) continue;
+        for (int s = rr.x; s < rr.y; ++s) {
+          const R dx = F.vx[s] - bx, dy = F.vy[s] - by;
+          const R r2 = dx * dx + dy * dy;
+          if (!(r2 < cp.rc2)) continue;
+          const R w = (kexp<R>(-(r2 * cp.hinv2)) - cp.eR) * cp.k2c;
+          S[j] += w;
+          U[j] += F.vu[s] * w;
+          V[j] += F.vv[s] * w;
+          P[j] += (F.p[s] - F.vrho[s] * (cp.gx * dx + cp.gy * dy)) * w;
+          // first strict minimum of the serial scan: ties go to the earlier run
+          if (near_run == kCellRuns || r2 < nearest || (r2 == nearest && r < near_run)) {
+            near_run = r;
+            nearest = r2;
+            phase = d.f_phase[s];
+          }
         }
       }
     }
   }
-  S = lanes_sum<kWallLanes>(S);
-  U = lanes_sum<kWallLanes>(U);
-  V = lanes_sum<kWallLanes>(V);
-  P = lanes_sum<kWallLanes>(P);
+  // the butterfly's levels 4, 2, ... that pair partials of one lane, then those across lanes
+#pragma unroll
+  for (int h = NP / 2; h > 0; h >>= 1) {
+#pragma unroll
+    for (int j = 0; j < h; ++j) {
+      S[j] += S[j + h];
+      U[j] += U[j + h];
+      V[j] += V[j + h];
+      P[j] += P[j + h];
+    }
+  }
+  const R Ssum = lanes_sum<kWallLanes>(S[0]), Usum = lanes_sum<kWallLanes>(U[0]);
+  const R Vsum = lanes_sum<kWallLanes>(V[0]), Psum = lanes_sum<kWallLanes>(P[0]);
   // the nearest fluid of the whole scan: smallest r^2, ties to the earlier run (the scan order
   // of the cells, so the first strict minimum of a serial scan)
 #pragma unroll
@@ -539,10 +560,10 @@ __device__ __forceinline__ void wall_update_block(const Dev<Pol> &d, int bid) {
   if (!active || l != 0) return;
   const auto &cp = d.cases[c];
   R u = 0, v = 0, p = 0;
-  if (S != R(0)) {
-    u = U / S;
-    v = V / S;
-    p = P / S;
+  if (Ssum != R(0)) {
+    u = Usum / Ssum;
+    v = Vsum / Ssum;
+    p = Psum / Ssum;
   }
   const R nx = d.w_n[2 * i], ny = d.w_n[2 * i + 1];
   const int bc = d.w_bc[i];

@@ -183,6 +183,28 @@ def test_batch_equals_single(gpu, precision):
                 assert np.array_equal(a[sp][k], b[sp][k]), (i, sp, k)
 
 
+@pytest.mark.parametrize("precision", ["p64", "m32"])
+def test_batch_equals_single_past_one_block_scan(gpu, precision):
+    """The first eight seeded ensemble members (5 k to 184 k fluid each) batched: together they
+    have more than 4096 sub-columns, so the batch plans its tiles through the separate tile
+    scan across case boundaries, while each member alone stays under that size; every member
+    still steps bit for bit as it steps alone."""
+    from sphnj_amd.sim import grid
+
+    cases = [Case.generate("ensemble_member", index=i) for i in range(8)]
+    cols = [2 * grid(c.params)[0] for c in cases]
+    assert sum(cols) > 4096 and max(cols) <= 4096, cols
+    batch = Simulation(cases, precision)
+    batch.step(3)
+    for i, c in enumerate(cases):
+        one = Simulation(c, precision)
+        one.step(3)
+        a, b = batch.download(i), one.download()
+        one.close()
+        for k in a["fluid"]:
+            assert np.array_equal(a["fluid"][k], b["fluid"][k]), (i, k)
+
+
 @pytest.mark.parametrize("precision,tol", [("p64", 1e-9), ("m32", 1e-4)])
 @pytest.mark.parametrize("name", ["dam_liquid", "dam_two_phase", "gate_fsi"])
 def test_matches_golden(gpu, name, precision, tol):

@@ -1949,12 +1949,17 @@ __device__ __forceinline__ void solid_couple_block(const Dev<Pol> &d, int bid) {
   using R = typename Pol::R;
   const int gid = bid * 256 + (int)threadIdx.x;
   const int i = gid / kSolidLanes, l = gid % kSolidLanes;
-  const bool active = i < d.ns;
   R nx = 0, ny = 0, P = 0, S = 0;
   int left = 0;
   int c = 0;
+  // a case without fluid is not coupled (coupleCalculation returns, src/solid.cu:494-499):
+  // its solids keep their pressure and normal, also in a batch with fluid in other cases
+  bool active = i < d.ns;
   if (active) {
     c = case_of(i, d.s_case_off, d.s_case_blk, d.ncase);
+    active = d.f_case_off[c + 1] > d.f_case_off[c];
+  }
+  if (active) {
     const auto &cp = d.cases[c];
     const int pk = d.s_cxy[i], cx = cxy_x(pk), cy = cxy_y(pk);
     const R xi = d.s_vx[i], yi = d.s_vy[i];

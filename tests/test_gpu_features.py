@@ -230,3 +230,26 @@ def test_degenerate_inputs(gpu, oracle_mod, what):
         for k in ("x", "y"):
             if len(b[sp]["x"]):
                 assert rel_l2(a[sp][k], b[sp][k]) <= 1e-12, (sp, k)
+
+
+@pytest.mark.parametrize("precision", ["p64", "m32"])
+def test_degenerate_members_in_batch(gpu, precision):
+    """A batch that mixes full cases with the degenerate ones (no particle, walls alone, one
+    fluid particle, solids alone): the empty slot ranges and cell grids of those members leave
+    the others untouched, and every member steps bit for bit as it steps alone."""
+    p = params()
+    cases = [gate(), case(p), case(p, w=wall(*lattice(5, 3, 0.02, 0.1, 0.0))),
+             case(p, f=fluid([0.5], [0.5])), case(p, s=solid(*lattice(4, 4, 0.02, 0.4, 0.4))),
+             two_phase_small()]
+    batch = Simulation(cases, precision)
+    batch.step(3)
+    assert batch.status() == 0
+    for i, c in enumerate(cases):
+        one = Simulation(c, precision)
+        one.step(3)
+        a, b = batch.download(i), one.download()
+        one.close()
+        for sp in ("fluid", "solid", "wall"):
+            assert len(a[sp]["x"]) == len(getattr(c, sp)["x"]), (i, sp)
+            for k in a[sp]:
+                assert np.array_equal(a[sp][k], b[sp][k]), (i, sp, k)

@@ -154,16 +154,19 @@ def test_gate_fsi_steps(gpu, oracle_mod, precision, tol):
                   solid_fields=("x", "y", "u", "v", "pressure"))
 
 
-def test_graph_equals_eager(gpu):
+@pytest.mark.parametrize("precision", ["p64", "m32"])
+def test_graph_equals_eager(gpu, precision):
+    """Replays of the captured step, with eager work and downloads between them, match the
+    eager step bit for bit."""
     case = small_gate()
-    a = Simulation(case, "p64", graph=True)
-    b = Simulation(case, "p64", graph=False)
+    a = Simulation(case, precision, graph=True)
+    b = Simulation(case, precision, graph=False)
     for n in (1, 2, 3, 1):
         a.step(n)
         b.step(n)
         da, db = a.download(), b.download()
-        for sp in ("fluid", "solid"):
-            for k in ("x", "y", "u", "v"):
+        for sp in ("fluid", "solid", "wall"):
+            for k in da[sp]:
                 assert np.array_equal(da[sp][k], db[sp][k]), (sp, k, n)
 
 

@@ -253,3 +253,35 @@ def test_degenerate_members_in_batch(gpu, precision):
             assert len(a[sp]["x"]) == len(getattr(c, sp)["x"]), (i, sp)
             for k in a[sp]:
                 assert np.array_equal(a[sp][k], b[sp][k]), (i, sp, k)
+
+
+def test_destroy_returns_device_memory(gpu):
+    """Contexts over a 2.6 M-cell grid (a few hundred MB of device memory each) are created,
+    stepped through their graphs, autotuned, downloaded and destroyed eight times: the device's
+    free memory comes back to within 64 MB of where it started."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so.7")
+
+    def free_bytes():
+        f, t = C.c_size_t(), C.c_size_t()
+        assert hip.hipMemGetInfo(C.byref(f), C.byref(t)) == 0
+        return f.value
+
+    c = Case.generate("dam_break", tank_w=17.0, tank_h=9.0, col_w=17.0, col_h=0.05, dx=0.002)
+    s = Simulation(c, "m32")  # the runtime's own first-use allocations happen here
+    s.step(1)
+    s.close()
+    f0 = free_bytes()
+    used = 0
+    for k in range(8):
+        s = Simulation(c, "m32" if k % 2 else "p64")
+        s.step(2)
+        if k == 0:
+            s.autotune()
+        used = max(used, f0 - free_bytes())
+        s.download()
+        s.close()
+    f1 = free_bytes()
+    assert used > 64 << 20, used  # the contexts were large enough to show a leak
+    assert f0 - f1 < 64 << 20, (f0 - f1, used)

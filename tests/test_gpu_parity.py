@@ -302,28 +302,35 @@ LONG_CASES = {
     # a two-phase dam break: 3.6 k water under 4.4 k air, K = 3
     "two_phase": lambda: Case.generate("dam_break", tank_w=1.0, tank_h=0.8, col_w=0.4, col_h=0.6, dx=0.01,
                                        air=1, K=3),
+    # an elastic gate: 3.5 k water, 120 solids, S = 4 solid sub-steps
+    "gate": small_gate,
 }
 _LONG_ORACLE = {}
 
 
 @pytest.mark.parametrize("precision", ["p64", "m32"])
-@pytest.mark.parametrize("name", ["dam5k", "two_phase"])
+@pytest.mark.parametrize("name", ["dam5k", "two_phase", "gate"])
 def test_precision_target_after_1000_steps(gpu, oracle_mod, name, precision):
     """The precision target where it is stated: relative L2 against the FP64 oracle after 1000
-    steps, per field (position, density, pressure, velocity as one vector and per component):
-    M32 within 1e-4 (seen: at most 3.4e-6, on p), P64 within 1e-9 (seen: at most 1.9e-13)."""
+    steps, per field (position, density, pressure, velocity as one vector and per component;
+    the gate's solid position, velocity, pressure and stress too): M32 within 1e-4 (seen: at most
+    4.6e-6, on the solid v), P64 within 1e-9 (seen: at most 8.9e-12)."""
     if name not in _LONG_ORACLE:  # one oracle run serves both precisions
         ora = oracle_mod.Oracle(LONG_CASES[name]())
         ora.step(1000)
-        _LONG_ORACLE[name] = ora.download()["fluid"]
+        _LONG_ORACLE[name] = ora.download()
         ora.close()
-    b = _LONG_ORACLE[name]
+    ref = _LONG_ORACLE[name]
     sim = Simulation(LONG_CASES[name](), precision)
     sim.step(1000)
     assert sim.status() == 0
-    a = sim.download()["fluid"]
+    out = sim.download()
+    a, b = out["fluid"], ref["fluid"]
     errs = {f: rel_l2(a[f], b[f]) for f in ("x", "y", "u", "v", "rho", "p")}
     errs["vel"] = rel_l2(np.stack([a["u"], a["v"]]), np.stack([b["u"], b["v"]]))
+    if len(ref["solid"]["x"]):
+        errs.update({"solid." + f: rel_l2(out["solid"][f], ref["solid"][f])
+                     for f in ("x", "y", "u", "v", "pressure", "stress")})
     print(name, precision, {k: f"{v:.2e}" for k, v in errs.items()}, flush=True)
     tol = 1e-9 if precision == "p64" else 1e-4
     bad = {k: v for k, v in errs.items() if not v <= tol}

@@ -285,3 +285,20 @@ def test_destroy_returns_device_memory(gpu):
     f1 = free_bytes()
     assert used > 64 << 20, used  # the contexts were large enough to show a leak
     assert f0 - f1 < 64 << 20, (f0 - f1, used)
+
+
+@pytest.mark.parametrize("precision", ["p64", "m32"])
+def test_stages_equal_step(gpu, precision):
+    """The six phases called one by one equal the step, where the coupling and the wall update
+    share one launch (a case with fluid, solids and walls), over three steps."""
+    c = gate()
+    a = Simulation(c, precision, graph=False)
+    b = Simulation(c, precision, graph=False)
+    for _ in range(3):
+        for st in ("cells", "couple", "wall", "fluid", "solid", "shift"):
+            a.stage(st)
+        b.step(1)
+    da, db = a.download(), b.download()
+    for sp in ("fluid", "solid", "wall"):
+        for k in da[sp]:
+            assert np.array_equal(da[sp][k], db[sp][k]), (sp, k)

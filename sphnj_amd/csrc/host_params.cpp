@@ -11,6 +11,7 @@
 #include <fstream>
 #include <map>
 #include <sstream>
+#include <vector>
 
 namespace sphnj {
 
@@ -19,52 +20,55 @@ void set_global_error(const std::string &msg) { g_error = msg; }
 
 namespace {
 
-void trim(std::string &s) {
-  static const char ws[] = " \n\t\v\r\f";
-  s.erase(0, s.find_first_not_of(ws));
-  s.erase(s.find_last_not_of(ws) + 1U);
+// The INI text is first split into physical lines, then grouped into entries: a line whose
+// code part (the text before the first '#') holds '=' opens an entry, and the lines after it
+// that carry a value but no '=' extend that entry's value.  The grouping rules reproduce the
+// observable behaviour of the reference's Config reader (src/config.cpp:59-129):
+//  * the key is the text before the first '=', the value the text after it, both stripped of
+//    surrounding white space;
+//  * a line that is entirely white space closes the open entry;
+//  * a line whose code part holds '=' closes it and opens the next one;
+//  * a comment-only line neither closes the entry nor adds to it;
+//  * any other line is appended to the value after a '\n';
+//  * lines before the first entry, and lines that open none, are ignored;
+//  * a key given twice keeps its last value.
+// The reference re-reads its stream and can append a file's last line twice when the file
+// does not end in a newline; every line is taken once here.
+const char *const kSpace = " \n\t\v\r\f";
+
+std::string strip(const std::string &s) {
+  const auto b = s.find_first_not_of(kSpace);
+  if (b == std::string::npos) return std::string();
+  const auto e = s.find_last_not_of(kSpace);
+  return s.substr(b, e - b + 1);
 }
 
-// Line reader with the continuation rule of Config::operator>> (src/config.cpp:59-129):
-// everything from '#' on is dropped; a line holding '=' starts a key; following lines that
-// are non-blank and hold no '=' are appended to the value (joined by '\n' when they are not
-// empty after comment removal); a blank line or a line with '=' ends the value; a repeated
-// key overwrites the earlier one; lines without '=' that follow no key are ignored.
+bool all_space(const std::string &s) { return s.find_first_not_of(kSpace) == std::string::npos; }
+
+std::string code_part(const std::string &s) { return s.substr(0, s.find('#')); }
+
 std::map<std::string, std::string> parse_ini(std::istream &is) {
+  std::vector<std::string> lines;
+  for (std::string l; std::getline(is, l);) lines.push_back(l);
+
   std::map<std::string, std::string> kv;
-  const std::string delim = "=", comm = "#";
-  std::string nextline;
-  while (is || !nextline.empty()) {
-    std::string line;
-    if (!nextline.empty()) {
-      line = nextline;
-      nextline.clear();
-    } else {
-      std::getline(is, line);
+  size_t i = 0;
+  while (i < lines.size()) {
+    const std::string head = code_part(lines[i++]);
+    const auto eq = head.find('=');
+    if (eq == std::string::npos) continue;
+    std::string value = strip(head.substr(eq + 1));
+    for (; i < lines.size(); ++i) {
+      if (all_space(lines[i])) {
+        ++i;  // the blank line is consumed with the entry it closes
+        break;
+      }
+      const std::string code = code_part(lines[i]);
+      if (code.find('=') != std::string::npos) break;  // the next entry starts here
+      const std::string piece = strip(code);
+      if (!piece.empty()) value += "\n" + piece;
     }
-    line = line.substr(0, line.find(comm));
-    std::string::size_type dpos = line.find(delim);
-    if (dpos == std::string::npos) continue;
-    std::string key = line.substr(0, dpos);
-    line.replace(0, dpos + delim.size(), "");
-    bool terminate = false;
-    while (!terminate && is) {
-      std::getline(is, nextline);
-      terminate = true;
-      std::string nl = nextline;
-      trim(nl);
-      if (nl.empty()) continue;
-      nextline = nextline.substr(0, nextline.find(comm));
-      if (nextline.find(delim) != std::string::npos) continue;
-      nl = nextline;
-      trim(nl);
-      if (!nl.empty()) line += "\n";
-      line += nextline;
-      terminate = false;
-    }
-    trim(key);
-    trim(line);
-    kv[key] = line;
+    kv[strip(head.substr(0, eq))] = strip(value);
   }
   return kv;
 }

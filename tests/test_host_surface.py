@@ -80,6 +80,26 @@ def test_ini_continuation_lines(tmp_path):
     assert p.yield_strain == 0.05
 
 
+@pytest.mark.parametrize("text,viscid", [
+    ("is_viscid = no\n   # note\nis_steady = 1\n", 0),    # a comment-only line does not extend the value
+    ("is_viscid = no\n  # note\ny\n", 1),                 # ... nor close it: "no\ny" is not a false spelling
+    ("is_viscid = no\n  # note\n\ny\n", 0),               # a blank line closes it; "y" then opens nothing
+    ("is_viscid = no\n \t \ny\n", 0),                     # white space only is blank
+    ("is_viscid =\nfalse\n", 0),                          # an empty first value continues on the next line
+    ("stray words\nis_viscid = 0 # off\n", 0),            # lines before the first key are ignored
+    ("is_viscid = n\n=\n", 0),                            # '=' alone opens an entry with an empty key
+])
+def test_ini_entry_grouping(tmp_path, text, viscid):
+    """Which lines extend a value, which close it (src/config.cpp:84-118)."""
+    p = _ini(tmp_path, text)
+    assert p.is_viscid == viscid
+
+
+def test_ini_last_line_without_newline(tmp_path):
+    assert _ini(tmp_path, "gravity = 4.5").gravity == 4.5
+    assert _ini(tmp_path, "is_viscid = \nno").is_viscid == 0
+
+
 def test_ini_missing_file(tmp_path):
     with pytest.raises(SphnjError) as e:
         params_from_ini(tmp_path / "nope.ini")

@@ -19,8 +19,9 @@ namespace sphnj {
 
 constexpr unsigned kStOutOfGrid = 1u, kStCellOver64 = 2u, kStNaN = 4u, kStRefListLeft = 8u;
 
-__device__ __forceinline__ int cxy_pack(int cx, int cy) { return (cx << 16) | cy; }
-__device__ __forceinline__ int cxy_x(int p) { return p >> 16; }
+// Packed as unsigned halves: a sub-column reaches 2 nx - 1 <= 65533, past the sign bit.
+__device__ __forceinline__ int cxy_pack(int cx, int cy) { return (int)(((unsigned)cx << 16) | (unsigned)cy); }
+__device__ __forceinline__ int cxy_x(int p) { return (int)((unsigned)p >> 16); }
 __device__ __forceinline__ int cxy_y(int p) { return p & 0xffff; }
 
 // Position view of a particle: M32 stores float offsets from the origin of the particle's own

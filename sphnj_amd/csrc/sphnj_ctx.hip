@@ -1140,7 +1140,7 @@ struct Engine : EngineBase {
           (rc = d2h(rg.data(), d.w_range + cp.cell_off, cp.ncell)))
         return rc;
       HIPC(hipStreamSynchronize(stream));
-      for (int t = 0; t < n; ++t) k[t] = cp.cell_off + (cxy[t] >> 16) * cp.ny + (cxy[t] & 0xffff);
+      for (int t = 0; t < n; ++t) k[t] = cp.cell_off + (int)((uint32_t)cxy[t] >> 16) * cp.ny + (cxy[t] & 0xffff);
     }
     HIPC(hipStreamSynchronize(stream));
     for (int t = 0; t < n; ++t) {

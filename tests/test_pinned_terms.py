@@ -1,0 +1,458 @@
+"""Closed forms for the pair, shift, viscosity, coupling and wall-phase terms.
+
+Each test builds a configuration of a few particles whose sums have one or two terms (or a
+brute-force O(N^2) sum over a few dozen particles, with no cell machinery), writes the expected
+result from the reference's equations, and checks it on every backend:
+
+* the FP64 oracle (CPU) -- this is what pins the oracle;
+* libsphnj at P64 and M32 (`-m gpu`) -- one GPU stage against the same numbers.
+
+The configurations are chosen so that a plausible slip fails: unequal densities separate the
+two density-weighted averages, unequal pressures the pressure-jump term, both regimes of
+min(3U, c) and a receding pair the Riemann dissipation, unequal phase viscosities the phase
+index, and so on.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import case, fluid, lattice, params, solid, wall
+from sphnj_amd import _abi as A
+
+C3, C32 = 5.551921, 2.120729
+
+BACKENDS = ["oracle", pytest.param("p64", marks=pytest.mark.gpu), pytest.param("m32", marks=pytest.mark.gpu)]
+TOL = {"oracle": 1e-10, "p64": 1e-9, "m32": 2e-5}
+
+
+@pytest.fixture(params=BACKENDS)
+def backend(request):
+    return request.param
+
+
+def run(backend, c):
+    """A context for the case on the backend (the oracle, or libsphnj at a precision)."""
+    if backend == "oracle":
+        from oracle.oracle_py import Oracle
+
+        return Oracle(c)
+    from sphnj_amd import Simulation
+
+    return Simulation(c, backend)
+
+
+def kernel(p):
+    """(h, rc^2, gradient weight w'(r^2), Shepard weight w(r^2)) from Kernel::initiate
+    (src/kernel.cpp:25-41) with R = 3."""
+    h, d = p.smoothing_length, p.particle_diameter
+    r3 = (h / d) ** -3
+    c32 = p.kernel_2d_coef or C32
+    kdc = 2.0 / C3 * r3 / (h * h) * c32
+    k2c = 1.0 / C3 * r3 * c32
+    return (h, 9.0 * h * h, lambda r2: np.exp(-np.asarray(r2) / (h * h)) * kdc,
+            lambda r2: (np.exp(-np.asarray(r2) / (h * h)) - math.exp(-9.0)) * k2c)
+
+
+def close(got, want, tol, what):
+    got, want = np.asarray(got, float), np.asarray(want, float)
+    scale = max(np.abs(want).max(), 1e-300)
+    err = np.abs(got - want).max() / scale
+    assert err <= tol, f"{what}: got {got}, want {want} (rel {err:.2e} > {tol})"
+
+
+# ---- Riemann pair: p* and v* (src/fluid.cu:44-57) -------------------------------------------
+
+@pytest.mark.parametrize("U,p0", [(2.0, 0.0), (2.0, 1500.0), (10.0, 1500.0), (0.0, 800.0), (-2.0, 1500.0),
+                                  (-10.0, 0.0)])
+def test_head_on_pair_dissipation(backend, U, p0):
+    """Two fluid particles r apart on the x axis, equal rho and p, closing at speed U (U < 0
+    recedes).  The interface pressure is p0 + (rho U / 2) min(3U, c) while they close and p0
+    while they recede; the interface velocity is their mean (0), so each density rate is
+    rho U r w' and the accelerations are -+2 p* r w' / rho."""
+    c, r, rho, dt = 20.0, 0.03, 1000.0, 1e-5
+    p = params(gravity=0.0, dt=dt, liquid_sound_speed=c, shift_per_step=0)
+    f = fluid([0.5, 0.5 + r], [0.5, 0.5], u=[U / 2, -U / 2], rho=rho, p=p0)
+    s = run(backend, case(p, f))
+    s.stage("cells")
+    s.stage("fluid")
+    g = s.download()["fluid"]
+    _, _, wg, _ = kernel(p)
+    w = wg(r * r)
+    ps = p0 + (0.5 * rho * U * min(3.0 * U, c) if U >= 0 else 0.0)
+    a = -2.0 * ps / rho * r * w
+    tol = TOL[backend]
+    close((g["u"] - f["u"]) / dt, [a, -a], tol, "a_x")
+    close(g["v"], [0.0, 0.0], 0, "a_y")
+    if U != 0.0:
+        close((g["rho"] - rho) / dt, [rho * U * r * w] * 2, tol, "Drho")
+    else:
+        assert np.all(g["rho"] == rho)
+    # L = (u_j - u_i) dx^T w': the xx entry is -U r w' on both sides
+    close(g["du_dx"][:, 0], [-U * r * w] * 2, tol if U else 0, "L_xx")
+    close(g["du_dx"][:, 1:], np.zeros((2, 3)), 0, "L other entries")
+
+
+@pytest.mark.parametrize("U", [0.0, 0.5, -0.5, 9.0])
+def test_unequal_pair_star_state(backend, U):
+    """Unequal densities and pressures.  The interface pressure and velocity,
+        p* = (rho_i p_j + rho_j p_i) / (rho_i + rho_j) [+ rho_i rho_j U / (rho_i + rho_j) min(3U, c)],
+        v* = (rho_i u_i + rho_j u_j) / (rho_i + rho_j) + (p_i - p_j) / ((rho_i + rho_j) c) e,
+    are the same seen from either particle; a_k = -2 p* dx_k w' / rho_k and
+    Drho_k = 2 rho_k (u_k - v*).dx_k w'.  (At rest this is the pressure-jump term alone:
+    Drho_i = -2 rho_i (p_i - p_j) r w' / ((rho_i + rho_j) c).)"""
+    c, r, dt = 20.0, 0.025, 1e-6
+    ri, rj, pi_, pj = 1000.0, 1100.0, 3000.0, 1000.0
+    p = params(gravity=0.0, dt=dt, liquid_sound_speed=c, shift_per_step=0)
+    f = fluid([0.5, 0.5 + r], [0.5, 0.5], u=[U, 0.0], rho=[ri, rj], p=[pi_, pj])
+    s = run(backend, case(p, f))
+    s.stage("cells")
+    s.stage("fluid")
+    g = s.download()["fluid"]
+    _, _, wg, _ = kernel(p)
+    w = wg(r * r)
+    ps = (ri * pj + rj * pi_) / (ri + rj)
+    if U >= 0:
+        ps += ri * rj * U / (ri + rj) * min(3.0 * U, c)
+    vs = (ri * U + rj * 0.0) / (ri + rj) + (pi_ - pj) / ((ri + rj) * c)
+    tol = TOL[backend]
+    close((g["u"] - f["u"]) / dt, [-2 * ps / ri * r * w, 2 * ps / rj * r * w], tol, "a_x")
+    close((g["rho"] - f["rho"]) / dt, [2 * ri * (U - vs) * r * w, 2 * rj * (0.0 - vs) * (-r) * w], tol, "Drho")
+    if U == 0.0:
+        close((g["rho"][0] - ri) / dt, -2 * ri * (pi_ - pj) * r * w / ((ri + rj) * c), tol, "pressure jump")
+
+
+# ---- fluid viscosity (src/fluid.cu:75-79) ----------------------------------------------------
+
+@pytest.mark.parametrize("viscid", [0, 1])
+def test_fluid_viscous_term(backend, viscid):
+    """A liquid particle and a gas particle separating along x at speed U, both at p = 0, so
+    the pressure terms vanish and only the viscous term 8 mu_phase(i) ((u_j - u_i).dx) /
+    (|dx|^2 + 0.01 h^2) / rho_i dx w' acts -- with each particle's own phase viscosity.
+    A third pair slides past each other (relative velocity normal to dx): no viscous force."""
+    muL, muG, r, U, V, dt, rho = 0.5, 2.0, 0.03, 0.8, 0.6, 1e-6, 1000.0
+    p = params(gravity=0.0, dt=dt, is_viscid=viscid, liquid_viscosity=muL, gas_viscosity=muG, gas_density=rho,
+               gas_sound_speed=20.0, shift_per_step=0)
+    f = fluid([0.3, 0.3 + r, 0.7, 0.7 + r], [0.3, 0.3, 0.7, 0.7], u=[0.0, U, 0.0, 0.0], v=[0.0, 0.0, 0.0, V],
+              rho=rho, phase=[A.LIQUID, A.GAS, A.LIQUID, A.LIQUID])
+    s = run(backend, case(p, f))
+    s.stage("cells")
+    s.stage("fluid")
+    g = s.download()["fluid"]
+    h, _, wg, _ = kernel(p)
+    w = wg(r * r)
+    eps = 0.01 * h * h
+    ai = viscid * 8 * muL * (U * r) / (r * r + eps) / rho * r * w
+    aj = viscid * 8 * muG * ((-U) * (-r)) / (r * r + eps) / rho * (-r) * w
+    tol = TOL[backend]
+    close((g["u"][:2] - f["u"][:2]) / dt, [ai, aj], tol, "viscous a_x")
+    assert np.all(g["v"][:2] == 0.0)
+    # sliding pair: (u_j - u_i).dx = 0, and p* = 0 (p = 0, du = 0): no acceleration
+    close((g["u"][2:] - f["u"][2:]) / dt, [0.0, 0.0], 0, "sliding a_x")
+    close((g["v"][2:] - f["v"][2:]) / dt, [0.0, 0.0], 0, "sliding a_y")
+
+
+# ---- particle shifting (src/fluid.cu:316-459) ------------------------------------------------
+
+def _shift_scene(fast, K):
+    """A dense liquid block (spacing 0.9 d, so its bulk colour passes 2.0), gas above it, an
+    isolated liquid particle, a wall row below and three solids beside it."""
+    rng = np.random.default_rng(17)
+    p = params(gravity=0.0, dt=1e-4, liquid_sound_speed=20.0, gas_sound_speed=15.0, gas_density=1000.0,
+               particle_shift_coef=0.7, shift_per_step=K)
+    s = 0.018
+    bx, by = lattice(8, 6, s, 0.3, 0.3)
+    gx, gy = lattice(8, 1, s, 0.3, 0.3 + 6 * s)
+    x = np.concatenate([bx, gx, [0.8]])
+    y = np.concatenate([by, gy, [0.8]])
+    phase = np.concatenate([np.zeros(bx.size, np.int32), np.ones(gx.size, np.int32), [0]]).astype(np.int32)
+    u = rng.normal(0, 0.3, x.size)
+    v = rng.normal(0, 0.3, x.size)
+    if fast:
+        u[5] = 5.0  # above the cap 0.1 min(c_L, c_G) = 1.5
+    pr = rng.uniform(0, 500, x.size)
+    wx, wy = lattice(10, 1, s, 0.28, 0.3 - 2 * s)
+    sx, sy = np.array([0.3 + 8 * s + 0.01] * 3), 0.3 + s * np.array([1.0, 2.0, 3.0])
+    return p, case(p, fluid(x, y, u, v, p=pr, phase=phase), s=solid(sx, sy), w=wall(wx, wy))
+
+
+def _classify(phase, col, near):
+    """shiftDisplacementCalculate's class (src/fluid.cu:376-413)."""
+    cls = np.full(phase.size, A.INNER)
+    liq = phase == A.LIQUID
+    low = liq & (col < 1.5)
+    cls[low & (near == A.NO_PARTICLE)] = A.SPLASH
+    cls[low & (near != A.NO_PARTICLE)] = A.FREE_SURFACE
+    cls[liq & (col >= 1.5) & (col < 2.0) & (near == A.HAS_NEAR_PARTICLE)] = A.VICINITY
+    return cls
+
+
+@pytest.mark.parametrize("fast", [False, True])
+@pytest.mark.parametrize("K", [1, 2])
+def test_shift_passes_brute_force(backend, fast, K):
+    """After a pair pass, K shift passes computed by brute force over all species (no cells):
+    G_i = sum dx w' over fluid (current positions), solid and wall particles in range;
+    delta = -C h dt G; SPLASH: 0; FREE_SURFACE / VICINITY: (I - n n^T) delta with n = G/|G|
+    (0 when shorter than 0.1); scaled by U_max = min(max |u|, 0.1 min(c_L, c_G)) taken once
+    before the passes; x += delta', u += L delta'."""
+    p, c = _shift_scene(fast, K)
+    s = run(backend, c)
+    for st in ("cells", "couple", "wall", "fluid"):
+        s.stage(st)
+    a = s.download()
+    s.stage("shift")
+    b = s.download()["fluid"]
+    fa = a["fluid"]
+    h, rc2, wg, _ = kernel(p)
+    umax = min(np.hypot(fa["u"], fa["v"]).max(), 0.1 * min(20.0, 15.0))
+    assert (umax == 1.5) == fast
+    cls = _classify(c.fluid["phase"], fa["color"], fa["near"])
+    assert set(cls.tolist()) == {A.SPLASH, A.FREE_SURFACE, A.VICINITY, A.INNER}
+    L = fa["du_dx"].reshape(-1, 2, 2)
+    x = np.stack([fa["x"], fa["y"]], 1)
+    u = np.stack([fa["u"], fa["v"]], 1)
+    others = np.concatenate([np.stack([a["solid"]["x"], a["solid"]["y"]], 1),
+                             np.stack([c.wall["x"], c.wall["y"]], 1)])
+    moves = []
+    for _ in range(K):
+        pts = np.concatenate([x, others])
+        d = pts[None, :, :] - x[:, None, :]
+        r2 = (d * d).sum(-1)
+        G = (d * (wg(r2) * (r2 < rc2))[..., None]).sum(1)
+        delta = -p.particle_shift_coef * h * p.dt * G
+        gn = np.linalg.norm(G, axis=1)
+        n = np.where(gn[:, None] > 0, G / np.where(gn > 0, gn, 1)[:, None], 0.0)
+        n[np.linalg.norm(n, axis=1) < 0.1] = 0.0
+        proj = (cls == A.FREE_SURFACE) | (cls == A.VICINITY)
+        delta[proj] -= n[proj] * (n[proj] * delta[proj]).sum(1)[:, None]
+        delta[cls == A.SPLASH] = 0.0
+        delta *= umax
+        moves.append((delta, G, proj))
+        x = x + delta
+        u = u + np.einsum("nij,nj->ni", L, delta)
+    tol = TOL[backend]
+    disp = np.stack([b["x"] - fa["x"], b["y"] - fa["y"]], 1)
+    close(disp, x - np.stack([fa["x"], fa["y"]], 1), tol, "shift displacement")
+    close(np.stack([b["u"] - fa["u"], b["v"] - fa["v"]], 1), u - np.stack([fa["u"], fa["v"]], 1), tol,
+          "Taylor correction")
+    assert np.array_equal(b["boundary_type"], cls)
+    if K == 1:
+        delta, G, proj = moves[0]
+        # the normal is the direction of G itself, and delta is parallel to G, so the
+        # projection (I - n n^T) delta leaves only rounding: surface particles do not shift
+        scale = np.abs(disp).max()
+        assert np.abs(disp[proj]).max() <= (1e-12 if backend != "m32" else 1e-6) * scale
+        # unprojected shifts point down the gradient
+        inner = (cls == A.INNER) & (np.linalg.norm(G, axis=1) > 0)
+        cosv = -(disp[inner] * G[inner]).sum(1) / (np.linalg.norm(disp[inner], axis=1) * np.linalg.norm(G[inner], axis=1))
+        assert cosv.min() > 1 - 1e-9
+        assert np.all(disp[cls == A.SPLASH] == 0.0)
+        # u_new - u_old = L (x_new - x_old)
+        close(np.stack([b["u"] - fa["u"], b["v"] - fa["v"]], 1), np.einsum("nij,nj->ni", L, disp), tol, "u - L dx")
+
+
+# ---- solid artificial viscosity (src/solid.cu:286-304) ---------------------------------------
+
+def test_solid_artificial_viscosity(backend):
+    """An undeformed 7 x 7 solid block (x = X, so F = I and P = 0) in which only the centre
+    particle k moves, at V.  Each of k's neighbours i then feels one term,
+    pi(nu) F_i L_i dX_ik w'(|dX_ik|^2) with nu = h (V.dx_ik) / (|dx_ik|^2 + 0.01 h^2) and
+    pi = alpha nu c_s - beta nu^2 (no division by the density), and k the sum of its
+    neighbours' terms with -V; the rest feel nothing."""
+    E, nu_p, rho_s, al, be = 1.0e3, 0.3, 1000.0, 1.5, 4.0
+    p = params(gravity=0.0, dt=1e-9, solid_sub_step=1, youngs_modulus=E, poisson_ratio=nu_p, solid_density=rho_s,
+               arti_vis_alpha=al, arti_vis_beta=be)
+    X, Y = lattice(7, 7, 0.02, 0.4, 0.4)
+    k = 24
+    V = np.array([0.7, -0.4])
+    u = np.zeros(49)
+    v = np.zeros(49)
+    u[k], v[k] = V
+    c = case(p, s=solid(X, Y, u=u, v=v))
+    s = run(backend, c)
+    s.stage("cells")
+    s.stage("solid")
+    g = s.download()["solid"]
+    h, rc2, wg, _ = kernel(p)
+    cs = math.sqrt(E / (3 * (1 - 2 * nu_p)) / rho_s)
+    L = g["correction"].reshape(-1, 2, 2)
+    P = np.stack([X, Y], 1)
+    want = np.zeros((49, 2))
+
+    def term(i, j, du):
+        dx = P[j] - P[i]
+        r2 = dx @ dx
+        nu = h * (du @ dx) / (r2 + 0.01 * h * h)
+        return (al * nu * cs - be * nu * nu) * (L[i] @ dx) * wg(r2)
+
+    for i in range(49):
+        if i == k:
+            continue
+        if ((P[k] - P[i]) ** 2).sum() < rc2:
+            want[i] = term(i, k, V)
+            want[k] += term(k, i, -V)
+    assert np.count_nonzero(np.abs(want).sum(1)) == 45     # the 4 corners are out of k's range
+    tol = TOL[backend] if backend != "m32" else 1e-4
+    close(g["du_dt"], want, tol, "viscous du_dt")
+
+
+# ---- solid-fluid coupling (src/solid.cu:211-255, 355-444; src/fluid.cu:82-134) ---------------
+
+@pytest.mark.parametrize("material", [A.LINEAR_ELASTIC, A.AUXETIC])
+@pytest.mark.parametrize("Us", [3.0, 12.0, -3.0])
+def test_solid_fluid_coupling_force(backend, material, Us):
+    """One solid with one fluid neighbour.  The solid's pressure is the Shepard average of
+    that one fluid particle's pressure; its coupling acceleration is -2 p* dx w' / rho_ref with
+    the fluid's density on both sides, p* = (p_f + p_s)/2 + (rho_f U / 2) min(3U, c_f) while
+    closing at speed U, and rho_ref the material's reference density.  A lone solid has F = 0,
+    so P = 0 and its acceleration is the coupling alone."""
+    c, r, rho_f, p_f = 20.0, 0.03, 1010.0, 2500.0
+    p = params(gravity=0.0, dt=1e-9, liquid_sound_speed=c, solid_density=2000.0, auxetic_density=700.0,
+               shift_per_step=0)
+    cs_ = case(p, fluid([0.5 + r], [0.5], rho=rho_f, p=p_f), s=solid([0.5], [0.5], u=[Us], material=material))
+    s = run(backend, cs_)
+    s.stage("cells")
+    s.stage("couple")
+    ps_s = s.download()["solid"]["pressure"][0]
+    tol = TOL[backend]
+    close(ps_s, p_f, tol, "solid pressure")
+    s.stage("solid")
+    g = s.download()["solid"]
+    _, _, wg, _ = kernel(p)
+    ps = 0.5 * (p_f + p_f) + (0.5 * rho_f * Us * min(3 * Us, c) if Us >= 0 else 0.0)
+    rho_ref = 2000.0 if material == A.LINEAR_ELASTIC else 700.0
+    want = [-2 * ps / rho_ref * r * wg(r * r), 0.0]
+    close(g["couple_dudt"][0], want, tol, "couple_dudt")
+    close(g["du_dt"][0], want, tol, "du_dt")
+
+
+def test_solid_pressure_and_normal(backend):
+    """Solid pressure = max(Shepard average of fluid pressures, p_min), 0 with no fluid in
+    range; solid normal = normalize(-sum over solids dx w'), 0 when shorter than 0.1."""
+    p = params(gravity=0.0, dt=1e-9, shift_per_step=0)
+    fx = [0.53, 0.45, 0.22]
+    fy = [0.5, 0.5, 0.8]
+    fp = [1000.0, 4000.0, -2.0e5]
+    sx = [0.5, 0.2, 0.8, 0.8, 0.82, 0.3, 0.3 + 0.077]
+    sy = [0.5, 0.8, 0.2, 0.8, 0.8, 0.2, 0.2]
+    c = case(p, fluid(fx, fy, p=fp), s=solid(sx, sy))
+    s = run(backend, c)
+    s.stage("cells")
+    s.stage("couple")
+    g = s.download()["solid"]
+    _, _, wg, ws = kernel(p)
+    w1, w2 = ws(0.03 ** 2), ws(0.05 ** 2)
+    tol = TOL[backend]
+    close(g["pressure"][0], (1000 * w1 + 4000 * w2) / (w1 + w2), tol, "Shepard pressure")
+    close(g["pressure"][1], -1.013e5, tol, "p_min floor")
+    assert g["pressure"][2] == 0.0 and g["pressure"][5] == 0.0
+    close(g["normal"][3], [-1.0, 0.0], tol, "normal of a pair, left")
+    close(g["normal"][4], [1.0, 0.0], tol, "normal of a pair, right")
+    assert 0.077 * wg(0.077 ** 2) < 0.1                 # the far pair's sums are too short
+    assert np.all(g["normal"][5:] == 0.0) and np.all(g["normal"][:3] == 0.0)
+
+
+def test_fluid_sees_solid_with_its_own_density(backend):
+    """Fluid f beside solid s (whose pressure mixes f and a second fluid out of f's range):
+    rho_j = rho_i, p_j = p_s, u_j = u_s; the solid adds to neither the colour nor the near flag."""
+    c, r = 20.0, 0.03
+    p = params(gravity=0.0, dt=1e-6, liquid_sound_speed=c, shift_per_step=0)
+    rf, pf, pf2 = 1050.0, 800.0, 3000.0
+    f = fluid([0.5, 0.5 + r + 0.05], [0.5, 0.5], u=[0.4, 0.0], rho=[rf, 1000.0], p=[pf, pf2])
+    sv = solid([0.5 + r], [0.5], u=[-1.0], v=[0.3])
+    cs_ = case(p, f, s=sv)
+    s = run(backend, cs_)
+    s.stage("cells")
+    s.stage("couple")
+    p_s = s.download()["solid"]["pressure"][0]
+    _, _, wg, ws = kernel(p)
+    wa, wb = ws(r * r), ws(0.05 ** 2)
+    tol = TOL[backend]
+    close(p_s, (pf * wa + pf2 * wb) / (wa + wb), tol, "solid pressure")
+    s.stage("fluid")
+    g = s.download()["fluid"]
+    w = wg(r * r)
+    ui, us = np.array([0.4, 0.0]), np.array([-1.0, 0.3])
+    e = np.array([1.0, 0.0])
+    du = (ui - us) @ e
+    ps = (rf * p_s + rf * pf) / (2 * rf) + (rf * rf * du / (2 * rf) * min(3 * du, c) if du >= 0 else 0.0)
+    vs = (ui + us) / 2 + (pf - p_s) / (2 * rf * c) * e
+    dx = r * e
+    a = -2 * ps / rf * dx * w
+    close([(g["u"][0] - 0.4) / p.dt, g["v"][0] / p.dt], a, tol, "a")
+    close((g["rho"][0] - rf) / p.dt, 2 * rf * ((ui - vs) @ dx) * w, tol, "Drho")
+    assert g["color"][0] == 0.0 and g["near"][0] == A.NO_PARTICLE
+    close(g["du_dx"][0].reshape(2, 2), np.outer(us - ui, dx) * w, tol, "L")
+
+
+# ---- wall phase and density (src/virt.cu:85-89, 143-148) -------------------------------------
+
+@pytest.mark.parametrize("left,right", [(A.GAS, A.LIQUID), (A.LIQUID, A.GAS)])
+def test_wall_phase_nearest_and_ties(backend, left, right):
+    """A wall takes the phase of its nearest fluid particle (the first strict minimum of r^2 in
+    the scan order of the 9 cells, a cell's fluids in (quadrant, original index) order), else
+    LIQUID; its density is rho(p_w) of that phase."""
+    p = params(gravity=0.0, dt=1e-9, gas_density=2.0, shift_per_step=0)
+    e = 2.0 ** -6
+    # wall 0: tie between x = 0.5 -+ e; the right particle comes first in original order
+    # wall 1: a gas particle nearer than a liquid one; wall 2: the liquid one nearer
+    # wall 3: no fluid in range (the only fluid near it is 0.1 away)
+    fx = [0.5 + e, 0.5 - e, 0.2 + 0.03, 0.2 - 0.02, 0.8 + 0.02, 0.8 - 0.035, 0.2]
+    fy = [0.5, 0.5, 0.2, 0.2, 0.2, 0.2, 0.9]
+    ph = [right, left, A.LIQUID, A.GAS, A.LIQUID, A.GAS, A.GAS]
+    c = case(p, fluid(fx, fy, phase=ph), w=wall([0.5, 0.2, 0.8, 0.2], [0.5, 0.2, 0.2, 0.8]))
+    s = run(backend, c)
+    s.stage("cells")
+    s.stage("wall")
+    w = s.download()["wall"]
+    assert list(w["phase"]) == [left, A.GAS, A.LIQUID, A.LIQUID]
+    rho0 = {A.LIQUID: 1000.0, A.GAS: 2.0}
+    close(w["rho"], [rho0[int(k)] for k in w["phase"]], TOL[backend], "wall density at p = 0")
+    assert w["p"][3] == 0.0
+
+
+def test_fluid_sees_wall_with_the_wall_phase_density(backend):
+    """A liquid particle beside a wall whose nearest fluid is gas: the wall enters its pair
+    terms with rho_j = rho0[GAS], p_j = p_w, u_j = u_w (src/fluid.cu:136-170)."""
+    c = 20.0
+    p = params(gravity=0.0, dt=1e-6, gas_density=10.0, gas_sound_speed=c, liquid_sound_speed=c, shift_per_step=0)
+    xl = np.array([0.53, 0.5])
+    xg = np.array([0.5, 0.51])
+    xw = np.array([0.5, 0.5])
+    f = fluid([xl[0], xg[0]], [xl[1], xg[1]], u=[0.3, -0.2], v=[0.0, 0.1], rho=[1000.0, 10.0],
+              p=[600.0, 200.0], phase=[A.LIQUID, A.GAS])
+    cs_ = case(p, f, w=wall([xw[0]], [xw[1]], bc=A.NO_SLIP))
+    s = run(backend, cs_)
+    s.stage("cells")
+    s.stage("wall")
+    wd = s.download()["wall"]
+    assert wd["phase"][0] == A.GAS
+    _, _, wg, ws = kernel(p)
+    wl, wgs = ws(((xl - xw) ** 2).sum()), ws(((xg - xw) ** 2).sum())
+    pw = (600.0 * wl + 200.0 * wgs) / (wl + wgs)
+    uw = -(np.array([0.3, 0.0]) * wl + np.array([-0.2, 0.1]) * wgs) / (wl + wgs)   # NO_SLIP reverses
+    tol = TOL[backend]
+    close(wd["p"][0], pw, tol, "p_w")
+    close([wd["u"][0], wd["v"][0]], uw, tol, "u_w")
+    s.stage("fluid")
+    g = s.download()["fluid"]
+
+    def pair(xi, ui, ri, pi_, xj, uj, rj, pj):
+        dx = xj - xi
+        r2 = dx @ dx
+        e = dx / math.sqrt(r2)
+        w = wg(r2)
+        ps = (ri * pj + rj * pi_) / (ri + rj)
+        du = (ui - uj) @ e
+        if du >= 0:
+            ps += ri * rj * du / (ri + rj) * min(3 * du, c)
+        vs = (ri * ui + rj * uj) / (ri + rj) + (pi_ - pj) / ((ri + rj) * c) * e
+        return -2 * ps / ri * dx * w, 2 * ri * ((ui - vs) @ dx) * w
+
+    ul, ug = np.array([0.3, 0.0]), np.array([-0.2, 0.1])
+    a1, d1 = pair(xl, ul, 1000.0, 600.0, xg, ug, 10.0, 200.0)
+    a2, d2 = pair(xl, ul, 1000.0, 600.0, xw, uw, 10.0, pw)
+    close([(g["u"][0] - 0.3) / p.dt, g["v"][0] / p.dt], a1 + a2, tol, "liquid a")
+    close((g["rho"][0] - 1000.0) / p.dt, d1 + d2, tol, "liquid Drho")
+    # the liquid particle ignores both gas-phase neighbours in its colour
+    assert g["color"][0] == 0.0

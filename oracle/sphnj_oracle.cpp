@@ -594,10 +594,14 @@ double plastic_elastic(const Consts &k, double strain, double *max_strain) {
   return -(k.harden_coef * e * e + k.yield_stress);
 }
 
-// solidStressCalculation (src/solid.cu:131-167) + the neo-Hookean extension.
-M2 stress(const Consts &k, const M2 &F, M2 *M, int mat) {
-  const M2 Et = mul(tr(F), F);
-  const M2 E = {0.5 * (Et.a - 1.0), 0.5 * Et.b, 0.5 * Et.c, 0.5 * (Et.d - 1.0)};
+// solidStressCalculation (src/solid.cu:131-167) + the neo-Hookean extension, for F = I + H.
+// The Green strain E = (F^T F - I) / 2 is formed as (H + H^T + H^T H) / 2 and J - 1 as
+// tr H + det H: the same values without the cancellation of F^T F - I, which would otherwise
+// bury a stiff solid's small strains (|H| ~ 1e-6) under rounding.
+M2 stress(const Consts &k, const M2 &H, M2 *M, int mat) {
+  const M2 F = {1.0 + H.a, H.b, H.c, 1.0 + H.d};
+  const M2 HtH = mul(tr(H), H);
+  const M2 E = {H.a + 0.5 * HtH.a, 0.5 * (H.b + H.c + HtH.b), 0.5 * (H.c + H.b + HtH.c), H.d + 0.5 * HtH.d};
   if (mat == SPHNJ_LINEAR_ELASTIC) {  // linearStressCalculation: St Venant-Kirchhoff
     const double trE = E.a + E.d;
     const M2 S = {2.0 * k.mu_s * E.a + k.lambda * trE, 2.0 * k.mu_s * E.b, 2.0 * k.mu_s * E.c,
@@ -619,9 +623,8 @@ M2 stress(const Consts &k, const M2 &F, M2 *M, int mat) {
     return mul(mul(mul(F, R), Sg), Ri);
   }
   if (mat == SPHNJ_NEO_HOOKEAN) {  // P = mu (F - F^-T) + lambda ln(J) F^-T
-    const double J = det(F);
     const M2 FiT = tr(inv(F));
-    const double lj = std::log(J);
+    const double lj = std::log1p(H.a + H.d + det(H));  // ln J, J - 1 = tr H + det H
     return {k.mu_s * (F.a - FiT.a) + k.lambda * lj * FiT.a, k.mu_s * (F.b - FiT.b) + k.lambda * lj * FiT.b,
             k.mu_s * (F.c - FiT.c) + k.lambda * lj * FiT.c, k.mu_s * (F.d - FiT.d) + k.lambda * lj * FiT.d};
   }
@@ -686,25 +689,32 @@ void solid_substeps(Case &c, unsigned *flags) {
   for (int sub = 0; sub < k.S; ++sub) {
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) {
-      M2 Ft = {0, 0, 0, 0}, B = {0, 0, 0, 0};
+      // F~ = sum (x_j - x_i) dX^T w' splits as B + D with B = sum dX dX^T w' and
+      // D = sum (d_j - d_i) dX^T w', d = x - X the displacement; with L = B^-1 the deformation
+      // gradient F~ L is I + D L, and with det B = 0 (L = I) it is B + D.  H = F - I is kept
+      // apart from the identity (see stress()).
+      M2 D = {0, 0, 0, 0}, B = {0, 0, 0, 0};
+      const double dix = c.sx[i] - c.sX[i], diy = c.sy[i] - c.sY[i];
       for (int e = c.ref_off[i]; e < c.ref_off[i + 1]; ++e) {
         const int j = c.ref_list[e];
         const double dX = c.sX[j] - c.sX[i], dY = c.sY[j] - c.sY[i];
         const double w = grad_w(k, dX * dX + dY * dY);
-        const double dx = c.sx[j] - c.sx[i], dy = c.sy[j] - c.sy[i];
-        Ft.a += dx * dX * w;
-        Ft.b += dx * dY * w;
-        Ft.c += dy * dX * w;
-        Ft.d += dy * dY * w;
+        const double ux = (c.sx[j] - c.sX[j]) - dix, uy = (c.sy[j] - c.sY[j]) - diy;
+        D.a += ux * dX * w;
+        D.b += ux * dY * w;
+        D.c += uy * dX * w;
+        D.d += uy * dY * w;
         B.a += dX * dX * w;
         B.b += dX * dY * w;
         B.c += dY * dX * w;
         B.d += dY * dY * w;
       }
-      const M2 L = det(B) != 0.0 ? inv(B) : M2{1, 0, 0, 1};
-      const M2 F = mul(Ft, L);
+      const bool degenerate = det(B) == 0.0;
+      const M2 L = degenerate ? M2{1, 0, 0, 1} : inv(B);
+      const M2 H = degenerate ? M2{B.a + D.a - 1.0, B.b + D.b, B.c + D.c, B.d + D.d - 1.0} : mul(D, L);
+      const M2 F = {1.0 + H.a, H.b, H.c, 1.0 + H.d};
       M2 Ms = {c.sM[4 * i], c.sM[4 * i + 1], c.sM[4 * i + 2], c.sM[4 * i + 3]};
-      const M2 P = stress(k, F, &Ms, c.smat[i]);
+      const M2 P = stress(k, H, &Ms, c.smat[i]);
       const double *fp = &F.a, *lp = &L.a, *pp = &P.a, *mp = &Ms.a;
       for (int t = 0; t < 4; ++t) {
         c.sF[4 * i + t] = fp[t];

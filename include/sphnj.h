@@ -74,9 +74,14 @@ enum {
   SPHNJ_ST_CELL_OVER_64 = 1u << 1,  /* some cell holds > 64 particles: the reference's fixed
                                        CELL_SIZE list would have overflowed (include/global.h:9) */
   SPHNJ_ST_NAN = 1u << 2,           /* a NaN appeared in a fluid or solid state field         */
-  SPHNJ_ST_REFLIST_LEFT = 1u << 3   /* a solid reference-configuration neighbour left the 3x3
+  SPHNJ_ST_REFLIST_LEFT = 1u << 3,  /* a solid reference-configuration neighbour left the 3x3
                                        current cells: the reference (src/solid.cu:184-199)
                                        would have dropped that pair                           */
+  SPHNJ_ST_SYNC_TIMEOUT = 1u << 4   /* a grid barrier of the one-launch solid sub-steps gave up
+                                       waiting (the device did not hold every block at once,
+                                       e.g. under heavy concurrent use): that step's solid
+                                       state is invalid; sphnj_set_persistent_solid(ctx, 0)
+                                       selects the 2 S launch form                           */
 };
 
 /* ---- parameters: POD mirror of Parameter (include/parameter.h:17-49) ------------------
@@ -223,6 +228,10 @@ void sphnj_destroy(sphnj_ctx *ctx);
 /* ---- execution options ---------------------------------------------------------------- */
 /* Capture the step into hipGraphs (default on) and replay them. */
 int sphnj_set_graph(sphnj_ctx *ctx, int enable);
+/* Run a fluid step's S solid sub-steps (solidCalculation, src/solid.cu:473-492) as one launch
+ * whose blocks meet at grid barriers between the deformation and force phases (default on),
+ * or as 2 S launches.  Both give the same results bit for bit. */
+int sphnj_set_persistent_solid(sphnj_ctx *ctx, int enable);
 /* Autotuner: times the fluid pair kernel (family 0) and the shift kernel (family 1) at block
  * sizes 64, 128 and 256 -- a block owns that many consecutive sorted particles, so the block
  * size also sets the cells each LDS tile spans -- on the current state with HIP events, and

@@ -19,7 +19,7 @@ SPLASH, FREE_SURFACE, VICINITY, INNER = 0, 1, 2, 3
 NO_PARTICLE, HAS_PARTICLE, HAS_NEAR_PARTICLE = 0, 1, 2
 LINEAR_ELASTIC, AUXETIC, NEO_HOOKEAN = 0, 1, 2
 UNCONSTRAINED, FIXED = 0, 1
-ST_OUT_OF_GRID, ST_CELL_OVER_64, ST_NAN, ST_REFLIST_LEFT = 1, 2, 4, 8
+ST_OUT_OF_GRID, ST_CELL_OVER_64, ST_NAN, ST_REFLIST_LEFT, ST_SYNC_TIMEOUT = 1, 2, 4, 8, 16
 STAGE_CELLS, STAGE_COUPLE, STAGE_WALL, STAGE_FLUID, STAGE_SOLID, STAGE_SHIFT = 1, 2, 3, 4, 5, 6
 FLUID, SOLID, WALL = 0, 1, 2
 
@@ -143,6 +143,7 @@ LIB_PROTOS = {
     "sphnj_counts": (C.c_int, [VP, C.c_int32, c_int32_p, c_int32_p, c_int32_p, c_int32_p]),
     "sphnj_last_error": (C.c_char_p, [VP]),
     "sphnj_set_graph": (C.c_int, [VP, C.c_int]),
+    "sphnj_set_persistent_solid": (C.c_int, [VP, C.c_int]),
     "sphnj_autotune": (C.c_int, [VP]),
     "sphnj_autotune_report": (C.c_int, [VP, C.c_int32, c_double_p]),
     "sphnj_get_block": (C.c_int, [VP, C.c_int32, c_int32_p]),

@@ -16,6 +16,9 @@
 //                   air gap margin cfl steps
 //   ensemble_member member `index` of a seeded ensemble of liquid dam breaks.
 //                   seed index layers K steps
+//   ref_dam_break, ref_water_tank, ref_beam_vibrate, ref_circle_entry, ref_beam_dam,
+//   ref_wedge_entry  the reference's six scenarios (cases/<name>), as its generators and
+//                   input.ini files define them.  dx steps (ref_wedge_entry also fixed_v)
 // Initial pressure is hydrostatic p = rho0 g (level - y) (as in
 // cases/water_tank/particle_generate.cpp:96-97), continuous across the air/water surface, and
 // the density is its inverse Tait value rho = rho0 ((p - p_bg) B^-1 + 1)^(1/gamma).
@@ -328,6 +331,322 @@ int ensemble_member(const Opts &o, sphnj_params *P, sphnj_particles *ps) {
   return dam_break(sub, P, ps);
 }
 
+// ---- the reference's own scenarios (cases/<name>/particle_generate.cpp + config/input.ini) -----
+// Each reproduces the reference generator's lattice, loop order (which fixes the original
+// index of every particle), phases, materials, initial fields and boundary flags, and the
+// parameters of its input.ini.  Positions are exact doubles (the reference writes its solid
+// and wall files with 6 significant digits).  `dx` rescales a case (h, and dt at a fixed
+// acoustic CFL number, follow it); `steps` sets iend.  Where a shipped input.ini does not fit
+// its generator, the deviation is named at the generator.
+
+// The parameters every reference input.ini shares: defaults, then the [solid]/[auxetic]
+// block the INIs repeat (auxetic_modulus 2e8 keeps the reference's validation satisfied).
+void ref_base(sphnj_params *P) {
+  sphnj_params_default(P);
+  P->auxetic_density = 636.0;
+  P->auxetic_modulus = 2.0e8;
+  P->auxetic_poisson_ratio = -0.2;
+  P->yield_stress = 0.7e7;
+  P->arti_vis_alpha = 2.5;
+  P->arti_vis_beta = 2.5;
+  P->impact_length_hsml_ratio = 3.0;
+  P->thread_num = 32;
+  P->istart = 0;
+}
+void ref_fluids(sphnj_params *P, double rl, double cl, double rg, double cg) {
+  P->liquid_density = rl;
+  P->liquid_sound_speed = cl;
+  P->liquid_gamma = 7.0;
+  P->liquid_viscosity = 1.01e-3;
+  P->gas_density = rg;
+  P->gas_sound_speed = cg;
+  P->gas_gamma = 1.4;
+  P->gas_viscosity = 17.9e-6;
+}
+void ref_boxes(sphnj_params *P, double cx0, double cx1, double cy0, double cy1, double dx0, double dx1, double dy0,
+               double dy1) {
+  P->cell_xmin = cx0;
+  P->cell_xmax = cx1;
+  P->cell_ymin = cy0;
+  P->cell_ymax = cy1;
+  P->domain_xmin = dx0;
+  P->domain_xmax = dx1;
+  P->domain_ymin = dy0;
+  P->domain_ymax = dy1;
+}
+// h = 1.2 dx (every reference INI); dt scaled from the INI's dt at its own h
+void ref_kernel_time(sphnj_params *P, const Opts &o, double dx_ref, double dt_ref, double h_ref_ini, int iend,
+                     int result, int S, int K) {
+  const double dx = o.d("dx", dx_ref);
+  P->particle_diameter = dx;
+  P->smoothing_length = 1.2 * dx;
+  P->dt = dt_ref * (1.2 * dx) / h_ref_ini;
+  P->iend = (int)o.i("steps", iend);
+  P->result_interval = result;
+  P->sort_every_step = 50;
+  P->solid_sub_step = S;
+  P->shift_per_step = K;
+}
+
+// A case run at a coarser spacing than its reference keeps its wall layers, which can then
+// reach past the INI's cell box: widen the box to hold every particle with a spacing to
+// spare (at the reference spacing every INI box already does, and is kept as it is).
+void fit_cell_box(sphnj_params *P, const sphnj_particles *ps) {
+  const double m = P->particle_diameter;
+  auto widen = [&](const std::vector<double> &x, const std::vector<double> &y) {
+    for (size_t i = 0; i < x.size(); ++i) {
+      P->cell_xmin = std::fmin(P->cell_xmin, x[i] - m);
+      P->cell_xmax = std::fmax(P->cell_xmax, x[i] + m);
+      P->cell_ymin = std::fmin(P->cell_ymin, y[i] - m);
+      P->cell_ymax = std::fmax(P->cell_ymax, y[i] + m);
+    }
+  };
+  widen(ps->fx, ps->fy);
+  widen(ps->sx, ps->sy);
+  widen(ps->wx, ps->wy);
+}
+
+// cases/dam_break: liquid block 2.0 x 1.0 in a 5.366 x 2.5 tank, gas elsewhere, both at
+// p = 0 and rho = rho0; 8 wall layers (particle_generate.cpp:57-113, input.ini).
+int ref_dam_break(const Opts &o, sphnj_params *P, sphnj_particles *ps) {
+  ref_base(P);
+  P->gravity = 1.0;
+  P->particle_shift_coef = 1.0;
+  P->min_pressure = -1.013e5;
+  ref_fluids(P, 1.0, 20.0, 1.0e-3, 20.0);
+  ref_boxes(P, -0.2, 5.57, -0.2, 2.7, 0.0, 5.37, 0.0, 2.5);
+  ref_kernel_time(P, o, 5.0e-3, 5.0e-5, 0.006, 200000, 2000, 1, 10);
+  const double dx = P->particle_diameter, xmax = 5.366, ymax = 2.5;
+  const int L = 8, nx = (int)(xmax / dx), ny = (int)(ymax / dx);
+  for (int i = -L; i < nx + L; ++i)
+    for (int j = -L; j < ny + L; ++j) {
+      const double x = (i + 0.5) * dx, y = (j + 0.5) * dx;
+      if (x < 0.0 || x > xmax || y < 0.0 || y > ymax)
+        add_wall(ps, x, y, SPHNJ_SLIP);
+      else if (x < 2.0 && y < 1.0)
+        add_fluid(ps, x, y, 1.0, 0.0, SPHNJ_LIQUID);
+      else
+        add_fluid(ps, x, y, 1.0e-3, 0.0, SPHNJ_GAS);
+    }
+  if (P->particle_diameter != 5.0e-3) fit_cell_box(P, ps);
+  return SPHNJ_OK;
+}
+
+// cases/water_tank: a 1 x 2 m water column at hydrostatic pressure (rho0 g (2 - y), Tait
+// density at c = 200) on an elastic floor plate 0.05 m thick that spans the side walls, all
+// particles of a column i generated together (particle_generate.cpp:57-111).  The plate's
+// rows are plain 7-column solids (UNCONSTRAINED), as the reference writes them.
+int ref_water_tank(const Opts &o, sphnj_params *P, sphnj_particles *ps) {
+  ref_base(P);
+  P->gravity = 9.81;
+  P->particle_shift_coef = 1.0;
+  P->is_viscid = 1;
+  ref_fluids(P, 1000.0, 200.0, 1.29, 200.0);
+  P->liquid_viscosity = 1.0;
+  P->solid_density = 2700.0;
+  P->youngs_modulus = 67.5e9;
+  P->poisson_ratio = 0.34;
+  ref_boxes(P, -0.1, 1.1, -0.1, 2.1, 0.0, 1.0, 0.0, 2.0);
+  ref_kernel_time(P, o, 2.5e-3, 2.0e-6, 0.003, 500000, 500, 20, 10);
+  const double dx = P->particle_diameter, xmax = 1.0, ymax = 2.0;
+  const int L = 8, nx = (int)std::lround(xmax / dx), ny = (int)std::lround(ymax / dx);
+  const int nys = (int)std::lround(0.05 / dx);
+  const double rho0 = 1000.0, c0 = 200.0;
+  for (int i = -L; i < nx + L; ++i) {
+    for (int j = 0; j < ny + L; ++j) {
+      const double x = (i + 0.5) * dx, y = (j + 0.5) * dx;
+      if (!(y > 0.0)) continue;
+      if (x < 0.0 || x > xmax || y > ymax) {
+        add_wall(ps, x, y, SPHNJ_SLIP);
+      } else {
+        const double p = rho0 * kG * (2.0 - y);
+        add_fluid(ps, x, y, tait_density(p, rho0, c0, 7.0, 0.0), p, SPHNJ_LIQUID);
+      }
+    }
+    for (int j = 0; j < nys; ++j) add_solid(ps, (i + 0.5) * dx, -(j + 0.5) * dx, 0.0, 0.0, SPHNJ_LINEAR_ELASTIC, 0);
+  }
+  if (P->particle_diameter != 2.5e-3) fit_cell_box(P, ps);
+  return SPHNJ_OK;
+}
+
+// cases/beam_vibrate: a free 0.244 x 0.024 m beam (4 columns behind x = 0) with the initial
+// transverse velocity of a cantilever's first mode, v(x) = xi c_s f(x) / f(L), xi = 0.01,
+// c_s = sqrt(K / rho), f the mode shape at kL = 1.875 (particle_generate.cpp:56-96); no fluid,
+// no walls.  The reference writes 7-column solids, so no particle is held.
+int ref_beam_vibrate(const Opts &o, sphnj_params *P, sphnj_particles *ps) {
+  ref_base(P);
+  P->gravity = 0.0;
+  P->particle_shift_coef = 1.0;
+  ref_fluids(P, 1.0, 20.0, 1.0e-3, 20.0);
+  P->solid_density = 1000.0;
+  P->youngs_modulus = 2.0e6;
+  P->poisson_ratio = 0.4;
+  ref_boxes(P, -0.1, 0.3, -0.05, 0.05, 0.0, 0.24, -0.012, 0.012);
+  ref_kernel_time(P, o, 1.0e-3, 1.0e-5, 0.0012, 100000, 100, 1, 1);
+  const double dx = P->particle_diameter, L_beam = 0.24, ymin = -0.012;
+  const int L = 4, nx = (int)std::lround(L_beam / dx), ny = (int)std::lround(0.024 / dx);
+  const double kL = 1.875, k = kL / L_beam;
+  auto mode = [&](double x) {
+    return (std::cos(kL) + std::cosh(kL)) * (std::cosh(k * x) - std::cos(k * x)) +
+           (std::sin(kL) - std::sinh(kL)) * (std::sinh(k * x) - std::sin(k * x));
+  };
+  const double cs = std::sqrt(2.0e6 / (3.0 * (1.0 - 2.0 * 0.4)) / 1000.0);
+  const double fL = mode(L_beam);
+  for (int i = -L; i < nx; ++i)
+    for (int j = 0; j < ny; ++j) {
+      const double x = (i + 0.5) * dx, y = (j + 0.5) * dx + ymin;
+      add_solid(ps, x, y, 0.0, 0.01 * cs * mode(x) / fL, SPHNJ_LINEAR_ELASTIC, 0);
+    }
+  if (P->particle_diameter != 1.0e-3) fit_cell_box(P, ps);
+  return SPHNJ_OK;
+}
+
+// cases/circle_entry: a ring (radius 0.05 m; layers of 4, 10 and 4 dx from the inside out:
+// elastic, auxetic, elastic) falling at 50 m/s into water 0.3 m deep under air, tank 0.3 x
+// 0.5 m (particle_generate.cpp:56-211, with its pi = 3.1415926).  Its input.ini is the wedge's
+// (particle_diameter 0.01, smoothing_length 0.012, dt 1e-6, no auxetic block) while the
+// generator places particles 1e-3 apart; here h = 1.2 dx, dt keeps the INI's dt c / h = 0.125,
+// and the auxetic block is the other INIs'.
+int ref_circle_entry(const Opts &o, sphnj_params *P, sphnj_particles *ps) {
+  ref_base(P);
+  P->gravity = 9.81;
+  P->particle_shift_coef = 2.0;
+  ref_fluids(P, 1000.0, 1500.0, 1.27, 340.0);
+  P->solid_density = 2700.0;
+  P->youngs_modulus = 67.5e9;
+  P->poisson_ratio = 0.34;
+  ref_boxes(P, -0.1, 3.1, -0.1, 3.6, 0.0, 3.0, 0.0, 3.5);
+  ref_kernel_time(P, o, 1.0e-3, 1.0e-6, 0.012, 10000000, 1000, 10, 10);
+  const double Pi = 3.1415926, dx = P->particle_diameter;
+  const double xmax = 0.3, ymax = 0.5, radius = 0.05, middle = 0.15, level = 0.3, shift = level + 0.1;
+  const int L = 8, nx = (int)std::lround(xmax / dx), ny = (int)std::lround(ymax / dx);
+  for (int i = -L; i < nx + L; ++i)
+    for (int j = -L; j < ny + L; ++j) {
+      const double x = (i + 0.5) * dx, y = (j + 0.5) * dx;
+      if (x < 0.0 || x > xmax || y < 0.0 || y > ymax) {
+        add_wall(ps, x, y, SPHNJ_SLIP);
+        continue;
+      }
+      if (std::hypot(x - middle, y - shift) <= radius + dx) continue;
+      if (y < level)
+        add_fluid(ps, x, y, 1000.0, 0.0, SPHNJ_LIQUID);
+      else
+        add_fluid(ps, x, y, 1.27, 0.0, SPHNJ_GAS);
+    }
+  const double t_in = 0.004, t_mid = 0.01, t_out = 0.004, total = t_in + t_mid + t_out;
+  auto ring = [&](double r0, double thick, int mat) {
+    const int layers = (int)std::lround(thick / dx);
+    for (int i = 0; i < layers; ++i) {
+      const double r = r0 + (i + 0.5) * dx;
+      const int n = (int)std::lround(2.0 * Pi * r / dx);
+      const double dth = 2.0 * Pi * r / (double)n;
+      for (int j = 0; j < n; ++j) {
+        const double th = (double)j * dth / r;
+        add_solid(ps, middle + r * std::cos(th), shift + r * std::sin(th), 0.0, -50.0, mat, 0);
+      }
+    }
+  };
+  ring(radius - total, t_in, SPHNJ_LINEAR_ELASTIC);
+  ring(radius - t_out - t_mid, t_mid, SPHNJ_AUXETIC);
+  ring(radius - t_out, t_out, SPHNJ_LINEAR_ELASTIC);
+  if (P->particle_diameter != 1.0e-3) fit_cell_box(P, ps);
+  return SPHNJ_OK;
+}
+
+// cases/beam_dam: water 0.1 m wide and 0.14 m deep at rest (p = 0, rho = rho0) right of an
+// elastic plate (x in [-0.005, 0], y in [0, 0.15]), air elsewhere in the 0.4 x 0.2 m tank
+// (particle_generate.cpp:57-135).  The plate is written as 7-column solids (UNCONSTRAINED).
+int ref_beam_dam(const Opts &o, sphnj_params *P, sphnj_particles *ps) {
+  ref_base(P);
+  P->gravity = 9.81;
+  P->particle_shift_coef = 1.0;
+  ref_fluids(P, 1000.0, 20.0, 1.0, 20.0);
+  P->solid_density = 1100.0;
+  P->youngs_modulus = 7.8e6;
+  P->poisson_ratio = 0.47;
+  ref_boxes(P, -0.31, 0.11, -0.01, 0.21, -0.3, 0.1, 0.0, 0.2);
+  ref_kernel_time(P, o, 1.25e-3, 5.0e-6, 0.0015, 1000000, 1000, 10, 1);
+  const double dx = P->particle_diameter, xmin = -0.3, xmax = 0.1, ymin = 0.0, ymax = 0.2;
+  const double sl = -0.005, sr = 0.0, su = 0.15, sd = 0.0, level = 0.14;
+  const int L = 8, nx = (int)std::lround((xmax - xmin) / dx), ny = (int)std::lround((ymax - ymin) / dx);
+  for (int i = -L; i < nx + L; ++i)
+    for (int j = -L; j < ny + L; ++j) {
+      const double x = (i + 0.5) * dx + xmin, y = (j + 0.5) * dx + ymin;
+      if (!(x < sl || x > sr || y < sd || y > su)) continue;
+      if (x < xmin || x > xmax || y < ymin || y > ymax)
+        add_wall(ps, x, y, SPHNJ_SLIP);
+      else if (x > sr && y < level)
+        add_fluid(ps, x, y, 1000.0, 0.0, SPHNJ_LIQUID);
+      else
+        add_fluid(ps, x, y, 1.0, 0.0, SPHNJ_GAS);
+    }
+  const int nxs = (int)std::lround((sr - sl) / dx), nys = (int)std::lround((su - sd) / dx);
+  for (int i = 0; i < nxs; ++i)
+    for (int j = 0; j < nys; ++j)
+      add_solid(ps, (i + 0.5) * dx + sl, (j + 0.5) * dx + sd, 0.0, 0.0, SPHNJ_LINEAR_ELASTIC, 0);
+  if (P->particle_diameter != 1.25e-3) fit_cell_box(P, ps);
+  return SPHNJ_OK;
+}
+
+// cases/wedge_entry: a 10 deg wedge 1.2 m wide, 0.08 m thick (its lower 0.04 m auxetic, the
+// rest aluminium) whose apex starts 0.5 m above water 2.0 m deep, at 30 m/s, in a 3 x 3.5 m
+// tank; the aluminium's two end columns and its top centre particles are FIXED
+// (particle_generate.cpp:62-203).  The input.ini (particle_diameter 0.01, smoothing_length
+// 0.012) does not fit the generator's 2.5e-3 spacing: h = 1.2 dx, and dt keeps the INI's
+// dt c / h = 0.125.  The FIXED particles move at solid_fixed_speed_y = fixed_v (default -30):
+// the INI leaves it at 0, while the wedge_disp probe (src/output.cu:307-337) subtracts a
+// prescribed descent of 30 m/s.
+int ref_wedge_entry(const Opts &o, sphnj_params *P, sphnj_particles *ps) {
+  ref_base(P);
+  P->gravity = 9.81;
+  P->particle_shift_coef = 1.0;
+  ref_fluids(P, 1000.0, 1500.0, 1.27, 340.0);
+  P->solid_density = 2700.0;
+  P->youngs_modulus = 67.5e9;
+  P->poisson_ratio = 0.34;
+  P->solid_fixed_speed_y = o.d("fixed_v", -30.0);
+  ref_boxes(P, -0.1, 3.1, -0.1, 3.6, 0.0, 3.0, 0.0, 3.5);
+  ref_kernel_time(P, o, 2.5e-3, 1.0e-6, 0.012, 40000, 100, 10, 1);
+  const double Pi = 3.1415926, dx = P->particle_diameter;
+  const double xmax = 3.0, ymax = 3.5, thick = 0.08, half_L = 0.6, middle = 1.5, level = 2.0, shift = 2.5;
+  const double ta = std::tan(10.0 * Pi / 180.0);
+  auto in_wedge = [&](double x, double y) {
+    if (y < shift || x < middle - half_L || x > middle + half_L) return false;
+    if (y > ta * half_L + shift + thick) return false;
+    const double yb = ta * std::fabs(x - middle) + shift;
+    return y > yb && y < yb + thick;
+  };
+  const int L = 8, nx = (int)std::lround(xmax / dx), ny = (int)std::lround(ymax / dx);
+  for (int i = -L; i < nx + L; ++i)
+    for (int j = -L; j < ny + L; ++j) {
+      const double x = (i + 0.5) * dx, y = (j + 0.5) * dx;
+      if (x < 0.0 || x > xmax || y < 0.0 || y > ymax) {
+        add_wall(ps, x, y, SPHNJ_SLIP);
+        continue;
+      }
+      if (in_wedge(x, y)) continue;
+      if (y < level)
+        add_fluid(ps, x, y, 1000.0, 0.0, SPHNJ_LIQUID);
+      else
+        add_fluid(ps, x, y, 1.27, 0.0, SPHNJ_GAS);
+    }
+  const int nxs = (int)std::lround(2.0 * half_L / dx), nys = (int)std::lround(thick / dx);
+  for (int i = 0; i < nxs; ++i)
+    for (int j = 0; j < nys; ++j) {
+      const double x = (i + 0.5) * dx + middle - half_L;
+      const double y = ta * std::fabs(x - middle) + shift + (j + 0.5) * dx;
+      const int mat = j * dx < 0.04 ? SPHNJ_AUXETIC : SPHNJ_LINEAR_ELASTIC;
+      int bc = SPHNJ_UNCONSTRAINED;
+      if (mat == SPHNJ_LINEAR_ELASTIC) {
+        if (x < middle - half_L + dx || x > middle + half_L - dx) bc = SPHNJ_FIXED;
+        if (x < middle + dx && x > middle - dx && j == nys - 1) bc = SPHNJ_FIXED;
+      }
+      add_solid(ps, x, y, 0.0, -30.0, mat, bc);
+    }
+  if (P->particle_diameter != 2.5e-3) fit_cell_box(P, ps);
+  return SPHNJ_OK;
+}
+
 int mkdir_p(const std::string &d) {
   std::string acc;
   std::istringstream is(d);
@@ -361,6 +680,18 @@ int sphnj_case_create(const char *name, const char *options, sphnj_params *param
     rc = wedge_entry(o, params, ps);
   else if (n == "ensemble_member")
     rc = ensemble_member(o, params, ps);
+  else if (n == "ref_dam_break")
+    rc = ref_dam_break(o, params, ps);
+  else if (n == "ref_water_tank")
+    rc = ref_water_tank(o, params, ps);
+  else if (n == "ref_beam_vibrate")
+    rc = ref_beam_vibrate(o, params, ps);
+  else if (n == "ref_circle_entry")
+    rc = ref_circle_entry(o, params, ps);
+  else if (n == "ref_beam_dam")
+    rc = ref_beam_dam(o, params, ps);
+  else if (n == "ref_wedge_entry")
+    rc = ref_wedge_entry(o, params, ps);
   else {
     set_global_error("unknown case " + n);
     rc = SPHNJ_E_ARG;

@@ -17,7 +17,7 @@
 
 namespace sphnj {
 
-constexpr unsigned kStOutOfGrid = 1u, kStCellOver64 = 2u, kStNaN = 4u, kStRefListLeft = 8u;
+constexpr unsigned kStOutOfGrid = 1u, kStCellOver64 = 2u, kStNaN = 4u, kStRefListLeft = 8u, kStSyncTimeout = 16u;
 
 // Packed as unsigned halves: a sub-column reaches 2 nx - 1 <= 65533, past the sign bit.
 __device__ __forceinline__ int cxy_pack(int cx, int cy) { return (int)(((unsigned)cx << 16) | (unsigned)cy); }
@@ -2054,9 +2054,8 @@ __device__ __forceinline__ R plastic_elastic(const CP &cp, R strain, R &max_stra
 // B^-1 depends only on reference positions and is precomputed at init.  Stress as
 // solidStressCalculation (src/solid.cu:131-167) plus the neo-Hookean extension.
 template <class Pol>
-__global__ void __launch_bounds__(256) k_solid_deform(Dev<Pol> d, int sub) {
+__device__ __forceinline__ void solid_deform_body(const Dev<Pol> &d, int sub, int gid) {
   using R = typename Pol::R;
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = gid / kSolidLanes, l = gid % kSolidLanes;
   const bool active = i < d.ns;
   int c = 0;
@@ -2163,12 +2162,16 @@ __global__ void __launch_bounds__(256) k_solid_deform(Dev<Pol> d, int sub) {
   o[3] = PL.d;
 }
 
+template <class Pol>
+__global__ void __launch_bounds__(256) k_solid_deform(Dev<Pol> d, int sub) {
+  solid_deform_body(d, sub, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
 // solidForceCalculate (src/solid.cu:307-353: solidFluidCouple :211-255, solidSolidForce
 // :257-305) fused with solidUpdate (:446-471); positions and velocities ping-pong.
 template <class Pol>
-__global__ void __launch_bounds__(256) k_solid_force_update(Dev<Pol> d, int sub) {
+__device__ __forceinline__ void solid_force_body(const Dev<Pol> &d, int sub, int gid) {
   using R = typename Pol::R;
-  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
   const int i = gid / kSolidLanes, l = gid % kSolidLanes;
   const bool active = i < d.ns;
   int c = 0;
@@ -2289,6 +2292,77 @@ __global__ void __launch_bounds__(256) k_solid_force_update(Dev<Pol> d, int sub)
     d.s_vy2[i] = view_y<Pol>(y, cy, cp);
   }
   if (isnan(x) || isnan(y)) atomicOr(d.status, kStNaN);
+}
+
+template <class Pol>
+__global__ void __launch_bounds__(256) k_solid_force_update(Dev<Pol> d, int sub) {
+  solid_force_body(d, sub, (int)(blockIdx.x * blockDim.x + threadIdx.x));
+}
+
+// ---- all sub-steps of a fluid step in one launch ----------------------------------------
+// A few thousand solids take a few microseconds per phase, so S sub-steps issued as 2 S
+// launches are bound by launch latency.  k_solid_substeps runs them in one launch whose
+// blocks all stay resident (the host sizes the grid to at most half of what the device holds
+// at once, so two contexts stepping concurrently still fit) and meet at a grid barrier
+// between the phases.  Every wait is bounded: a block that waits longer than spin_limit polls
+// raises SPHNJ_ST_SYNC_TIMEOUT, sets the abort word (which ends every other block's waits)
+// and goes on, so every block exits whatever happens.  The last block out resets the barrier
+// words for the next launch.
+struct GridSync {
+  unsigned *count;  // arrivals since the launch began
+  unsigned *done;   // blocks that have left
+  unsigned *abort;  // non-zero once a wait has timed out
+};
+
+// Waits until `target` arrivals have been counted.  Every thread's earlier global stores are
+// released at agent scope before its block arrives (each XCD has its own L2, so they must be
+// written back for blocks elsewhere to see them); thread 0 acquires once the count is reached,
+// which invalidates this CU's caches for the whole block.
+__device__ __forceinline__ void grid_sync(const GridSync &g, unsigned target, int spin_limit, unsigned *status) {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    __hip_atomic_fetch_add(g.count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    int spins = 0;
+    while (__hip_atomic_load(g.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+      if (__hip_atomic_load(g.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
+      if (++spins > spin_limit) {
+        __hip_atomic_fetch_or(status, kStSyncTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(g.abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        break;
+      }
+      __builtin_amdgcn_s_sleep(1);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  }
+  __syncthreads();
+}
+
+// d0 is the state as the sub-steps begin; d1 the same with every ping-pong pair of solid
+// buffers swapped (odd sub-steps read what even ones wrote).  The host swaps its own pointers
+// S times after the launch, as after 2 S launches.
+template <class Pol>
+__global__ void __launch_bounds__(256) k_solid_substeps(Dev<Pol> d0, Dev<Pol> d1, int S, GridSync g,
+                                                        int spin_limit) {
+  const int n = d0.ns * kSolidLanes, stride = (int)(gridDim.x * blockDim.x);
+  unsigned target = 0;
+  for (int k = 0; k < S; ++k) {
+    const Dev<Pol> &e = (k & 1) ? d1 : d0;
+    for (int gid = blockIdx.x * blockDim.x + threadIdx.x; gid - (int)threadIdx.x < n; gid += stride)
+      solid_deform_body(e, k, gid);
+    grid_sync(g, target += gridDim.x, spin_limit, d0.status);
+    for (int gid = blockIdx.x * blockDim.x + threadIdx.x; gid - (int)threadIdx.x < n; gid += stride)
+      solid_force_body(e, k, gid);
+    if (k + 1 < S) grid_sync(g, target += gridDim.x, spin_limit, d0.status);
+  }
+  // the last block out resets the barrier words (every other block has made its last read)
+  __syncthreads();
+  if (threadIdx.x == 0 &&
+      __hip_atomic_fetch_add(g.done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1) {
+    __hip_atomic_store(g.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(g.abort, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(g.done, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+  }
 }
 
 // ---- init: reference-configuration neighbour lists --------------------------------------

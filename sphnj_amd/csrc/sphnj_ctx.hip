@@ -57,6 +57,7 @@ struct EngineBase {
   std::string err;
   hipStream_t stream = nullptr;
   bool use_graph = true;
+  bool persistent_solid = true;  // S solid sub-steps as one launch (k_solid_substeps)
   int blk_pair = 256, blk_shift = 256, blk_stream = 256;
   // per-stage timing: HIP events recorded on the context stream around each stage while on
   bool timing = false;
@@ -111,7 +112,6 @@ struct Engine : EngineBase {
   std::vector<int> f_off, s_off, w_off;  // ncase + 1
   std::vector<Derived> Dh;
   int ncase = 0, total_cells = 0, S = 1, K = 0;
-  int move_epoch = 0;  // position-updating fluid launches since the last hash
   // fluid tiles: global sub-columns G, row table size; per tile size 64 / 128 / 256: tile
   // counts per sub-column, their exclusive scan (+ total), plans, and whether the plans
   // match the current hash
@@ -126,6 +126,11 @@ struct Engine : EngineBase {
   int max_tiles(int B) const { return kMaxPieces * (nblk(d.nf, B) + G); }
   int grid_tiles(int B) const { return (max_tiles(B) + kXcds - 1) / kXcds * kXcds; }
   bool viscid = false;
+  // the persistent solid kernel: barrier words and the grid (0 = the device cannot hold the
+  // solids' blocks resident at half its capacity: the sub-steps fall back to 2 S launches)
+  GridSync gsync{};
+  int solid_grid = 0;
+  static constexpr int kSpinLimit = 1 << 22;  // polls of ~0.1 us: a stuck barrier gives up in < 1 s
   RadixSorter ssort;
   int *s_sorted_keys = nullptr;
   int *cnt_tsum = nullptr, *cnt_toff = nullptr;  // counting sort: tile sums and their scan
@@ -437,6 +442,18 @@ struct Engine : EngineBase {
         (rc = dalloc(&d.s_range, nc)) || (rc = dalloc(&d.s_ref_off, ns + 1)))
       return rc;
     HIPC(hipMemsetAsync(d.s_range, 0, (size_t)nc * sizeof(int2), stream));  // no solid: stays empty
+    if (ns > 0) {
+      unsigned *words;
+      if ((rc = dalloc(&words, 3))) return rc;
+      HIPC(hipMemsetAsync(words, 0, 3 * sizeof(unsigned), stream));
+      gsync = GridSync{words, words + 1, words + 2};
+      int dev = 0, cus = 0, per_cu = 0;
+      HIPC(hipGetDevice(&dev));
+      HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_solid_substeps<Pol>, 256, 0));
+      const int want = nblk(ns * kSolidLanes, 256), fit = per_cu * cus / 2;
+      solid_grid = fit >= 1 ? std::min(want, fit) : 0;
+    }
     // the first cell stage sorts the solids: every key differs from -1, and the flag starts raised
     HIPC(hipMemsetAsync(d.s_key, 0xff, std::max(ns, 1) * sizeof(int), stream));
     HIPC(hipMemsetAsync(d.s_moved, 1, sizeof(int), stream));
@@ -641,7 +658,6 @@ struct Engine : EngineBase {
       HIPC(radix_sort_pairs(ssort, d.s_key, &s_sorted_keys, &d.s_sorted, stream, moved));
       k_cell_bounds<<<nblk(ns, 256), 256, 0, stream>>>(s_sorted_keys, ns, d.s_range, moved);
     }
-    move_epoch = 0;
     plan_ok[0] = plan_ok[1] = plan_ok[2] = false;
     if (nf > 0) {  // row tables, the cell audit and the tile counts of both block sizes
       PlanJobs jobs{};
@@ -816,7 +832,6 @@ struct Engine : EngineBase {
       case SPHNJ_STAGE_FLUID:
         if (nf > 0) {
           const int rc = launch_pair(blk_pair);
-          ++move_epoch;
           if (rc) return rc;
           k_umax_final<Pol><<<ncase, 64, 0, stream>>>(d);
           LAUNCH_CHECK();
@@ -826,10 +841,24 @@ struct Engine : EngineBase {
         return SPHNJ_OK;
       case SPHNJ_STAGE_SOLID:
         if (ns > 0) {
-          for (int k = 0; k < S; ++k) {
-            k_solid_deform<Pol><<<nblk(ns * kSolidLanes, 256), 256, 0, stream>>>(d, k);
-            k_solid_force_update<Pol><<<nblk(ns * kSolidLanes, 256), 256, 0, stream>>>(d, k);
+          const bool one = persistent_solid && solid_grid > 0;
+          if (one) {
+            Dev<Pol> d1 = d;
+            std::swap(d1.sx, d1.sx2);
+            std::swap(d1.sy, d1.sy2);
+            std::swap(d1.su, d1.su2);
+            std::swap(d1.sv, d1.sv2);
+            std::swap(d1.s_vx, d1.s_vx2);
+            std::swap(d1.s_vy, d1.s_vy2);
+            k_solid_substeps<Pol><<<solid_grid, 256, 0, stream>>>(d, d1, S, gsync, kSpinLimit);
             LAUNCH_CHECK();
+          }
+          for (int k = 0; k < S; ++k) {
+            if (!one) {
+              k_solid_deform<Pol><<<nblk(ns * kSolidLanes, 256), 256, 0, stream>>>(d, k);
+              k_solid_force_update<Pol><<<nblk(ns * kSolidLanes, 256), 256, 0, stream>>>(d, k);
+              LAUNCH_CHECK();
+            }
             std::swap(d.sx, d.sx2);
             std::swap(d.sy, d.sy2);
             std::swap(d.su, d.su2);
@@ -843,7 +872,6 @@ struct Engine : EngineBase {
         if (nf > 0) {
           for (int k = 0; k < K; ++k) {
             const int rc = launch_shift(blk_shift, k);
-            ++move_epoch;
             if (rc) return rc;
             // views and path lengths ping-pong (P64 views are the positions)
             std::swap(d.fa.vx, d.fb.vx);
@@ -864,7 +892,7 @@ struct Engine : EngineBase {
   int launches_of(int st) const {
     if (st == SPHNJ_STAGE_SHIFT) return d.nf > 0 ? K : 0;
     if (st == SPHNJ_STAGE_FLUID) return d.nf > 0 ? 2 : 0;
-    if (st == SPHNJ_STAGE_SOLID) return d.ns > 0 ? 2 * S : 0;
+    if (st == SPHNJ_STAGE_SOLID) return d.ns > 0 ? ((persistent_solid && solid_grid > 0) ? 1 : 2 * S) : 0;
     return 1;
   }
 
@@ -974,7 +1002,6 @@ struct Engine : EngineBase {
       }
       HIPC(hipGraphLaunch(it->second.exec, stream));
       d = it->second.post;
-      move_epoch = (d.nf > 0) ? 1 + K : 0;
     }
     return SPHNJ_OK;
   }
@@ -1310,6 +1337,7 @@ struct sphnj_ctx {
   double last_ms = 0.0;
   bool inited = false;
   bool use_graph = true;
+  bool persistent_solid = true;
   std::string err;
   int fail(int code, const std::string &m) {
     err = m;
@@ -1425,6 +1453,7 @@ int sphnj_init(sphnj_ctx *c) {
     c->eng.reset(new Engine<M32>());
   c->eng->stream = c->stream;
   c->eng->use_graph = c->use_graph;
+  c->eng->persistent_solid = c->persistent_solid;
   const int rc = c->eng->init(c->D, c->staged);
   if (rc) {
     c->err = c->eng->err;
@@ -1516,6 +1545,16 @@ int sphnj_set_graph(sphnj_ctx *c, int enable) {
   if (c->eng) {
     c->eng->use_graph = c->use_graph;
     if (!c->use_graph) c->eng->drop_graphs();
+  }
+  return SPHNJ_OK;
+}
+
+int sphnj_set_persistent_solid(sphnj_ctx *c, int enable) {
+  if (!c) return SPHNJ_E_ARG;
+  c->persistent_solid = enable != 0;
+  if (c->eng) {
+    if (c->eng->persistent_solid != c->persistent_solid) c->eng->drop_graphs();
+    c->eng->persistent_solid = c->persistent_solid;
   }
   return SPHNJ_OK;
 }

@@ -262,7 +262,8 @@ class Simulation(ContextBase):
 
     prefix = "sphnj_"
 
-    def __init__(self, cases: Case | list[Case], precision: str = "p64", graph: bool = True):
+    def __init__(self, cases: Case | list[Case], precision: str = "p64", graph: bool = True,
+                 persistent_solid: bool = True):
         lib = library()
         cases = [cases] if isinstance(cases, Case) else list(cases)
         prec = {"p64": A.PREC_P64, "m32": A.PREC_M32}[precision]
@@ -272,6 +273,7 @@ class Simulation(ContextBase):
         super().__init__(lib, h, cases)
         self.precision = precision
         _check(lib.sphnj_set_graph(h, int(graph)), "set_graph")
+        _check(lib.sphnj_set_persistent_solid(h, int(persistent_solid)), "set_persistent_solid")
         self._upload_all()
 
     def _err(self) -> str:

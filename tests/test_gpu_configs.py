@@ -207,3 +207,33 @@ def test_grid_wider_than_16384_cells(gpu, oracle_mod, precision):
     ora.step(20)
     assert sim.status() == ora.status() == 0
     check(errors(sim.download(), ora.download()), TOL[precision], f"wide grid {precision}")
+
+
+# The reference's six scenarios at coarser spacing (same geometry, phases, materials and
+# flags; h and dt follow dx): (options, steps).
+REF_RUNS = {
+    "ref_dam_break": (dict(dx=0.02), 60),
+    "ref_water_tank": (dict(dx=0.01), 60),        # viscous, S = 20, free floor plate
+    "ref_beam_vibrate": (dict(), 200),              # solids only, first-mode velocity
+    "ref_circle_entry": (dict(dx=4e-3), 100),      # elastic/auxetic ring at 50 m/s, S = 10, K = 10
+    "ref_beam_dam": (dict(dx=2.5e-3), 100),        # S = 10
+    "ref_wedge_entry": (dict(dx=0.01), 60),        # auxetic + aluminium wedge, FIXED rim at -30 m/s
+}
+
+
+@pytest.mark.parametrize("precision", ["p64", "m32"])
+@pytest.mark.parametrize("name", sorted(REF_RUNS))
+def test_reference_scenarios_against_oracle(gpu, oracle_mod, name, precision):
+    opts, steps = REF_RUNS[name]
+    make = lambda: Case.generate(name, **opts)  # noqa: E731
+    c = make()
+    nf, ns, nw = c.counts
+    ref, st = oracle_state(oracle_mod, name, make, steps)
+    sim = Simulation(c, precision)
+    sim.step(steps)
+    assert sim.status() == st
+    out = sim.download()
+    errs = errors(out, ref, ("x", "y", "u", "v", "pressure", "stress") if ns else ()) if nf else {}
+    if ns and not nf:
+        errs = {"solid." + f: rel_l2(out["solid"][f], ref["solid"][f]) for f in ("x", "y", "u", "v", "stress")}
+    check(errs, TOL[precision], f"{name} ({nf} fluid, {ns} solid, {nw} wall) {precision}")

@@ -302,3 +302,31 @@ def test_stages_equal_step(gpu, precision):
     for sp in ("fluid", "solid", "wall"):
         for k in da[sp]:
             assert np.array_equal(da[sp][k], db[sp][k]), (sp, k)
+
+
+@pytest.mark.parametrize("precision", ["p64", "m32"])
+def test_one_launch_solid_substeps_equal_launch_pairs(gpu, precision):
+    """The S solid sub-steps of a fluid step as one launch with grid barriers (the default)
+    equal the 2 S launch form bit for bit, through graph replays, on a batch whose cases have
+    different sub-step counts (S = 4 and S = 10: the case with fewer carries its state over)."""
+    from test_gpu_parity import small_gate
+
+    cases = [small_gate(), small_gate(S=10, gate_t=0.008)]
+    a = Simulation(cases, precision)
+    b = Simulation(cases, precision, persistent_solid=False)
+    for n in (1, 3, 7):
+        a.step(n)
+        b.step(n)
+        for c in range(2):
+            da, db = a.download(c), b.download(c)
+            for sp in ("fluid", "solid"):
+                for k in da[sp]:
+                    assert np.array_equal(da[sp][k], db[sp][k]), (n, c, sp, k)
+    assert a.status() == b.status() == 0
+    # per-stage timing counts the sub-steps as one launch
+    a.set_stage_timing(True)
+    a.step(1)
+    assert a.stage_time("solid")[0] == 1
+    b.set_stage_timing(True)
+    b.step(1)
+    assert b.stage_time("solid")[0] == 2 * 10

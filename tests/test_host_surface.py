@@ -250,3 +250,86 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
     assert lib.sphnj_abi_version() == 1
+
+
+# ---- the reference's six scenarios (cases/<name>/particle_generate.cpp + config/input.ini) ----
+
+REF_CASES = {
+    # name: (fluid, liquid, solid, wall) at the reference's own spacing, from the generators'
+    # loops: e.g. dam_break 1073 x 500 fluid columns/rows with a 2.0 x 1.0 liquid block
+    # (400 x 200), walls = (1073 + 16)(500 + 16) - 536500
+    "ref_dam_break": (536500, 80000, 0, 25424),
+    "ref_water_tank": (320000, 320000, 416 * 20, 416 * 808 - 320000),
+    "ref_beam_vibrate": (0, 0, 244 * 24, 0),
+    "ref_beam_dam": (320 * 160 - 4 * 120, 80 * 112, 4 * 120, 336 * 176 - 320 * 160),
+    "ref_wedge_entry": (1664640, 1200 * 800, 480 * 32, 1216 * 1416 - 1200 * 1400),
+}
+
+
+@pytest.mark.parametrize("name", sorted(REF_CASES))
+def test_reference_scenario_counts(name):
+    c = Case.generate(name)
+    nf, nl, ns, nw = REF_CASES[name]
+    assert c.counts == (nf, ns, nw)
+    assert c.n_liquid == nl
+    p = c.params
+    assert p.smoothing_length == pytest.approx(1.2 * p.particle_diameter)
+    assert validate_params(p) == 0
+
+
+def test_reference_scenario_details():
+    # water tank: hydrostatic column at c = 200, viscous, S = 20, K = 10; floor plate below y = 0
+    c = Case.generate("ref_water_tank")
+    f, p = c.fluid, c.params
+    assert p.is_viscid == 1 and p.liquid_viscosity == 1.0 and p.solid_sub_step == 20 and p.shift_per_step == 10
+    assert np.allclose(f["p"], 1000.0 * 9.81 * (2.0 - f["y"]))
+    assert np.allclose(f["rho"], 1000.0 * (f["p"] * 7.0 / 200.0 ** 2 / 1000.0 + 1.0) ** (1 / 7))
+    assert c.solid["y"].max() < 0 and np.all(c.solid["bc"] == A.UNCONSTRAINED)
+    # wedge: 10 deg, auxetic lower half (j < 16), aluminium upper half, rim columns and the
+    # top centre pair FIXED, everything at -30 m/s
+    c = Case.generate("ref_wedge_entry")
+    s = c.solid
+    assert np.all(s["v"] == -30.0) and c.params.solid_fixed_speed_y == -30.0
+    assert (s["material"] == A.AUXETIC).sum() == 480 * 16
+    fixed = s["bc"] == A.FIXED
+    assert fixed.sum() == 2 * 16 + 2 and np.all(s["material"][fixed] == A.LINEAR_ELASTIC)
+    apex = np.abs(s["x"] - 1.5) < 2.5e-3
+    assert np.allclose(s["y"][apex].min(), math.tan(10 * 3.1415926 / 180) * 1.25e-3 + 2.5 + 1.25e-3)
+    # circle: 4 + 10 + 4 layers (elastic, auxetic, elastic) at 50 m/s; no fluid within r + dx
+    c = Case.generate("ref_circle_entry")
+    r = np.hypot(c.solid["x"] - 0.15, c.solid["y"] - 0.4)
+    aux = c.solid["material"] == A.AUXETIC
+    assert r[aux].min() > 0.05 - 0.014 and r[aux].max() < 0.05 - 0.004
+    assert r.min() > 0.05 - 0.018 and r.max() < 0.05
+    assert np.all(c.solid["v"] == -50.0) and c.params.particle_shift_coef == 2.0
+    assert np.hypot(c.fluid["x"] - 0.15, c.fluid["y"] - 0.4).min() > 0.051
+    # beam: first-mode velocity profile, zero at the root, largest at the tip
+    c = Case.generate("ref_beam_vibrate")
+    v = c.solid["v"]
+    tip = v[np.argmax(c.solid["x"])] / (0.01 * math.sqrt(2e6 / 0.6 / 1000))   # v(L) = xi c_s, the tip row is dx/2 short
+    assert c.params.gravity == 0.0 and 0.99 < tip < 1.0
+    root = np.abs(c.solid["x"] - 5e-4) < 1e-9
+    assert np.abs(v[root]).max() < 1e-4 * np.abs(v).max()
+    # dam break: the reference's nondimensional one (g = 1, rho = 1 / 1e-3, c = 20), p = 0
+    c = Case.generate("ref_dam_break")
+    assert c.params.gravity == 1.0 and np.all(c.fluid["p"] == 0.0)
+    assert set(np.unique(c.fluid["rho"])) == {1.0, 1.0e-3}
+
+
+@pytest.mark.parametrize("name", sorted(REF_CASES) + ["ref_circle_entry"])
+def test_reference_scenario_round_trips_through_files(tmp_path, name):
+    """sphnj_case_write then the reference's surfaces back (input.ini + data/*.dat)."""
+    import ctypes as C
+
+    c = Case.generate(name, dx={"ref_wedge_entry": 0.02, "ref_dam_break": 0.02, "ref_circle_entry": 0.004,
+                                "ref_water_tank": 0.01}.get(name, 2.5e-3 if name == "ref_beam_dam" else 1e-3))
+    h = C.c_void_p()
+    opts = ""
+    lib = library()
+    p = A.Params()
+    assert lib.sphnj_case_create(name.encode(), opts.encode(), C.byref(p), C.byref(h)) == 0
+    assert lib.sphnj_case_write(C.byref(c.params), h, str(tmp_path).encode()) == 0
+    lib.sphnj_particles_destroy(h)
+    back = Case.read(tmp_path)
+    for k, _ in A.Params._fields_:
+        assert getattr(back.params, k) == pytest.approx(getattr(c.params, k), rel=1e-15), k

@@ -6,6 +6,8 @@
 //
 //   sphnj run <case_dir> [--precision p64|m32] [--steps N] [--no-output] [--no-graph]
 //   sphnj gen <case_name> <case_dir> [options]      (writes input.ini + data/*.dat)
+//   sphnj vtk <case_dir> <out_dir>                  (the input particles as VTK, no GPU)
+#include <cerrno>
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -21,29 +23,48 @@ namespace {
 int usage() {
   std::fprintf(stderr,
                "usage: sphnj run <case_dir> [--precision p64|m32] [--steps N] [--no-output] [--no-graph]\n"
-               "       sphnj gen <case_name> <case_dir> [key=value;...]\n");
+               "       sphnj gen <case_name> <case_dir> [key=value;...]\n"
+               "       sphnj vtk <case_dir> <out_dir>\n");
   return 2;
 }
 
 // Legacy-VTK POLYDATA of one species (the reference writes one file per field,
-// src/output.cu:7-248; here one file carries every field, with true names).
-void write_vtk(const std::string &path, int n, const double *x, const double *y,
-               const std::vector<std::pair<std::string, const double *>> &scalars,
-               const std::vector<std::pair<std::string, const int32_t *>> &ints) {
+// src/output.cu:7-248; here one file carries every field, with true names, at 17 significant
+// digits so a reader gets the doubles back).  Returns 0, or SPHNJ_E_IO with a message on
+// stderr when the file cannot be opened or written.
+int write_vtk(const std::string &path, int n, const double *x, const double *y,
+              const std::vector<std::pair<std::string, const double *>> &scalars,
+              const std::vector<std::pair<std::string, const int32_t *>> &ints) {
   FILE *f = std::fopen(path.c_str(), "w");
-  if (!f) return;
+  if (!f) {
+    std::fprintf(stderr, "cannot open %s for writing\n", path.c_str());
+    return SPHNJ_E_IO;
+  }
   std::fprintf(f, "# vtk DataFile Version 3.0\nsphnj\nASCII\nDATASET POLYDATA\nPOINTS %d double\n", n);
-  for (int i = 0; i < n; ++i) std::fprintf(f, "%.9g %.9g 0\n", x[i], y[i]);
+  for (int i = 0; i < n; ++i) std::fprintf(f, "%.17g %.17g 0\n", x[i], y[i]);
   std::fprintf(f, "POINT_DATA %d\n", n);
   for (auto &s : scalars) {
     std::fprintf(f, "SCALARS %s double 1\nLOOKUP_TABLE default\n", s.first.c_str());
-    for (int i = 0; i < n; ++i) std::fprintf(f, "%.9g\n", s.second[i]);
+    for (int i = 0; i < n; ++i) std::fprintf(f, "%.17g\n", s.second[i]);
   }
   for (auto &s : ints) {
     std::fprintf(f, "SCALARS %s int 1\nLOOKUP_TABLE default\n", s.first.c_str());
     for (int i = 0; i < n; ++i) std::fprintf(f, "%d\n", s.second[i]);
   }
-  std::fclose(f);
+  const bool bad = std::ferror(f) != 0;
+  if (std::fclose(f) != 0 || bad) {
+    std::fprintf(stderr, "error writing %s\n", path.c_str());
+    return SPHNJ_E_IO;
+  }
+  return 0;
+}
+
+int make_dir(const std::string &d) {
+  if (mkdir(d.c_str(), 0755) != 0 && errno != EEXIST) {
+    std::fprintf(stderr, "cannot create %s\n", d.c_str());
+    return SPHNJ_E_IO;
+  }
+  return 0;
 }
 
 int output(sphnj_ctx *ctx, const std::string &dir, int step) {
@@ -63,22 +84,52 @@ int output(sphnj_ctx *ctx, const std::string &dir, int step) {
   int rc = sphnj_download(ctx, 0, &fo, &so, &wo);
   if (rc) return rc;
   const std::string t = std::to_string(step);
-  if (nf) write_vtk(dir + "/fluid-" + t + ".vtk", nf, fx.data(), fy.data(),
-                    {{"density", fr.data()}, {"pressure", fp.data()}, {"u", fu.data()}, {"v", fv.data()}},
-                    {{"phase", fph.data()}, {"boundary_type", fcls.data()}});
+  if (nf && (rc = write_vtk(dir + "/fluid-" + t + ".vtk", nf, fx.data(), fy.data(),
+                            {{"density", fr.data()}, {"pressure", fp.data()}, {"u", fu.data()}, {"v", fv.data()}},
+                            {{"phase", fph.data()}, {"boundary_type", fcls.data()}})))
+    return rc;
   if (ns) {
     std::vector<double> pxx(ns), pyy(ns);
     for (int i = 0; i < ns; ++i) {
       pxx[i] = sP[4 * i];
       pyy[i] = sP[4 * i + 3];
     }
-    write_vtk(dir + "/solid-" + t + ".vtk", ns, sx.data(), sy.data(),
-              {{"pressure", sp.data()}, {"u", su.data()}, {"v", sv.data()}, {"P_xx", pxx.data()}, {"P_yy", pyy.data()}},
-              {});
+    if ((rc = write_vtk(dir + "/solid-" + t + ".vtk", ns, sx.data(), sy.data(),
+                        {{"pressure", sp.data()}, {"u", su.data()}, {"v", sv.data()}, {"P_xx", pxx.data()},
+                         {"P_yy", pyy.data()}},
+                        {})))
+      return rc;
   }
-  if (nw) write_vtk(dir + "/virt-" + t + ".vtk", nw, wx.data(), wy.data(), {{"pressure", wp.data()}}, {});
+  if (nw && (rc = write_vtk(dir + "/virt-" + t + ".vtk", nw, wx.data(), wy.data(), {{"pressure", wp.data()}}, {})))
+    return rc;
   std::printf("Outputing Data... --step=%d\n", step);
   return 0;
+}
+
+// The input particles of a case as VTK (fluid, solid, virt at step "input"), through the
+// same writer as the run's output.
+int vtk(int argc, char **argv) {
+  if (argc < 4) return usage();
+  sphnj_particles *ps = nullptr;
+  if (sphnj_particles_read((std::string(argv[2]) + "/data").c_str(), &ps)) {
+    std::fprintf(stderr, "FAILED TO READ PARTICLES DATA: %s\n", sphnj_global_error());
+    return 1;
+  }
+  sphnj_fluid_in f;
+  sphnj_solid_in s;
+  sphnj_wall_in w;
+  sphnj_particles_views(ps, &f, &s, &w, nullptr);
+  const std::string out = argv[3];
+  int rc = make_dir(out);
+  if (!rc && f.n)
+    rc = write_vtk(out + "/fluid-input.vtk", f.n, f.x, f.y,
+                   {{"density", f.rho}, {"pressure", f.p}, {"u", f.u}, {"v", f.v}}, {{"phase", f.phase}});
+  if (!rc && s.n)
+    rc = write_vtk(out + "/solid-input.vtk", s.n, s.x, s.y, {{"u", s.u}, {"v", s.v}, {"X", s.X}, {"Y", s.Y}},
+                   {{"material", s.material}});
+  if (!rc && w.n) rc = write_vtk(out + "/virt-input.vtk", w.n, w.x, w.y, {}, {{"boundary_type", w.bc}});
+  sphnj_particles_destroy(ps);
+  return rc ? 1 : 0;
 }
 
 int run(int argc, char **argv) {
@@ -133,7 +184,10 @@ int run(int argc, char **argv) {
     return 1;
   }
   const std::string odir = dir + "/out";
-  mkdir(odir.c_str(), 0755);
+  if (out && make_dir(odir)) {
+    sphnj_destroy(ctx);
+    return 1;
+  }
   const int n = steps >= 0 ? steps : P.iend - P.istart;
   const int every = P.result_interval > 0 ? P.result_interval : 1;
   auto s0 = std::chrono::steady_clock::now();
@@ -182,5 +236,6 @@ int main(int argc, char **argv) {
   if (argc < 2) return usage();
   if (!std::strcmp(argv[1], "run")) return run(argc, argv);
   if (!std::strcmp(argv[1], "gen")) return gen(argc, argv);
+  if (!std::strcmp(argv[1], "vtk")) return vtk(argc, argv);
   return usage();
 }

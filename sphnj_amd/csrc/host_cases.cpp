@@ -18,7 +18,9 @@
 //                   seed index layers K steps
 //   ref_dam_break, ref_water_tank, ref_beam_vibrate, ref_circle_entry, ref_beam_dam,
 //   ref_wedge_entry  the reference's six scenarios (cases/<name>), as its generators and
-//                   input.ini files define them.  dx steps (ref_wedge_entry also fixed_v)
+//                   input.ini files define them.  dx steps; ref_circle_entry and
+//                   ref_wedge_entry also height (the body's start above the water),
+//                   ref_wedge_entry fixed_v
 // Initial pressure is hydrostatic p = rho0 g (level - y) (as in
 // cases/water_tank/particle_generate.cpp:96-97), continuous across the air/water surface, and
 // the density is its inverse Tait value rho = rho0 ((p - p_bg) B^-1 + 1)^(1/gamma).
@@ -518,7 +520,8 @@ int ref_circle_entry(const Opts &o, sphnj_params *P, sphnj_particles *ps) {
   ref_boxes(P, -0.1, 3.1, -0.1, 3.6, 0.0, 3.0, 0.0, 3.5);
   ref_kernel_time(P, o, 1.0e-3, 1.0e-6, 0.012, 10000000, 1000, 10, 10);
   const double Pi = 3.1415926, dx = P->particle_diameter;
-  const double xmax = 0.3, ymax = 0.5, radius = 0.05, middle = 0.15, level = 0.3, shift = level + 0.1;
+  const double xmax = 0.3, ymax = 0.5, radius = 0.05, middle = 0.15, level = 0.3;
+  const double shift = level + o.d("height", 0.1);  // ring centre above the water (0.1 m)
   const int L = 8, nx = (int)std::lround(xmax / dx), ny = (int)std::lround(ymax / dx);
   for (int i = -L; i < nx + L; ++i)
     for (int j = -L; j < ny + L; ++j) {
@@ -608,7 +611,8 @@ int ref_wedge_entry(const Opts &o, sphnj_params *P, sphnj_particles *ps) {
   ref_boxes(P, -0.1, 3.1, -0.1, 3.6, 0.0, 3.0, 0.0, 3.5);
   ref_kernel_time(P, o, 2.5e-3, 1.0e-6, 0.012, 40000, 100, 10, 1);
   const double Pi = 3.1415926, dx = P->particle_diameter;
-  const double xmax = 3.0, ymax = 3.5, thick = 0.08, half_L = 0.6, middle = 1.5, level = 2.0, shift = 2.5;
+  const double xmax = 3.0, ymax = 3.5, thick = 0.08, half_L = 0.6, middle = 1.5, level = 2.0;
+  const double shift = level + o.d("height", 0.5);  // apex above the water (0.5 m)
   const double ta = std::tan(10.0 * Pi / 180.0);
   auto in_wedge = [&](double x, double y) {
     if (y < shift || x < middle - half_L || x > middle + half_L) return false;

@@ -2308,27 +2308,40 @@ __global__ void __launch_bounds__(256) k_solid_force_update(Dev<Pol> d, int sub)
 // raises SPHNJ_ST_SYNC_TIMEOUT, sets the abort word (which ends every other block's waits)
 // and goes on, so every block exits whatever happens.  The last block out resets the barrier
 // words for the next launch.
+constexpr int kSubstepBlock = 512;
+constexpr int kSyncLine = 32;  // barrier words one 128-byte line apart
 struct GridSync {
-  unsigned *count;  // arrivals since the launch began
-  unsigned *done;   // blocks that have left
-  unsigned *abort;  // non-zero once a wait has timed out
+  unsigned *words;  // [k * kSyncLine]: k < 8 arrivals of XCD group k, 8 groups done, 9 blocks
+                    // that have left, 10 abort
+  __device__ unsigned *group(int k) const { return words + k * kSyncLine; }
+  __device__ unsigned *top() const { return words + 8 * kSyncLine; }
+  __device__ unsigned *done() const { return words + 9 * kSyncLine; }
+  __device__ unsigned *abort() const { return words + 10 * kSyncLine; }
 };
 
-// Waits until `target` arrivals have been counted.  Every thread's earlier global stores are
-// released at agent scope before its block arrives (each XCD has its own L2, so they must be
-// written back for blocks elsewhere to see them); thread 0 acquires once the count is reached,
-// which invalidates this CU's caches for the whole block.
-__device__ __forceinline__ void grid_sync(const GridSync &g, unsigned target, int spin_limit, unsigned *status) {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+// The e-th grid barrier of the launch (e = 1, 2, ...).  Blocks are counted per XCD group
+// (block b runs on XCD b mod 8), and the last block of a group counts the group once on the
+// top word, so no word takes more than a group's arrivals.  Each wave waits for its own
+// global stores; then one thread per block writes the CU's XCD L2 back (the release every
+// XCD needs to see them) before arriving, and invalidates its caches after the wait.
+__device__ __forceinline__ void grid_sync(const GridSync &g, unsigned e, int spin_limit, unsigned *status) {
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // s_waitcnt vmcnt(0): this wave's stores have landed
   __syncthreads();
   if (threadIdx.x == 0) {
-    __hip_atomic_fetch_add(g.count, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned G = gridDim.x, x = blockIdx.x % 8u;
+    const unsigned groups = G < 8u ? G : 8u, in_group = (G - x + 7u) / 8u;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    const unsigned old = __hip_atomic_fetch_add(g.group((int)x), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == e * in_group - 1u) {  // the group's last arrival counts the group
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
+      __hip_atomic_fetch_add(g.top(), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     int spins = 0;
-    while (__hip_atomic_load(g.count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-      if (__hip_atomic_load(g.abort, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
+    while (__hip_atomic_load(g.top(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < e * groups) {
+      if (__hip_atomic_load(g.abort(), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0u) break;
       if (++spins > spin_limit) {
         __hip_atomic_fetch_or(status, kStSyncTimeout, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(g.abort, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(g.abort(), 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         break;
       }
       __builtin_amdgcn_s_sleep(1);
@@ -2342,26 +2355,27 @@ __device__ __forceinline__ void grid_sync(const GridSync &g, unsigned target, in
 // buffers swapped (odd sub-steps read what even ones wrote).  The host swaps its own pointers
 // S times after the launch, as after 2 S launches.
 template <class Pol>
-__global__ void __launch_bounds__(256) k_solid_substeps(Dev<Pol> d0, Dev<Pol> d1, int S, GridSync g,
-                                                        int spin_limit) {
+__global__ void __launch_bounds__(kSubstepBlock) k_solid_substeps(Dev<Pol> d0, Dev<Pol> d1, int S, GridSync g,
+                                                                  int spin_limit) {
   const int n = d0.ns * kSolidLanes, stride = (int)(gridDim.x * blockDim.x);
-  unsigned target = 0;
+  unsigned e = 0;
   for (int k = 0; k < S; ++k) {
-    const Dev<Pol> &e = (k & 1) ? d1 : d0;
+    const Dev<Pol> &d = (k & 1) ? d1 : d0;
     for (int gid = blockIdx.x * blockDim.x + threadIdx.x; gid - (int)threadIdx.x < n; gid += stride)
-      solid_deform_body(e, k, gid);
-    grid_sync(g, target += gridDim.x, spin_limit, d0.status);
+      solid_deform_body(d, k, gid);
+    grid_sync(g, ++e, spin_limit, d0.status);
     for (int gid = blockIdx.x * blockDim.x + threadIdx.x; gid - (int)threadIdx.x < n; gid += stride)
-      solid_force_body(e, k, gid);
-    if (k + 1 < S) grid_sync(g, target += gridDim.x, spin_limit, d0.status);
+      solid_force_body(d, k, gid);
+    if (k + 1 < S) grid_sync(g, ++e, spin_limit, d0.status);
   }
   // the last block out resets the barrier words (every other block has made its last read)
   __syncthreads();
   if (threadIdx.x == 0 &&
-      __hip_atomic_fetch_add(g.done, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1) {
-    __hip_atomic_store(g.count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(g.abort, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(g.done, 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      __hip_atomic_fetch_add(g.done(), 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1) {
+    for (int k = 0; k < 8; ++k) __hip_atomic_store(g.group(k), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(g.top(), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(g.abort(), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(g.done(), 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
   }
 }
 

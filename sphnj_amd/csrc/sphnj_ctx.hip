@@ -444,14 +444,14 @@ struct Engine : EngineBase {
     HIPC(hipMemsetAsync(d.s_range, 0, (size_t)nc * sizeof(int2), stream));  // no solid: stays empty
     if (ns > 0) {
       unsigned *words;
-      if ((rc = dalloc(&words, 3))) return rc;
-      HIPC(hipMemsetAsync(words, 0, 3 * sizeof(unsigned), stream));
-      gsync = GridSync{words, words + 1, words + 2};
+      if ((rc = dalloc(&words, 11 * kSyncLine))) return rc;
+      HIPC(hipMemsetAsync(words, 0, 11 * kSyncLine * sizeof(unsigned), stream));
+      gsync = GridSync{words};
       int dev = 0, cus = 0, per_cu = 0;
       HIPC(hipGetDevice(&dev));
       HIPC(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
-      HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_solid_substeps<Pol>, 256, 0));
-      const int want = nblk(ns * kSolidLanes, 256), fit = per_cu * cus / 2;
+      HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_solid_substeps<Pol>, kSubstepBlock, 0));
+      const int want = nblk(ns * kSolidLanes, kSubstepBlock), fit = per_cu * cus / 2;
       solid_grid = fit >= 1 ? std::min(want, fit) : 0;
     }
     // the first cell stage sorts the solids: every key differs from -1, and the flag starts raised
@@ -850,7 +850,7 @@ struct Engine : EngineBase {
             std::swap(d1.sv, d1.sv2);
             std::swap(d1.s_vx, d1.s_vx2);
             std::swap(d1.s_vy, d1.s_vy2);
-            k_solid_substeps<Pol><<<solid_grid, 256, 0, stream>>>(d, d1, S, gsync, kSpinLimit);
+            k_solid_substeps<Pol><<<solid_grid, kSubstepBlock, 0, stream>>>(d, d1, S, gsync, kSpinLimit);
             LAUNCH_CHECK();
           }
           for (int k = 0; k < S; ++k) {

@@ -114,3 +114,24 @@ def golden(name):
     sp = {s: {k.split(".", 2)[2]: a for k, a in z.items() if k.startswith(f"in.{s}.")}
           for s in ("fluid", "solid", "wall")}
     return Case(p, sp["fluid"], sp["solid"], sp["wall"]), z
+
+
+def read_vtk(path):
+    """A legacy-VTK POLYDATA file as written by the sphnj driver: (points (n, 3), {name: values})."""
+    lines = open(path).read().split("\n")
+    assert lines[0].startswith("# vtk DataFile") and lines[2] == "ASCII" and lines[3] == "DATASET POLYDATA"
+    head = lines[4].split()
+    assert head[0] == "POINTS"
+    n = int(head[1])
+    pts = np.array([list(map(float, ln.split())) for ln in lines[5:5 + n]]).reshape(n, 3)
+    k = 5 + n
+    assert lines[k] == f"POINT_DATA {n}"
+    k += 1
+    fields = {}
+    while k < len(lines) and lines[k]:
+        tag, name, typ, _ = lines[k].split()
+        assert tag == "SCALARS" and lines[k + 1] == "LOOKUP_TABLE default"
+        conv = float if typ == "double" else int
+        fields[name] = np.array([conv(v) for v in lines[k + 2:k + 2 + n]])
+        k += 2 + n
+    return pts, fields

@@ -118,17 +118,11 @@ def test_wedge_entry_reduced(gpu, oracle_mod, precision):
     assert sim.status() == st
     out = sim.download()
     assert np.abs(ref["solid"]["pressure"]).max() > 1e4, "the wedge must be in contact with the water"
-    errs = errors(out, ref, ("x", "y", "u", "v", "pressure", "stress"))
-    check({k: v for k, v in errs.items() if k not in ("solid.stress", "solid.u")}, TOL[precision], f"wedge {precision}")
-    # A 70 GPa hull strains by |F - I| ~ 1e-7 here, and its stress (and through it the solid
-    # acceleration) is a function of F - I: a rounding-level difference in F (the oracle forms
-    # F = F~ L and cancels, the library forms F - I from displacement differences) is
-    # amplified by 1 / |F - I|.  The P64 bar for these two fields is that conditioning times
-    # a few dozen ulps; M32 keeps its 1e-4.
-    F = ref["solid"]["deformation"].reshape(-1, 2, 2)
-    strain = np.abs(F - np.eye(2)).max()
-    tol = TOL[precision] if precision == "m32" else max(1e-9, 64 * np.finfo(float).eps / strain)
-    check({k: errs[k] for k in ("solid.stress", "solid.u")}, tol, f"wedge {precision} (|F - I| = {strain:.1e})")
+    errs = errors(out, ref)
+    errs.update(solid_errors(out, ref, ("x", "y", "pressure")))
+    check(errs, TOL[precision], f"wedge {precision}")
+    serr = {"solid.stress": rel_l2(out["solid"]["stress"], ref["solid"]["stress"])}
+    check(serr, STIFF_SOLID_P64_TOL if precision == "p64" else TOL[precision], f"wedge stress {precision}")
 
 
 def ensemble_small_members(k=4, pool=48):
@@ -215,10 +209,26 @@ REF_RUNS = {
     "ref_dam_break": (dict(dx=0.02), 60),
     "ref_water_tank": (dict(dx=0.01), 60),        # viscous, S = 20, free floor plate
     "ref_beam_vibrate": (dict(), 200),              # solids only, first-mode velocity
-    "ref_circle_entry": (dict(dx=4e-3), 100),      # elastic/auxetic ring at 50 m/s, S = 10, K = 10
+    # elastic/auxetic ring at 50 m/s, S = 10, K = 10, started just above the water so that it
+    # hits it within the run
+    "ref_circle_entry": (dict(dx=4e-3, height=0.056), 150),
     "ref_beam_dam": (dict(dx=2.5e-3), 100),        # S = 10
-    "ref_wedge_entry": (dict(dx=0.01), 60),        # auxetic + aluminium wedge, FIXED rim at -30 m/s
+    # auxetic + aluminium wedge, FIXED rim at -30 m/s, apex started at the water surface
+    "ref_wedge_entry": (dict(dx=0.01, height=0.0), 100),
 }
+# Solids of Young's modulus above 1 GPa strain by 1e-7 .. 1e-4 here, and their stress is a
+# function of that small strain: a rounding-level difference in position is amplified by
+# 1 / strain in the stress, and thousands of sub-steps (S = 10 or 20) accumulate it.  For
+# these the P64 bar on solid stress is 1e-7 (the others keep 1e-9); M32 keeps 1e-4.
+STIFF_SOLID_P64_TOL = 1e-7
+
+
+def solid_errors(a, b, fields):
+    """Solid fields, with the velocity as one vector field (a falling body's horizontal
+    velocity alone is zero by symmetry, up to rounding)."""
+    e = {"solid." + f: rel_l2(a["solid"][f], b["solid"][f]) for f in fields}
+    e["solid.vel"] = rel_l2(np.stack([a["solid"]["u"], a["solid"]["v"]]), np.stack([b["solid"]["u"], b["solid"]["v"]]))
+    return e
 
 
 @pytest.mark.parametrize("precision", ["p64", "m32"])
@@ -233,7 +243,14 @@ def test_reference_scenarios_against_oracle(gpu, oracle_mod, name, precision):
     sim.step(steps)
     assert sim.status() == st
     out = sim.download()
-    errs = errors(out, ref, ("x", "y", "u", "v", "pressure", "stress") if ns else ()) if nf else {}
-    if ns and not nf:
-        errs = {"solid." + f: rel_l2(out["solid"][f], ref["solid"][f]) for f in ("x", "y", "u", "v", "stress")}
+    errs = errors(out, ref) if nf else {}
+    if ns:
+        errs.update(solid_errors(out, ref, ("x", "y", "pressure") if nf else ("x", "y")))
+        if np.abs(ref["solid"]["stress"]).max() > 0:
+            serr = {"solid.stress": rel_l2(out["solid"]["stress"], ref["solid"]["stress"])}
+            stiff = c.params.youngs_modulus > 1e9
+            check(serr, STIFF_SOLID_P64_TOL if (stiff and precision == "p64") else TOL[precision],
+                  f"{name} stress {precision}")
+    if name in ("ref_circle_entry", "ref_wedge_entry"):
+        assert np.abs(ref["solid"]["pressure"]).max() > 1e4, "the body must be in the water"
     check(errs, TOL[precision], f"{name} ({nf} fluid, {ns} solid, {nw} wall) {precision}")

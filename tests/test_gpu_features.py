@@ -330,3 +330,40 @@ def test_one_launch_solid_substeps_equal_launch_pairs(gpu, precision):
     b.set_stage_timing(True)
     b.step(1)
     assert b.stage_time("solid")[0] == 2 * 10
+
+
+def test_driver_run_writes_what_download_returns(gpu, tmp_path):
+    """`sphnj run` on a generated case (the reference's main loop: input.ini + data/*.dat in,
+    out/*.vtk every result_interval steps): the last VTK files hold what the library returns
+    for the same case stepped as many times."""
+    import subprocess
+
+    from helpers import read_vtk
+    from test_gpu_parity import small_gate
+
+    driver = __import__("pathlib").Path(__file__).resolve().parent.parent / "sphnj_amd" / "bin" / "sphnj"
+    case_dir = tmp_path / "case"
+    r = subprocess.run([str(driver), "gen", "gate_fsi", str(case_dir), "dx=2.0e-3;tank_w=0.3;tank_h=0.2;S=4;steps=12"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    ini = case_dir / "config" / "input.ini"
+    ini.write_text(ini.read_text().replace("result_interval = 1000000", "result_interval = 5"))
+    r = subprocess.run([str(driver), "run", str(case_dir)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "total time=" in r.stdout
+    out = sorted(p.name for p in (case_dir / "out").iterdir())
+    assert out == sorted(f"{s}-{k}.vtk" for s in ("fluid", "solid", "virt") for k in (0, 5, 10, 12)), out
+    c = Case.read(case_dir)
+    sim = Simulation(c, "p64")
+    sim.step(12)
+    d = sim.download()
+    pts, f = read_vtk(case_dir / "out" / "fluid-12.vtk")
+    assert np.array_equal(pts[:, 0], d["fluid"]["x"]) and np.array_equal(pts[:, 1], d["fluid"]["y"])
+    for k, name in (("rho", "density"), ("p", "pressure"), ("u", "u"), ("v", "v")):
+        assert np.array_equal(f[name], d["fluid"][k]), k
+    assert np.array_equal(f["boundary_type"], d["fluid"]["boundary_type"])
+    pts, f = read_vtk(case_dir / "out" / "solid-12.vtk")
+    assert np.array_equal(pts[:, 1], d["solid"]["y"]) and np.array_equal(f["pressure"], d["solid"]["pressure"])
+    assert np.array_equal(f["P_yy"], d["solid"]["stress"][:, 3])
+    _, f = read_vtk(case_dir / "out" / "virt-12.vtk")
+    assert np.array_equal(f["pressure"], d["wall"]["p"])

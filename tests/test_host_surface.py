@@ -333,3 +333,36 @@ def test_reference_scenario_round_trips_through_files(tmp_path, name):
     back = Case.read(tmp_path)
     for k, _ in A.Params._fields_:
         assert getattr(back.params, k) == pytest.approx(getattr(c.params, k), rel=1e-15), k
+
+
+DRIVER = ROOT / "sphnj_amd" / "bin" / "sphnj"
+
+
+def test_driver_writes_input_particles_as_vtk(tmp_path):
+    """`sphnj gen` then `sphnj vtk` (no GPU): the files parse back to the generated arrays, and
+    an unwritable output directory is an error, not a silent success."""
+    import subprocess
+
+    from helpers import read_vtk
+
+    case_dir = tmp_path / "case"
+    r = subprocess.run([str(DRIVER), "gen", "gate_fsi", str(case_dir), "dx=4e-3;tank_w=0.3;tank_h=0.2;S=2"],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    c = Case.read(case_dir)
+    r = subprocess.run([str(DRIVER), "vtk", str(case_dir), str(tmp_path / "vtk")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    pts, f = read_vtk(tmp_path / "vtk" / "fluid-input.vtk")
+    assert np.array_equal(pts[:, 0], c.fluid["x"]) and np.array_equal(pts[:, 1], c.fluid["y"])
+    assert np.array_equal(f["density"], c.fluid["rho"]) and np.array_equal(f["pressure"], c.fluid["p"])
+    assert np.array_equal(f["phase"], c.fluid["phase"])
+    pts, f = read_vtk(tmp_path / "vtk" / "solid-input.vtk")
+    assert np.array_equal(pts[:, 1], c.solid["y"]) and np.array_equal(f["X"], c.solid["X"])
+    assert np.array_equal(f["material"], c.solid["material"])
+    pts, f = read_vtk(tmp_path / "vtk" / "virt-input.vtk")
+    assert len(pts) == c.counts[2] and np.array_equal(f["boundary_type"], c.wall["bc"])
+    # a regular file where the output directory should be
+    (tmp_path / "blocker").write_text("x")
+    r = subprocess.run([str(DRIVER), "vtk", str(case_dir), str(tmp_path / "blocker" / "out")],
+                       capture_output=True, text=True)
+    assert r.returncode != 0 and "cannot" in r.stderr

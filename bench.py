@@ -239,8 +239,8 @@ def main() -> int:
     ap.add_argument("--accuracy-steps", type=int, default=1000,
                     help="steps after which M32 is compared with P64 (both continued to this count)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default block sizes")
-    ap.add_argument("--solid-launches", action="store_true",
-                    help="solid sub-steps as 2 S launches instead of one launch with grid barriers")
+    ap.add_argument("--solid-one-launch", action="store_true",
+                    help="solid sub-steps as one launch with grid barriers instead of 2 S launches")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run device copies of MIB MiB (counter calibration for rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -273,7 +273,7 @@ def main() -> int:
     n_total = nf + ns + nw
     print(f"# {desc}: {nf} fluid, {ns} solid, {nw} wall", file=sys.stderr, flush=True)
 
-    sim = Simulation(case, "m32", persistent_solid=not args.solid_launches)
+    sim = Simulation(case, "m32", persistent_solid=args.solid_one_launch)
     sim.step(args.warmup)
     tune = sim.autotune() if not args.no_tune else None
     t = timed_steps(sim, args.steps)
@@ -285,7 +285,7 @@ def main() -> int:
 
     extra = {}
     if not args.no_p64:
-        ref = Simulation(case, "p64", persistent_solid=not args.solid_launches)
+        ref = Simulation(case, "p64", persistent_solid=args.solid_one_launch)
         ref.step(args.warmup)
         tune64 = ref.autotune() if not args.no_tune else None
         tp = timed_steps(ref, args.steps)

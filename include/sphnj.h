@@ -229,8 +229,9 @@ void sphnj_destroy(sphnj_ctx *ctx);
 /* Capture the step into hipGraphs (default on) and replay them. */
 int sphnj_set_graph(sphnj_ctx *ctx, int enable);
 /* Run a fluid step's S solid sub-steps (solidCalculation, src/solid.cu:473-492) as one launch
- * whose blocks meet at grid barriers between the deformation and force phases (default on),
- * or as 2 S launches.  Both give the same results bit for bit. */
+ * whose blocks meet at grid barriers between the deformation and force phases, or as 2 S
+ * launches (default).  Both give the same results bit for bit.  On MI355X a grid barrier must
+ * write back and invalidate the L2 of every XCD, which costs more than a launch boundary. */
 int sphnj_set_persistent_solid(sphnj_ctx *ctx, int enable);
 /* Autotuner: times the fluid pair kernel (family 0) and the shift kernel (family 1) at block
  * sizes 64, 128 and 256 -- a block owns that many consecutive sorted particles, so the block

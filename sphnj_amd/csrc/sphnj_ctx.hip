@@ -57,7 +57,7 @@ struct EngineBase {
   std::string err;
   hipStream_t stream = nullptr;
   bool use_graph = true;
-  bool persistent_solid = true;  // S solid sub-steps as one launch (k_solid_substeps)
+  bool persistent_solid = false;  // S solid sub-steps as one launch (k_solid_substeps)
   int blk_pair = 256, blk_shift = 256, blk_stream = 256;
   // per-stage timing: HIP events recorded on the context stream around each stage while on
   bool timing = false;
@@ -1337,7 +1337,7 @@ struct sphnj_ctx {
   double last_ms = 0.0;
   bool inited = false;
   bool use_graph = true;
-  bool persistent_solid = true;
+  bool persistent_solid = false;
   std::string err;
   int fail(int code, const std::string &m) {
     err = m;

@@ -263,7 +263,7 @@ class Simulation(ContextBase):
     prefix = "sphnj_"
 
     def __init__(self, cases: Case | list[Case], precision: str = "p64", graph: bool = True,
-                 persistent_solid: bool = True):
+                 persistent_solid: bool = False):
         lib = library()
         cases = [cases] if isinstance(cases, Case) else list(cases)
         prec = {"p64": A.PREC_P64, "m32": A.PREC_M32}[precision]

@@ -306,14 +306,14 @@ def test_stages_equal_step(gpu, precision):
 
 @pytest.mark.parametrize("precision", ["p64", "m32"])
 def test_one_launch_solid_substeps_equal_launch_pairs(gpu, precision):
-    """The S solid sub-steps of a fluid step as one launch with grid barriers (the default)
-    equal the 2 S launch form bit for bit, through graph replays, on a batch whose cases have
+    """The S solid sub-steps of a fluid step as one launch with grid barriers equal the 2 S
+    launch form (the default) bit for bit, through graph replays, on a batch whose cases have
     different sub-step counts (S = 4 and S = 10: the case with fewer carries its state over)."""
     from test_gpu_parity import small_gate
 
     cases = [small_gate(), small_gate(S=10, gate_t=0.008)]
-    a = Simulation(cases, precision)
-    b = Simulation(cases, precision, persistent_solid=False)
+    a = Simulation(cases, precision, persistent_solid=True)
+    b = Simulation(cases, precision)
     for n in (1, 3, 7):
         a.step(n)
         b.step(n)

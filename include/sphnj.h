@@ -287,6 +287,30 @@ int sphnj_solid_reflist(sphnj_ctx *ctx, int32_t case_index, int32_t *offsets, in
  * out[4] solid strain energy (src/output.cu:426-455 form), out[5] number of fluid particles. */
 int sphnj_energy(sphnj_ctx *ctx, int32_t case_index, double out[6]);
 
+/* ---- probes: the reference's global outputs (src/output.cu:283-582), on the device ------
+ * Computed from the current state of one case by kernels enqueued on the context stream
+ * after the steps already enqueued, outside the step graph; the call synchronises.  The
+ * reference's probe points are hard-coded to its case geometries; here they are arguments. */
+enum {
+  SPHNJ_PROBE_FORCE_X = 0,        /* outputTotalForce (:283-305): sum couple_dudt rho_ref d^2, x  */
+  SPHNJ_PROBE_FORCE_Y = 1,        /*   ... y                                                      */
+  SPHNJ_PROBE_FRONT = 2,          /* outputEdge (:339-369): largest liquid x (at least 0) + d/2  */
+  SPHNJ_PROBE_HEIGHT = 3,         /* outputEdge (:339-369): largest y of a liquid particle within
+                                     d/2 of x = height_x, + d/2 (0 when there is none)          */
+  SPHNJ_PROBE_VM_ELASTIC = 4,     /* outputVonMisesStress (:503-544): largest von Mises stress of
+                                     F^-1 P over the elastic (non-auxetic) solids               */
+  SPHNJ_PROBE_VM_AUXETIC = 5,     /*   ... over the auxetic solids                              */
+  SPHNJ_PROBE_VM_MAX = 6,         /*   ... the larger of the two                                */
+  SPHNJ_PROBE_MEAN_PRESSURE = 7,  /* outputMeanPressure (:546-560): mean solid surface pressure */
+  SPHNJ_PROBE_COUNT = 8
+};
+int sphnj_probe_global(sphnj_ctx *ctx, int32_t case_index, double height_x, double out[SPHNJ_PROBE_COUNT]);
+/* Point probe of outputTankBeamEnd (:457-478), outputEndPosition (:399-424),
+ * outputMiddlePosition (:371-397) and outputWedgeDisp (:307-337): the current position of the
+ * solid particle whose reference position lies within d/2 of (X, Y) (the lowest input index
+ * among several); *index receives that index.  SPHNJ_E_ARG (index -1) when there is none. */
+int sphnj_probe_solid(sphnj_ctx *ctx, int32_t case_index, double X, double Y, double pos[2], int32_t *index);
+
 /* ---- timing (HIP events on the context stream) ------------------------------------------ */
 /* Elapsed time of the last sphnj_step call. */
 int sphnj_last_step_ms(sphnj_ctx *ctx, double *ms);

@@ -1133,6 +1133,64 @@ int oracle_energy(oracle_ctx *o, int32_t ci, double out[6]) {
   return SPHNJ_OK;
 }
 
+// Output::outputTotalForce, outputEdge, outputVonMisesStress, outputMeanPressure
+// (src/output.cu:283-305, 339-369, 503-544, 546-560) with the probe points as arguments.
+int oracle_probe_global(oracle_ctx *o, int32_t ci, double height_x, double out[SPHNJ_PROBE_COUNT]) {
+  if (!o || !out || ci < 0 || ci >= (int)o->C.cases.size()) return SPHNJ_E_ARG;
+  const Case &c = o->C.cases[ci];
+  const Consts &k = c.k;
+  const double d = k.d, d2 = d * d;
+  double fx = 0, fy = 0, front = 0.0, height = -0.5 * d, vm_el = 0, vm_aux = 0, psum = 0;
+  for (size_t i = 0; i < c.fx.size(); ++i) {
+    if (c.fph[i] != SPHNJ_LIQUID) continue;
+    if (c.fx[i] > front) front = c.fx[i];
+    if (c.fx[i] < height_x + 0.5 * d && c.fx[i] > height_x - 0.5 * d && c.fy[i] > height) height = c.fy[i];
+  }
+  const size_t ns = c.sx.size();
+  for (size_t i = 0; i < ns; ++i) {
+    const double rho = rho_ref(k, c.smat[i]);
+    fx += c.scp[2 * i] * d2 * rho;
+    fy += c.scp[2 * i + 1] * d2 * rho;
+    const M2 F = {c.sF[4 * i], c.sF[4 * i + 1], c.sF[4 * i + 2], c.sF[4 * i + 3]};
+    const M2 P = {c.sP[4 * i], c.sP[4 * i + 1], c.sP[4 * i + 2], c.sP[4 * i + 3]};
+    const M2 S = mul(inv(F), P);
+    const double m = std::sqrt(0.5 * (S.a - S.d) * (S.a - S.d) + 3.0 * S.b * S.b);
+    if (c.smat[i] == SPHNJ_AUXETIC)
+      vm_aux = std::max(vm_aux, m);
+    else
+      vm_el = std::max(vm_el, m);
+    psum += c.sp[i];
+  }
+  out[SPHNJ_PROBE_FORCE_X] = fx;
+  out[SPHNJ_PROBE_FORCE_Y] = fy;
+  out[SPHNJ_PROBE_FRONT] = front + 0.5 * d;
+  out[SPHNJ_PROBE_HEIGHT] = height + 0.5 * d;
+  out[SPHNJ_PROBE_VM_ELASTIC] = vm_el;
+  out[SPHNJ_PROBE_VM_AUXETIC] = vm_aux;
+  out[SPHNJ_PROBE_VM_MAX] = std::max(vm_el, vm_aux);
+  out[SPHNJ_PROBE_MEAN_PRESSURE] = ns ? psum / (double)ns : 0.0;
+  return SPHNJ_OK;
+}
+
+// The point probes of Output::outputTankBeamEnd etc. (src/output.cu:457-478): the first solid
+// in index order whose reference position lies within d/2 of (X, Y).
+int oracle_probe_solid(oracle_ctx *o, int32_t ci, double X, double Y, double pos[2], int32_t *index) {
+  if (!o || !pos || ci < 0 || ci >= (int)o->C.cases.size()) return SPHNJ_E_ARG;
+  const Case &c = o->C.cases[ci];
+  const double r = 0.5 * c.k.d;
+  for (size_t i = 0; i < c.sx.size(); ++i) {
+    if (std::hypot(c.sX[i] - X, c.sY[i] - Y) < r) {
+      pos[0] = c.sx[i];
+      pos[1] = c.sy[i];
+      if (index) *index = (int32_t)i;
+      return SPHNJ_OK;
+    }
+  }
+  if (index) *index = -1;
+  g_err = "no solid particle within d/2 of the probe point";
+  return SPHNJ_E_ARG;
+}
+
 int oracle_derived(oracle_ctx *o, double out[15]) {
   if (!o || !out) return SPHNJ_E_ARG;
   const Consts &k = o->C.cases[0].k;

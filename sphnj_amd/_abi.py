@@ -20,6 +20,7 @@ NO_PARTICLE, HAS_PARTICLE, HAS_NEAR_PARTICLE = 0, 1, 2
 LINEAR_ELASTIC, AUXETIC, NEO_HOOKEAN = 0, 1, 2
 UNCONSTRAINED, FIXED = 0, 1
 ST_OUT_OF_GRID, ST_CELL_OVER_64, ST_NAN, ST_REFLIST_LEFT, ST_SYNC_TIMEOUT = 1, 2, 4, 8, 16
+PROBES = ("force_x", "force_y", "front", "height", "vm_elastic", "vm_auxetic", "vm_max", "mean_pressure")
 STAGE_CELLS, STAGE_COUPLE, STAGE_WALL, STAGE_FLUID, STAGE_SOLID, STAGE_SHIFT = 1, 2, 3, 4, 5, 6
 FLUID, SOLID, WALL = 0, 1, 2
 
@@ -121,6 +122,8 @@ CTX_PROTOS = {
     "stage_neighbors": (C.c_int, [VP, C.c_int32, c_int32_p, c_int32_p, C.c_int64, P(C.c_int64)]),
     "solid_reflist": (C.c_int, [VP, C.c_int32, c_int32_p, c_int32_p, C.c_int64]),
     "energy": (C.c_int, [VP, C.c_int32, c_double_p]),
+    "probe_global": (C.c_int, [VP, C.c_int32, C.c_double, c_double_p]),
+    "probe_solid": (C.c_int, [VP, C.c_int32, C.c_double, C.c_double, c_double_p, c_int32_p]),
     "destroy": (None, [VP]),
 }
 

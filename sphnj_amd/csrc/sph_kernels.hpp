@@ -2570,4 +2570,73 @@ __global__ void __launch_bounds__(256) k_energy_partial(Dev<Pol> d, int flo, int
   if (t < 5) part[blockIdx.x * 5 + t] = sm[t][0];
 }
 
+// ---- probes of the reference's global outputs (src/output.cu:283-582) ----------------------
+// One case.  Per-block partials in a fixed order (the host combines them block by block, so a
+// probe's result does not depend on scheduling):
+//   [0], [1] fluid force on the solids, sum couple_dudt rho_ref d^2 (outputTotalForce)
+//   [2] largest x of a liquid particle (outputEdge's frontier), -inf if none
+//   [3] largest y of a liquid particle within d/2 of x = xh (outputEdge's heights), -inf if none
+//   [4], [5] largest von Mises stress of the elastic / auxetic solids, with the stress
+//       F^-1 P (outputVonMisesStress)
+//   [6] sum of the solid pressures (outputMeanPressure)
+constexpr int kProbeParts = 7, kProbeBlocks = 256;
+struct ProbeArgs {
+  int flo, fhi, slo, shi;
+  double d2, rho_s, rho_aux, xh, half_d;
+};
+template <class Pol>
+__global__ void __launch_bounds__(256) k_probe_partial(Dev<Pol> d, ProbeArgs a, double *part) {
+  __shared__ double sm[kProbeParts][256];
+  const int t = threadIdx.x;
+  double v[kProbeParts] = {0.0, 0.0, -INFINITY, -INFINITY, 0.0, 0.0, 0.0};
+  for (int i = a.flo + blockIdx.x * 256 + t; i < a.fhi; i += gridDim.x * 256) {
+    if (d.f_phase[i] != 0) continue;
+    const double x = d.fa.x[i];
+    v[2] = fmax(v[2], x);
+    if (fabs(x - a.xh) < a.half_d) v[3] = fmax(v[3], d.fa.y[i]);
+  }
+  for (int i = a.slo + blockIdx.x * 256 + t; i < a.shi; i += gridDim.x * 256) {
+    const bool aux = d.s_mat[i] == 1;
+    const double rr = aux ? a.rho_aux : a.rho_s;
+    v[0] += (double)d.s_cp[2 * i] * rr * a.d2;
+    v[1] += (double)d.s_cp[2 * i + 1] * rr * a.d2;
+    const double f0 = d.s_F[4 * i], f1 = d.s_F[4 * i + 1], f2 = d.s_F[4 * i + 2], f3 = d.s_F[4 * i + 3];
+    const double p0 = d.s_P[4 * i], p1 = d.s_P[4 * i + 1], p2 = d.s_P[4 * i + 2], p3 = d.s_P[4 * i + 3];
+    const double idet = 1.0 / (f0 * f3 - f1 * f2);  // F^-1 = [[f3, -f1], [-f2, f0]] / det F
+    const double s00 = (f3 * p0 - f1 * p2) * idet, s01 = (f3 * p1 - f1 * p3) * idet;
+    const double s11 = (-f2 * p1 + f0 * p3) * idet;
+    const double vm = sqrt(0.5 * (s00 - s11) * (s00 - s11) + 3.0 * s01 * s01);
+    v[aux ? 5 : 4] = fmax(v[aux ? 5 : 4], vm);
+    v[6] += (double)d.s_p[i];
+  }
+#pragma unroll
+  for (int k = 0; k < kProbeParts; ++k) sm[k][t] = v[k];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w) {
+      sm[0][t] += sm[0][t + w];
+      sm[1][t] += sm[1][t + w];
+      sm[2][t] = fmax(sm[2][t], sm[2][t + w]);
+      sm[3][t] = fmax(sm[3][t], sm[3][t + w]);
+      sm[4][t] = fmax(sm[4][t], sm[4][t + w]);
+      sm[5][t] = fmax(sm[5][t], sm[5][t + w]);
+      sm[6][t] += sm[6][t + w];
+    }
+    __syncthreads();
+  }
+  if (t < kProbeParts) part[blockIdx.x * kProbeParts + t] = sm[t][0];
+}
+
+// The lowest index among the case's solids whose reference position lies within d/2 of
+// (X, Y) (outputTankBeamEnd, outputEndPosition, outputMiddlePosition, outputWedgeDisp find
+// their point this way), into *best (INT_MAX when none); the caller reads its position.
+template <class Pol>
+__global__ void __launch_bounds__(256) k_probe_point(Dev<Pol> d, int slo, int shi, double X, double Y, double r2,
+                                                     int *best) {
+  const int i = slo + blockIdx.x * 256 + (int)threadIdx.x;
+  if (i >= shi) return;
+  const double dX = d.sX[i] - X, dY = d.sY[i] - Y;
+  if (dX * dX + dY * dY < r2) atomicMin(best, i);
+}
+
 }  // namespace sphnj

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <array>
 #include <cmath>
 #include <cstring>
@@ -98,6 +99,8 @@ struct EngineBase {
   virtual int neighbors(int c, int *counts, int *list, int64_t cap, int64_t *total) = 0;
   virtual int reflist(int c, int *off, int *list, int64_t cap) = 0;
   virtual int energy(int c, double out[6]) = 0;
+  virtual int probe_global(int c, double xh, double out[SPHNJ_PROBE_COUNT]) = 0;
+  virtual int probe_solid(int c, double X, double Y, double pos[2], int *index) = 0;
   virtual int stage_time(int st, int *launches, double *ms) = 0;
   virtual int autotune() = 0;
   double tune_ms[2][3] = {{0, 0, 0}, {0, 0, 0}};  // [pair, shift][64, 128, 256]: ms per launch
@@ -130,6 +133,9 @@ struct Engine : EngineBase {
   // solids' blocks resident at half its capacity: the sub-steps fall back to 2 S launches)
   GridSync gsync{};
   int solid_grid = 0;
+  // probe workspace, allocated once: per-block partials and the point probe's index
+  double *probe_part = nullptr;
+  int *probe_best = nullptr;
   static constexpr int kSpinLimit = 1 << 22;  // polls of ~0.1 us: a stuck barrier gives up in < 1 s
   RadixSorter ssort;
   int *s_sorted_keys = nullptr;
@@ -442,6 +448,7 @@ struct Engine : EngineBase {
         (rc = dalloc(&d.s_range, nc)) || (rc = dalloc(&d.s_ref_off, ns + 1)))
       return rc;
     HIPC(hipMemsetAsync(d.s_range, 0, (size_t)nc * sizeof(int2), stream));  // no solid: stays empty
+    if ((rc = dalloc(&probe_part, (size_t)kProbeBlocks * kProbeParts)) || (rc = dalloc(&probe_best, 1))) return rc;
     if (ns > 0) {
       unsigned *words;
       if ((rc = dalloc(&words, 11 * kSyncLine))) return rc;
@@ -1276,6 +1283,64 @@ struct Engine : EngineBase {
     return SPHNJ_OK;
   }
 
+  int probe_global(int c, double xh, double out[SPHNJ_PROBE_COUNT]) override {
+    const Derived &D = Dh[c];
+    const ProbeArgs a{f_off[c], f_off[c + 1], s_off[c], s_off[c + 1], D.d * D.d, D.rho_s, D.rho_aux, xh, 0.5 * D.d};
+    std::vector<double> hp((size_t)kProbeBlocks * kProbeParts);
+    k_probe_partial<Pol><<<kProbeBlocks, 256, 0, stream>>>(d, a, probe_part);
+    LAUNCH_CHECK();
+    int rc;
+    if ((rc = d2h(hp.data(), probe_part, hp.size()))) return rc;
+    HIPC(hipStreamSynchronize(stream));
+    double v[kProbeParts] = {0.0, 0.0, -INFINITY, -INFINITY, 0.0, 0.0, 0.0};
+    for (int b = 0; b < kProbeBlocks; ++b) {  // block order: the same sums every call
+      const double *q = hp.data() + (size_t)b * kProbeParts;
+      v[0] += q[0];
+      v[1] += q[1];
+      v[2] = std::max(v[2], q[2]);
+      v[3] = std::max(v[3], q[3]);
+      v[4] = std::max(v[4], q[4]);
+      v[5] = std::max(v[5], q[5]);
+      v[6] += q[6];
+    }
+    const int ns = s_off[c + 1] - s_off[c];
+    out[SPHNJ_PROBE_FORCE_X] = v[0];
+    out[SPHNJ_PROBE_FORCE_Y] = v[1];
+    out[SPHNJ_PROBE_FRONT] = std::max(v[2], 0.0) + 0.5 * D.d;
+    out[SPHNJ_PROBE_HEIGHT] = std::max(v[3], -0.5 * D.d) + 0.5 * D.d;
+    out[SPHNJ_PROBE_VM_ELASTIC] = v[4];
+    out[SPHNJ_PROBE_VM_AUXETIC] = v[5];
+    out[SPHNJ_PROBE_VM_MAX] = std::max(v[4], v[5]);
+    out[SPHNJ_PROBE_MEAN_PRESSURE] = ns > 0 ? v[6] / ns : 0.0;
+    return SPHNJ_OK;
+  }
+
+  int probe_solid(int c, double X, double Y, double pos[2], int *index) override {
+    const Derived &D = Dh[c];
+    const int lo = s_off[c], hi = s_off[c + 1];
+    const int none = INT_MAX;
+    int rc, best = none;
+    if ((rc = h2d(probe_best, &none, 1))) return rc;
+    if (hi > lo) {
+      k_probe_point<Pol><<<nblk(hi - lo, 256), 256, 0, stream>>>(d, lo, hi, X, Y, 0.25 * D.d * D.d, probe_best);
+      LAUNCH_CHECK();
+    }
+    if ((rc = d2h(&best, probe_best, 1))) return rc;
+    HIPC(hipStreamSynchronize(stream));
+    if (best == none) {
+      if (index) *index = -1;
+      err = "no solid particle has its reference position within d/2 of the probe point";
+      return SPHNJ_E_ARG;
+    }
+    double xy[2];
+    if ((rc = d2h(xy, d.sx + best, 1)) || (rc = d2h(xy + 1, d.sy + best, 1))) return rc;
+    HIPC(hipStreamSynchronize(stream));
+    pos[0] = xy[0];
+    pos[1] = xy[1];
+    if (index) *index = best - lo;
+    return SPHNJ_OK;
+  }
+
   int energy(int c, double out[6]) override {
     int rc;
     const Derived &D = Dh[c];
@@ -1633,6 +1698,18 @@ int sphnj_energy(sphnj_ctx *c, int32_t ci, double out[6]) {
   if (!c || !out || ci < 0 || ci >= c->ncase) return SPHNJ_E_ARG;
   if (!c->inited) return c->fail(SPHNJ_E_STATE, "before init");
   return c->from_engine(c->eng->energy(ci, out));
+}
+
+int sphnj_probe_global(sphnj_ctx *c, int32_t ci, double height_x, double out[SPHNJ_PROBE_COUNT]) {
+  if (!c || !out || ci < 0 || ci >= c->ncase) return SPHNJ_E_ARG;
+  if (!c->inited) return c->fail(SPHNJ_E_STATE, "before init");
+  return c->from_engine(c->eng->probe_global(ci, height_x, out));
+}
+
+int sphnj_probe_solid(sphnj_ctx *c, int32_t ci, double X, double Y, double pos[2], int32_t *index) {
+  if (!c || !pos || ci < 0 || ci >= c->ncase) return SPHNJ_E_ARG;
+  if (!c->inited) return c->fail(SPHNJ_E_STATE, "before init");
+  return c->from_engine(c->eng->probe_solid(ci, X, Y, pos, index));
 }
 
 }  // extern "C"

@@ -256,6 +256,21 @@ class ContextBase:
         self._call("energy", case, _dptr(out))
         return out
 
+    def probe_global(self, case: int = 0, height_x: float = 0.0) -> dict:
+        """The reference's global outputs (src/output.cu:283-560) of the current state:
+        force_x/y, front, height (at x = height_x), vm_elastic, vm_auxetic, vm_max, mean_pressure."""
+        out = np.zeros(len(A.PROBES))
+        self._call("probe_global", case, float(height_x), _dptr(out))
+        return dict(zip(A.PROBES, out.tolist()))
+
+    def probe_solid(self, X: float, Y: float, case: int = 0) -> tuple[np.ndarray, int]:
+        """Current position of the solid particle whose reference position lies within d/2 of
+        (X, Y), and its input index."""
+        pos = np.zeros(2)
+        idx = C.c_int32()
+        self._call("probe_solid", case, float(X), float(Y), _dptr(pos), C.byref(idx))
+        return pos, idx.value
+
 
 class Simulation(ContextBase):
     """One libsphnj context over one case (the drop-in) or a batch of cases."""

@@ -367,3 +367,30 @@ def test_driver_run_writes_what_download_returns(gpu, tmp_path):
     assert np.array_equal(f["P_yy"], d["solid"]["stress"][:, 3])
     _, f = read_vtk(case_dir / "out" / "virt-12.vtk")
     assert np.array_equal(f["pressure"], d["wall"]["p"])
+
+
+@pytest.mark.parametrize("precision,tol", [("p64", 1e-9), ("m32", 1e-4)])
+def test_probes_match_oracle(gpu, oracle_mod, precision, tol):
+    """The device probes after 40 steps of the FSI gate against the oracle's: total force,
+    front, water height at the gate, von Mises stresses, mean pressure, and the gate's free end
+    (outputTankBeamEnd's point, here the plate's bottom-left particle)."""
+    from test_gpu_parity import small_gate
+
+    c = small_gate()
+    sim = Simulation(c, precision)
+    ora = oracle_mod.Oracle(c)
+    sim.step(40)
+    ora.step(40)
+    xh = 0.05
+    a, b = sim.probe_global(0, xh), ora.probe_global(0, xh)
+    for k in b:
+        if b[k] == 0.0:
+            assert a[k] == 0.0, k
+        else:
+            assert abs(a[k] - b[k]) <= tol * abs(b[k]), (k, a[k], b[k])
+    assert b["force_x"] != 0.0 and b["vm_elastic"] > 0.0 and b["height"] > 0.0
+    X0, Y0 = c.solid["X"].min(), c.solid["Y"].min()
+    pa, ia = sim.probe_solid(X0, Y0)
+    pb, ib = ora.probe_solid(X0, Y0)
+    assert ia == ib
+    assert np.abs(pa - pb).max() <= tol * 1e-2

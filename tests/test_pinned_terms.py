@@ -460,3 +460,52 @@ def test_fluid_sees_wall_with_the_wall_phase_density(backend):
     close((g["rho"][0] - 1000.0) / p.dt, d1 + d2, tol, "liquid Drho")
     # the liquid particle ignores both gas-phase neighbours in its colour
     assert g["color"][0] == 0.0
+
+
+# ---- probes of the reference's global outputs (src/output.cu:283-560) ------------------------
+
+def test_probes_closed_forms(backend):
+    """Front and height of a known liquid set; mean solid pressure under a uniform fluid
+    pressure; the von Mises stress of a uniaxial stretch (SVK: S = 2 mu E + lambda tr E I, so
+    sqrt(0.5 (S00 - S11)^2) = sqrt(2) mu e_g); the total force of one solid beside one fluid
+    particle (its coupling acceleration times rho_ref d^2); the point probe."""
+    E, nu, e = 1.0e7, 0.3, 0.01
+    mu = E / (2 * (1 + nu))
+    p = params(gravity=0.0, dt=1e-9, solid_sub_step=1, youngs_modulus=E, poisson_ratio=nu, shift_per_step=0)
+    # solids: a 9 x 9 block stretched by F = diag(1 + e, 1) around (0.4, 0.4), far from the fluid
+    X, Y = lattice(9, 9, 0.02, 0.3, 0.3)
+    sx, sy = 0.4 + (1 + e) * (X - 0.4), Y.copy()
+    # fluid: a liquid block and a gas particle further right (gas is not in the front)
+    fx, fy = lattice(5, 4, 0.02, 0.7, 0.1)
+    fx = np.concatenate([fx, [0.95]])
+    fy = np.concatenate([fy, [0.5]])
+    ph = np.concatenate([np.zeros(20, np.int32), [A.GAS]]).astype(np.int32)
+    c = case(p, fluid(fx, fy, p=700.0, phase=ph), s=solid(sx, sy, X, Y))
+    s = run(backend, c)
+    for st in ("cells", "couple", "solid"):
+        s.stage(st)
+    g = s.probe_global(0, height_x=0.75)
+    tol = TOL[backend]
+    close(g["front"], 0.7 + 4.5 * 0.02 + 0.01, 1e-15, "front")
+    close(g["height"], 0.1 + 3.5 * 0.02 + 0.01, 1e-15, "height")
+    assert s.probe_global(0, height_x=0.2)["height"] == 0.0     # no liquid within d/2 of x = 0.2
+    eg = e + e * e / 2
+    close(g["vm_elastic"], math.sqrt(2) * mu * eg, tol if backend != "m32" else 1e-4, "von Mises")
+    assert g["vm_auxetic"] == 0.0 and g["vm_max"] == g["vm_elastic"]
+    assert g["mean_pressure"] == 0.0                            # no fluid within reach of the block
+    pos, idx = s.probe_solid(X[40], Y[40])
+    assert idx == 40
+    close(pos, [sx[40], sy[40]], 1e-15, "point probe")
+    with pytest.raises(Exception):
+        s.probe_solid(5.0, 5.0)
+    # one solid beside one fluid particle at uniform pressure: mean pressure = p, and the
+    # total force is its coupling acceleration times rho_ref d^2
+    c2 = case(p, fluid([0.53], [0.5], p=700.0), s=solid([0.5], [0.5], material=A.AUXETIC))
+    s2 = run(backend, c2)
+    for st in ("cells", "couple", "solid"):
+        s2.stage(st)
+    g2 = s2.probe_global(0)
+    close(g2["mean_pressure"], 700.0, tol, "mean pressure")
+    cp = s2.download()["solid"]["couple_dudt"][0]
+    close([g2["force_x"], g2["force_y"]], cp * 636.0 * 0.02 ** 2, 1e-12, "total force")
+    assert g2["force_x"] < 0          # the fluid pushes the solid away from it

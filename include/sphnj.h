@@ -287,6 +287,21 @@ int sphnj_solid_reflist(sphnj_ctx *ctx, int32_t case_index, int32_t *offsets, in
  * out[4] solid strain energy (src/output.cu:426-455 form), out[5] number of fluid particles. */
 int sphnj_energy(sphnj_ctx *ctx, int32_t case_index, double out[6]);
 
+/* ---- snapshots: binary restart files of one case ----------------------------------------
+ * sphnj_snapshot_write: the case's state in input order -- what sphnj_upload takes plus what a
+ *   step carries into the next (fluid boundary class, solid max_strain, surface pressure and
+ *   normal) -- with its parameters and the caller's step count, to path (SoA, doubles and
+ *   int32, a 64-bit hash at the end).  Synchronises the context.
+ * sphnj_snapshot_info: the parameters, step count and (fluid, solid, wall) counts of a snapshot.
+ * sphnj_snapshot_load: in place of sphnj_upload for case_index (before sphnj_init), from a
+ *   snapshot whose parameters equal the context's for that case (SPHNJ_E_PARAM otherwise;
+ *   SPHNJ_E_IO for a damaged file).  Stepping the restored context gives the same results,
+ *   bit for bit, as stepping the context the snapshot was written from.
+ * The reference has no restart files (its output is VTK only, src/output.cu:7-281). */
+int sphnj_snapshot_write(sphnj_ctx *ctx, int32_t case_index, int64_t step, const char *path);
+int sphnj_snapshot_info(const char *path, sphnj_params *params, int64_t *step, int32_t counts[3]);
+int sphnj_snapshot_load(sphnj_ctx *ctx, int32_t case_index, const char *path);
+
 /* ---- probes: the reference's global outputs (src/output.cu:283-582), on the device ------
  * Computed from the current state of one case by kernels enqueued on the context stream
  * after the steps already enqueued, outside the step graph; the call synchronises.  The

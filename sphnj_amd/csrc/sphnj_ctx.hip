@@ -8,6 +8,7 @@
 #include <climits>
 #include <array>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -52,6 +53,10 @@ struct Staged {
   std::vector<int8_t> smat, sbc;
   std::vector<double> wx, wy;
   std::vector<int8_t> wbc;
+  // state a step carries over, restored from a snapshot (empty: the initial values of
+  // include/host_data.cuh -- class FREE_SURFACE, max_strain -yield_strain I, p_s = 0, n_s = 0)
+  std::vector<int8_t> fcls;
+  std::vector<double> sM, sp, sn;
 };
 
 struct EngineBase {
@@ -420,7 +425,14 @@ struct Engine : EngineBase {
       }
       std::vector<int> orig(nf);
       std::iota(orig.begin(), orig.end(), 0);
-      std::vector<int8_t> cls(nf, (int8_t)SPHNJ_FREE_SURFACE), nearv(nf, 0);
+      std::vector<int8_t> cls, nearv(nf, 0);
+      for (int c = 0; c < ncase; ++c) {
+        const Staged &s = St[c];
+        if (s.fcls.empty())
+          cls.insert(cls.end(), s.fx.size(), (int8_t)SPHNJ_FREE_SURFACE);
+        else
+          cls.insert(cls.end(), s.fcls.begin(), s.fcls.end());
+      }
       std::vector<R> zero4(4 * (size_t)nf, (R)0), zero1(nf, (R)0);
       if ((rc = h2d(d.fa.x, x.data(), nf)) || (rc = h2d(d.fa.y, y.data(), nf)) || (rc = h2d(d.fa.u, u.data(), nf)) ||
           (rc = h2d(d.fa.v, v.data(), nf)) || (rc = h2d(d.fa.rho, rho.data(), nf)) ||
@@ -490,15 +502,20 @@ struct Engine : EngineBase {
         bc.insert(bc.end(), s.sbc.begin(), s.sbc.end());
         for (size_t i = 0; i < s.sx.size(); ++i) {  // include/host_data.cuh:77-83
           const R ys = (R)(-D[c].yield_strain);
-          M4.insert(M4.end(), {ys, (R)0, (R)0, ys});
+          if (s.sM.empty())
+            M4.insert(M4.end(), {ys, (R)0, (R)0, ys});
+          else
+            M4.insert(M4.end(), {(R)s.sM[4 * i], (R)s.sM[4 * i + 1], (R)s.sM[4 * i + 2], (R)s.sM[4 * i + 3]});
+          Z1.push_back(s.sp.empty() ? (R)0 : (R)s.sp[i]);
+          Z2.push_back(s.sn.empty() ? (R)0 : (R)s.sn[2 * i]);
+          Z2.push_back(s.sn.empty() ? (R)0 : (R)s.sn[2 * i + 1]);
         }
       }
       for (int i = 0; i < ns; ++i) {
         I4.insert(I4.end(), {(R)1, (R)0, (R)0, (R)1});
         Z4.insert(Z4.end(), {(R)0, (R)0, (R)0, (R)0});
       }
-      Z2.assign(2 * (size_t)ns, (R)0);
-      Z1.assign(ns, (R)0);
+
       std::vector<double> Z2d(2 * (size_t)ns, 0.0);
       if ((rc = h2d(d.sx, x.data(), ns)) || (rc = h2d(d.sy, y.data(), ns)) || (rc = h2d(d.su, u.data(), ns)) ||
           (rc = h2d(d.sv, v.data(), ns)) || (rc = h2d(d.sX, X.data(), ns)) || (rc = h2d(d.sY, Y.data(), ns)) ||
@@ -1396,6 +1413,8 @@ struct sphnj_ctx {
   std::vector<sphnj_params> P;
   std::vector<Derived> D;
   std::vector<Staged> staged;
+  // per case, kept after init for snapshots: solid material and bc, wall bc (input order)
+  std::vector<std::vector<int32_t>> smat, sbc;
   std::unique_ptr<EngineBase> eng;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -1524,6 +1543,12 @@ int sphnj_init(sphnj_ctx *c) {
     c->err = c->eng->err;
     c->eng.reset();
     return rc;
+  }
+  c->smat.assign(c->ncase, {});
+  c->sbc.assign(c->ncase, {});
+  for (int i = 0; i < c->ncase; ++i) {
+    c->smat[i].assign(c->staged[i].smat.begin(), c->staged[i].smat.end());
+    c->sbc[i].assign(c->staged[i].sbc.begin(), c->staged[i].sbc.end());
   }
   c->staged.clear();
   c->staged.shrink_to_fit();
@@ -1698,6 +1723,202 @@ int sphnj_energy(sphnj_ctx *c, int32_t ci, double out[6]) {
   if (!c || !out || ci < 0 || ci >= c->ncase) return SPHNJ_E_ARG;
   if (!c->inited) return c->fail(SPHNJ_E_STATE, "before init");
   return c->from_engine(c->eng->energy(ci, out));
+}
+
+// ---- snapshots ------------------------------------------------------------------------
+// Layout (little-endian): "SPHNJSNP", uint32 version, uint32 sizeof(sphnj_params), the
+// parameter struct, int64 step, int32 nf ns nw; fluid x y u v rho p (double), phase class
+// (int32); solid x y u v X Y p (double), normal (2 per), max_strain (4 per) (double),
+// material bc (int32); wall x y (double), bc (int32); uint64 FNV-1a hash of everything
+// before it.
+}  // extern "C"
+
+namespace {
+constexpr char kSnapMagic[8] = {'S', 'P', 'H', 'N', 'J', 'S', 'N', 'P'};
+constexpr uint32_t kSnapVersion = 1;
+
+struct SnapWriter {
+  std::FILE *f;
+  uint64_t h = 1469598103934665603ull;
+  bool ok = true;
+  void put(const void *p, size_t n) {
+    const auto *b = static_cast<const unsigned char *>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+    ok = ok && std::fwrite(p, 1, n, f) == n;
+  }
+  template <class T>
+  void vec(const std::vector<T> &v) {
+    if (!v.empty()) put(v.data(), v.size() * sizeof(T));
+  }
+};
+struct SnapReader {
+  std::FILE *f;
+  uint64_t h = 1469598103934665603ull;
+  bool ok = true;
+  void get(void *p, size_t n) {
+    ok = ok && std::fread(p, 1, n, f) == n;
+    if (!ok) return;
+    const auto *b = static_cast<const unsigned char *>(p);
+    for (size_t i = 0; i < n; ++i) h = (h ^ b[i]) * 1099511628211ull;
+  }
+  template <class T>
+  void vec(std::vector<T> &v, size_t n) {
+    v.resize(n);
+    if (n) get(v.data(), n * sizeof(T));
+  }
+};
+
+struct SnapHeader {
+  sphnj_params P;
+  int64_t step = 0;
+  int32_t n[3] = {0, 0, 0};
+};
+
+// Reads and checks the header; leaves the reader at the particle arrays.
+int snap_header(SnapReader &r, SnapHeader *hd, std::string *err) {
+  char magic[8];
+  uint32_t ver = 0, psize = 0;
+  r.get(magic, 8);
+  r.get(&ver, 4);
+  r.get(&psize, 4);
+  if (!r.ok || std::memcmp(magic, kSnapMagic, 8) != 0) {
+    *err = "not a sphnj snapshot";
+    return SPHNJ_E_IO;
+  }
+  if (ver != kSnapVersion || psize != sizeof(sphnj_params)) {
+    *err = "snapshot written by another version of the library";
+    return SPHNJ_E_IO;
+  }
+  r.get(&hd->P, sizeof(sphnj_params));
+  r.get(&hd->step, 8);
+  r.get(hd->n, 12);
+  if (!r.ok || hd->n[0] < 0 || hd->n[1] < 0 || hd->n[2] < 0) {
+    *err = "truncated snapshot header";
+    return SPHNJ_E_IO;
+  }
+  return SPHNJ_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int sphnj_snapshot_write(sphnj_ctx *c, int32_t ci, int64_t step, const char *path) {
+  if (!c || !path || ci < 0 || ci >= c->ncase) return SPHNJ_E_ARG;
+  if (!c->inited) return c->fail(SPHNJ_E_STATE, "before init");
+  int32_t n[3];
+  sphnj_counts(c, ci, &n[0], &n[1], &n[2], nullptr);
+  const size_t nf = n[0], ns = n[1], nw = n[2];
+  std::vector<double> fx(nf), fy(nf), fu(nf), fv(nf), fr(nf), fp(nf);
+  std::vector<int32_t> fph(nf), fcl(nf);
+  std::vector<double> sx(ns), sy(ns), su(ns), sv(ns), sX(ns), sY(ns), sp(ns), sn(2 * ns), sM(4 * ns);
+  std::vector<double> wx(nw), wy(nw);
+  std::vector<int32_t> wbc(nw);
+  sphnj_fluid_out fo{};
+  fo.x = fx.data(); fo.y = fy.data(); fo.u = fu.data(); fo.v = fv.data(); fo.rho = fr.data(); fo.p = fp.data();
+  fo.phase = fph.data(); fo.boundary_type = fcl.data();
+  sphnj_solid_out so{};
+  so.x = sx.data(); so.y = sy.data(); so.u = su.data(); so.v = sv.data(); so.X = sX.data(); so.Y = sY.data();
+  so.pressure = sp.data(); so.normal = sn.data(); so.max_strain = sM.data();
+  sphnj_wall_out wo{};
+  wo.x = wx.data(); wo.y = wy.data(); wo.bc = wbc.data();
+  int rc = sphnj_download(c, ci, &fo, &so, &wo);
+  if (rc) return rc;
+  std::FILE *f = std::fopen(path, "wb");
+  if (!f) return c->fail(SPHNJ_E_IO, std::string("cannot open ") + path);
+  SnapWriter w{f};
+  const uint32_t ver = kSnapVersion, psize = sizeof(sphnj_params);
+  w.put(kSnapMagic, 8);
+  w.put(&ver, 4);
+  w.put(&psize, 4);
+  w.put(&c->P[ci], sizeof(sphnj_params));
+  w.put(&step, 8);
+  w.put(n, 12);
+  for (auto *v : {&fx, &fy, &fu, &fv, &fr, &fp}) w.vec(*v);
+  w.vec(fph);
+  w.vec(fcl);
+  for (auto *v : {&sx, &sy, &su, &sv, &sX, &sY, &sp, &sn, &sM}) w.vec(*v);
+  w.vec(c->smat[ci]);
+  w.vec(c->sbc[ci]);
+  w.vec(wx);
+  w.vec(wy);
+  w.vec(wbc);
+  const uint64_t hash = w.h;
+  w.put(&hash, 8);
+  const bool closed = std::fclose(f) == 0;
+  if (!w.ok || !closed) return c->fail(SPHNJ_E_IO, std::string("error writing ") + path);
+  return SPHNJ_OK;
+}
+
+int sphnj_snapshot_info(const char *path, sphnj_params *params, int64_t *step, int32_t counts[3]) {
+  if (!path) return SPHNJ_E_ARG;
+  std::FILE *f = std::fopen(path, "rb");
+  if (!f) {
+    set_global_error(std::string("cannot open ") + path);
+    return SPHNJ_E_IO;
+  }
+  SnapReader r{f};
+  SnapHeader hd;
+  std::string err;
+  const int rc = snap_header(r, &hd, &err);
+  std::fclose(f);
+  if (rc) {
+    set_global_error(err);
+    return rc;
+  }
+  if (params) *params = hd.P;
+  if (step) *step = hd.step;
+  if (counts) std::memcpy(counts, hd.n, sizeof(hd.n));
+  return SPHNJ_OK;
+}
+
+int sphnj_snapshot_load(sphnj_ctx *c, int32_t ci, const char *path) {
+  if (!c || !path) return SPHNJ_E_ARG;
+  if (ci < 0 || ci >= c->ncase) return c->fail(SPHNJ_E_ARG, "case index out of range");
+  if (c->inited) return c->fail(SPHNJ_E_STATE, "load after init");
+  std::FILE *f = std::fopen(path, "rb");
+  if (!f) return c->fail(SPHNJ_E_IO, std::string("cannot open ") + path);
+  SnapReader r{f};
+  SnapHeader hd;
+  std::string err;
+  int rc = snap_header(r, &hd, &err);
+  if (rc) {
+    std::fclose(f);
+    return c->fail(rc, err);
+  }
+  const size_t nf = hd.n[0], ns = hd.n[1], nw = hd.n[2];
+  std::vector<double> fx, fy, fu, fv, fr, fp, sx, sy, su, sv, sX, sY, sp, sn, sM, wx, wy;
+  std::vector<int32_t> fph, fcl, smat, sbc, wbc;
+  for (auto *v : {&fx, &fy, &fu, &fv, &fr, &fp}) r.vec(*v, nf);
+  r.vec(fph, nf);
+  r.vec(fcl, nf);
+  for (auto *v : {&sx, &sy, &su, &sv, &sX, &sY, &sp}) r.vec(*v, ns);
+  r.vec(sn, 2 * ns);
+  r.vec(sM, 4 * ns);
+  r.vec(smat, ns);
+  r.vec(sbc, ns);
+  r.vec(wx, nw);
+  r.vec(wy, nw);
+  r.vec(wbc, nw);
+  const uint64_t expect = r.h;
+  uint64_t hash = 0;
+  r.get(&hash, 8);
+  char extra;
+  const bool tail = std::fread(&extra, 1, 1, f) == 1;
+  std::fclose(f);
+  if (!r.ok || hash != expect || tail) return c->fail(SPHNJ_E_IO, "snapshot truncated or damaged (hash mismatch)");
+  if (std::memcmp(&hd.P, &c->P[ci], sizeof(sphnj_params)) != 0)
+    return c->fail(SPHNJ_E_PARAM, "snapshot parameters differ from the context's for this case");
+  const sphnj_fluid_in fi{hd.n[0], fx.data(), fy.data(), fu.data(), fv.data(), fr.data(), fp.data(), fph.data()};
+  const sphnj_solid_in si{hd.n[1], sx.data(), sy.data(), su.data(), sv.data(), sX.data(), sY.data(), smat.data(),
+                          sbc.data()};
+  const sphnj_wall_in wi{hd.n[2], wx.data(), wy.data(), wbc.data()};
+  if ((rc = sphnj_upload(c, ci, &fi, &si, &wi))) return rc;
+  Staged &S = c->staged[ci];
+  S.fcls.assign(fcl.begin(), fcl.end());
+  S.sM = std::move(sM);
+  S.sp = std::move(sp);
+  S.sn = std::move(sn);
+  return SPHNJ_OK;
 }
 
 int sphnj_probe_global(sphnj_ctx *c, int32_t ci, double height_x, double out[SPHNJ_PROBE_COUNT]) {

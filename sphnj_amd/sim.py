@@ -78,6 +78,15 @@ def grid(p: A.Params) -> tuple[int, int, float, float]:
     return nx.value, ny.value, dx.value, dy.value
 
 
+def snapshot_info(path: str | Path):
+    """(params, step, (n_fluid, n_solid, n_wall)) of a snapshot file."""
+    p = A.Params()
+    step = C.c_int64()
+    n = (C.c_int32 * 3)()
+    _check(library().sphnj_snapshot_info(str(path).encode(), C.byref(p), C.byref(step), n), "snapshot_info")
+    return p, step.value, tuple(n)
+
+
 class Case:
     """One case: a parameter set plus its particles (the contents of config/input.ini and
     data/*.dat), as numpy arrays in input (original index) order."""
@@ -317,6 +326,32 @@ class Simulation(ContextBase):
             self._call("get_block", fam, C.byref(blk))
             out[name] = {"ms": dict(zip((64, 128, 256), ms.tolist())), "block": blk.value}
         return out
+
+    def snapshot(self, path: str | Path, step: int = 0, case: int = 0) -> None:
+        """Binary restart file of one case (sphnj_snapshot_write)."""
+        self._call("snapshot_write", case, int(step), str(path).encode())
+
+    @classmethod
+    def restore(cls, paths, precision: str = "p64", graph: bool = True, persistent_solid: bool = False):
+        """A context whose cases come from snapshots (sphnj_snapshot_load in place of uploads)."""
+        paths = [paths] if isinstance(paths, (str, Path)) else list(paths)
+        infos = [snapshot_info(p) for p in paths]
+        lib = library()
+        prec = {"p64": A.PREC_P64, "m32": A.PREC_M32}[precision]
+        arr = (A.Params * len(paths))(*[i[0] for i in infos])
+        h = C.c_void_p()
+        _check(lib.sphnj_batch_create(arr, len(paths), prec, C.byref(h)), "batch_create")
+        self = cls.__new__(cls)
+        z = lambda n: np.zeros(n)  # noqa: E731
+        cases = [Case(p, {"x": z(nf)}, {"x": z(ns)}, {"x": z(nw)}, 0) for p, _, (nf, ns, nw) in infos]
+        ContextBase.__init__(self, lib, h, cases)
+        self.precision = precision
+        _check(lib.sphnj_set_graph(h, int(graph)), "set_graph")
+        _check(lib.sphnj_set_persistent_solid(h, int(persistent_solid)), "set_persistent_solid")
+        for i, p in enumerate(paths):
+            self._call("snapshot_load", i, str(p).encode())
+        self._call("init")
+        return self
 
     def set_stage_timing(self, on: bool) -> None:
         self._call("set_stage_timing", int(on))

@@ -394,3 +394,44 @@ def test_probes_match_oracle(gpu, oracle_mod, precision, tol):
     pb, ib = ora.probe_solid(X0, Y0)
     assert ia == ib
     assert np.abs(pa - pb).max() <= tol * 1e-2
+
+
+@pytest.mark.parametrize("precision", ["p64", "m32"])
+def test_snapshot_restart_is_bit_exact(gpu, tmp_path, precision):
+    """Write a snapshot after 15 steps, restore it into a new context, and step both on: the
+    restored run equals the original bit for bit (an auxetic gate, so the max_strain history
+    and the fluid boundary class must both come back).  A damaged file or a context with other
+    parameters is refused."""
+    from sphnj_amd.sim import snapshot_info
+
+    c = Case.generate("gate_fsi", dx=2.0e-3, tank_w=0.3, tank_h=0.2, S=4, material=1, K=2)
+    c.params.is_plastic = 1          # a soft, plastic auxetic gate, so it yields within the run
+    c.params.yield_stress, c.params.yield_strain = 1.0e2, 1.0e-5
+    a = Simulation(c, precision)
+    a.step(15)
+    f = tmp_path / "gate.snap"
+    a.snapshot(f, step=15)
+    p, step, counts = snapshot_info(f)
+    assert step == 15 and counts == c.counts and p.yield_strain == 1.0e-5
+    b = Simulation.restore(f, precision)
+    for n in (1, 9):
+        a.step(n)
+        b.step(n)
+        da, db = a.download(), b.download()
+        for sp in ("fluid", "solid", "wall"):
+            for k in da[sp]:
+                assert np.array_equal(da[sp][k], db[sp][k]), (n, sp, k)
+    assert np.any(da["solid"]["max_strain"][:, 0] < -1.0e-5), "some particle must have yielded"
+    raw = bytearray(f.read_bytes())
+    raw[200] ^= 1
+    g = tmp_path / "damaged.snap"
+    g.write_bytes(bytes(raw))
+    with pytest.raises(Exception, match="damaged"):
+        Simulation.restore(g, precision)
+    other = Case.generate("gate_fsi", dx=2.0e-3, tank_w=0.3, tank_h=0.2, S=5)
+    import ctypes as C
+    from sphnj_amd import _abi as AB, library
+    h = C.c_void_p()
+    assert library().sphnj_create(C.byref(other.params), 0, C.byref(h)) == 0
+    assert library().sphnj_snapshot_load(h, 0, str(f).encode()) == AB.E_PARAM
+    library().sphnj_destroy(h)

@@ -366,3 +366,20 @@ def test_driver_writes_input_particles_as_vtk(tmp_path):
     r = subprocess.run([str(DRIVER), "vtk", str(case_dir), str(tmp_path / "blocker" / "out")],
                        capture_output=True, text=True)
     assert r.returncode != 0 and "cannot" in r.stderr
+
+
+def test_snapshot_info_rejects_other_files(tmp_path):
+    from sphnj_amd.sim import snapshot_info
+
+    with pytest.raises(SphnjError) as e:
+        snapshot_info(tmp_path / "none.snap")
+    assert e.value.code == A.E_IO
+    f = tmp_path / "bad.snap"
+    f.write_bytes(b"SPHNJSNQ" + bytes(200))
+    with pytest.raises(SphnjError) as e:
+        snapshot_info(f)
+    assert e.value.code == A.E_IO and "not a sphnj snapshot" in str(e.value)
+    f.write_bytes(b"SPHNJSNP" + bytes(4))
+    with pytest.raises(SphnjError) as e:
+        snapshot_info(f)
+    assert e.value.code == A.E_IO

@@ -35,7 +35,7 @@
 extern "C" {
 #endif
 
-#define SPHNJ_ABI_VERSION 1
+#define SPHNJ_ABI_VERSION 2
 
 /* ---- error codes -------------------------------------------------------------------- */
 enum {
@@ -120,6 +120,11 @@ typedef struct sphnj_params {
   /* extensions (no counterpart in the reference; defaults reproduce it exactly) */
   double background_pressure;  /* 0: added to both phases' Tait EOS                          */
   double kernel_2d_coef;       /* 0 = the reference constant COEF_3D_TO_2D = 2.120729        */
+  double adaptive_cfl;         /* 0 = every step takes dt (the reference).  C > 0: each step
+                                  takes dt_n = min(dt, C h / (c_max + U), S C h / c_s) with U
+                                  the largest fluid speed after the previous step's update (the
+                                  input speeds for the first step), c_max the larger phase sound
+                                  speed, the last term only with solids                        */
 } sphnj_params;
 
 /* ---- host-side particle input (caller-owned, copied during sphnj_upload) -------------- */
@@ -286,6 +291,11 @@ int sphnj_solid_reflist(sphnj_ctx *ctx, int32_t case_index, int32_t *offsets, in
  * out[1] fluid mass sum(rho d^2), out[2] max |u| over fluid, out[3] solid kinetic energy,
  * out[4] solid strain energy (src/output.cu:426-455 form), out[5] number of fluid particles. */
 int sphnj_energy(sphnj_ctx *ctx, int32_t case_index, double out[6]);
+
+/* Simulated time of one case (the sum of the step sizes taken since sphnj_init) and the step
+ * size of its last step (SimTime::current_time, src/sim_time.cpp:27-55; with adaptive_cfl the
+ * size varies from step to step).  Synchronises the context. */
+int sphnj_time(sphnj_ctx *ctx, int32_t case_index, double *time, double *last_dt);
 
 /* ---- snapshots: binary restart files of one case ----------------------------------------
  * sphnj_snapshot_write: the case's state in input order -- what sphnj_upload takes plus what a

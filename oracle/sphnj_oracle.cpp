@@ -51,6 +51,7 @@ struct Consts {  // restates Kernel/FluidProperties/SolidProperties/Domain::init
   double dxmin, dymin, dxmax, dymax;
   double dt, dt_s;
   int S, K;
+  double cfl, cfl_h, c_max, dt_solid_cap;  // adaptive step size (extension; cfl = 0: off)
 };
 
 bool make_consts(const sphnj_params &p, Consts *k) {
@@ -132,6 +133,10 @@ bool make_consts(const sphnj_params &p, Consts *k) {
   c.S = p.solid_sub_step > 0 ? p.solid_sub_step : 1;
   c.K = p.shift_per_step > 0 ? p.shift_per_step : 0;
   c.dt_s = p.dt / (double)c.S;
+  c.cfl = p.adaptive_cfl > 0.0 ? p.adaptive_cfl : 0.0;
+  c.cfl_h = c.cfl * c.h;
+  c.c_max = std::max(cc[0], cc[1]);
+  c.dt_solid_cap = (double)c.S * c.cfl_h / c.c_s;
   return true;
 }
 
@@ -182,6 +187,9 @@ struct Case {
   std::vector<int> wph, wbc;
   CellIndex cf, cs, cw;
   bool uploaded = false;
+  // step size: the current step's, the time at its end, and max |u|^2 after the last fluid
+  // update (the input speeds before the first)
+  double dt_cur = 0.0, t_sim = 0.0, u2_last = 0.0;
 };
 
 struct Ctx {
@@ -444,9 +452,10 @@ void fluid_calc(Case &c) {
       ax += -0.1 * ui;
       ay += -0.1 * vi;
     }
-    double x = xi + ui * k.dt, y = yi + vi * k.dt;
-    const double u = ui + (ax + k.gx) * k.dt, v = vi + (ay + k.gy) * k.dt;
-    double rho = rhoi + A.drho * k.dt;
+    const double dt = c.dt_cur;
+    double x = xi + ui * dt, y = yi + vi * dt;
+    const double u = ui + (ax + k.gx) * dt, v = vi + (ay + k.gy) * dt;
+    double rho = rhoi + A.drho * dt;
     rho = std::max(rho, k.rho_min[phi]);
     const double p = rho_to_p(k, rho, phi);
     x = std::min(std::max(x, k.dxmin), k.dxmax);
@@ -464,6 +473,24 @@ void fluid_calc(Case &c) {
   c.fv.swap(nv_);
   c.frho.swap(nr_);
   c.fp.swap(np_);
+  double m2 = 0.0;  // the speed the next step's size is taken from
+  for (int i = 0; i < n; ++i) m2 = std::max(m2, c.fu[i] * c.fu[i] + c.fv[i] * c.fv[i]);
+  c.u2_last = m2;
+}
+
+// The step size of the step that begins (extension adaptive_cfl; the reference's SimTime
+// steps by the fixed dt, src/sim_time.cpp:27-55): dt, or min(dt, C h / (c_max + U),
+// S C h / c_s) with U the largest fluid speed after the last update, the last term only
+// with solids.
+void step_size(Case &c) {
+  const Consts &k = c.k;
+  double dt = k.dt;
+  if (k.cfl > 0.0) {
+    dt = std::min(dt, k.cfl_h / (k.c_max + std::sqrt(c.u2_last)));
+    if (!c.sx.empty()) dt = std::min(dt, k.dt_solid_cap);
+  }
+  c.dt_cur = dt;
+  c.t_sim += dt;
 }
 
 // fluidCorrection (src/fluid.cu:442-459): getMaxSpeed once, then shift_per_step passes of
@@ -515,7 +542,7 @@ void fluid_shift(Case &c, unsigned *flags) {
     }
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) {
-      const double coef = k.shift_coef * k.h * k.dt;
+      const double coef = k.shift_coef * k.h * c.dt_cur;
       const double dxs = -coef * gx[i], dys = -coef * gy[i];
       double sx = dxs, sy = dys;
       int cls;
@@ -686,6 +713,7 @@ void solid_substeps(Case &c, unsigned *flags) {
   const int n = (int)c.sx.size();
   if (n == 0) return;
   std::vector<double> nx_(n), ny_(n), nu_(n), nv_(n);
+  const double dt_s = c.dt_cur / (double)k.S;  // src/solid.cu:456
   for (int sub = 0; sub < k.S; ++sub) {
 #pragma omp parallel for schedule(static)
     for (int i = 0; i < n; ++i) {
@@ -780,8 +808,8 @@ void solid_substeps(Case &c, unsigned *flags) {
       if (c.sbc[i] == SPHNJ_FIXED) {  // solidUpdate
         u = k.fixed_u;
         v = k.fixed_v;
-        x += u * k.dt_s;
-        y += v * k.dt_s;
+        x += u * dt_s;
+        y += v * dt_s;
         ax_ = 0.0;
         ay_ = 0.0;
       } else {
@@ -789,10 +817,10 @@ void solid_substeps(Case &c, unsigned *flags) {
           ax_ += -0.1 * ui;
           ay_ += -0.1 * vi;
         }
-        x += u * k.dt_s;
-        y += v * k.dt_s;
-        u += (ax_ + k.gx) * k.dt_s;
-        v += (ay_ + k.gy) * k.dt_s;
+        x += u * dt_s;
+        y += v * dt_s;
+        u += (ax_ + k.gx) * dt_s;
+        v += (ay_ + k.gy) * dt_s;
       }
       c.sa[2 * i] = ax_;
       c.sa[2 * i + 1] = ay_;
@@ -832,7 +860,8 @@ void build_reflists(Case &c, unsigned *flags) {
 void stage(Ctx &C, int st) {
   for (auto &c : C.cases) {
     switch (st) {
-      case SPHNJ_STAGE_CELLS:
+      case SPHNJ_STAGE_CELLS:  // a step begins: its size first
+        step_size(c);
         hash_species(c, c.fx, c.fy, &c.cf, &C.flags, true);
         hash_species(c, c.sx, c.sy, &c.cs, &C.flags);
         {
@@ -965,6 +994,10 @@ int oracle_upload(oracle_ctx *o, int32_t ci, const sphnj_fluid_in *f, const sphn
   c.wrho.assign(nw, c.k.rho0[0]);  // include/host_data.cuh:111
   c.wph.assign(nw, SPHNJ_LIQUID);
   c.wn.assign(2 * (size_t)nw, 0.0);
+  c.u2_last = 0.0;
+  for (int i = 0; i < nf; ++i) c.u2_last = std::max(c.u2_last, c.fu[i] * c.fu[i] + c.fv[i] * c.fv[i]);
+  c.dt_cur = c.k.dt;
+  c.t_sim = 0.0;
   c.uploaded = true;
   return SPHNJ_OK;
 }
@@ -1189,6 +1222,13 @@ int oracle_probe_solid(oracle_ctx *o, int32_t ci, double X, double Y, double pos
   if (index) *index = -1;
   g_err = "no solid particle within d/2 of the probe point";
   return SPHNJ_E_ARG;
+}
+
+int oracle_time(oracle_ctx *o, int32_t ci, double *time, double *last_dt) {
+  if (!o || ci < 0 || ci >= (int)o->C.cases.size()) return SPHNJ_E_ARG;
+  if (time) *time = o->C.cases[ci].t_sim;
+  if (last_dt) *last_dt = o->C.cases[ci].dt_cur;
+  return SPHNJ_OK;
 }
 
 int oracle_derived(oracle_ctx *o, double out[15]) {

@@ -35,6 +35,7 @@ int oracle_stage_neighbors(oracle_ctx *o, int32_t case_index, int32_t *counts, i
                            int64_t *total);
 int oracle_solid_reflist(oracle_ctx *o, int32_t case_index, int32_t *offsets, int32_t *list, int64_t cap);
 int oracle_energy(oracle_ctx *o, int32_t case_index, double out[6]);
+int oracle_time(oracle_ctx *o, int32_t case_index, double *time, double *last_dt);
 int oracle_probe_global(oracle_ctx *o, int32_t case_index, double height_x, double out[SPHNJ_PROBE_COUNT]);
 int oracle_probe_solid(oracle_ctx *o, int32_t case_index, double X, double Y, double pos[2], int32_t *index);
 /* Derived constants of case 0 for the host-logic tests:

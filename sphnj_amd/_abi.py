@@ -53,7 +53,7 @@ class Params(C.Structure):
         ("shift_per_step", C.c_int32),
         ("smoothing_length", C.c_double), ("particle_diameter", C.c_double),
         ("impact_length_hsml_ratio", C.c_double), ("thread_num", C.c_int32),
-        ("background_pressure", C.c_double), ("kernel_2d_coef", C.c_double),
+        ("background_pressure", C.c_double), ("kernel_2d_coef", C.c_double), ("adaptive_cfl", C.c_double),
     ]
 
     def as_dict(self) -> dict:
@@ -123,6 +123,7 @@ CTX_PROTOS = {
     "solid_reflist": (C.c_int, [VP, C.c_int32, c_int32_p, c_int32_p, C.c_int64]),
     "energy": (C.c_int, [VP, C.c_int32, c_double_p]),
     "probe_global": (C.c_int, [VP, C.c_int32, C.c_double, c_double_p]),
+    "time": (C.c_int, [VP, C.c_int32, c_double_p, c_double_p]),
     "probe_solid": (C.c_int, [VP, C.c_int32, C.c_double, C.c_double, c_double_p, c_int32_p]),
     "destroy": (None, [VP]),
 }

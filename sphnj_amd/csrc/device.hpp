@@ -37,8 +37,12 @@ struct CaseDev {
   double p_min_d, p_bg_d;
   R p_min;
   R gx, gy;
-  double gx_d, gy_d, dt, dt_s;
-  R shift_k;     // particle_shift_coef * h * dt
+  double gx_d, gy_d, dt;
+  double shift_ch;  // particle_shift_coef * h (times the step's dt: the shift scale)
+  // adaptive step size (cfl = 0: every step takes dt): C h, the larger sound speed, and the
+  // solid cap S C h / c_s (0 without solids)
+  double cfl_h, c_max, dt_solid_cap;
+  int adaptive;
   R umax_cap;    // 0.1 * min(c_L, c_G)
   // domain / cell grid (Domain::initiate, src/domain.cpp:37-48)
   double cxmin, cymin, cdx, cdy, inv_cdx, inv_cdy;
@@ -117,6 +121,9 @@ struct Dev {
   // clearing run only then (a body moves by far less than a cell per step)
   int *s_moved;
   unsigned long long *umax_slot, *umax2;
+  // per case: max |u|^2 after the last fluid update (the input speeds before the first), the
+  // step size of the current step and the simulated time at its end
+  double *u2_last, *dtv, *tsim;
   R *umax;  // per case: min(sqrt(umax2), umax_cap), the speed the shift scales with
   // solid (original order, never permuted)
   double *sx, *sy, *su, *sv;     // cur

@@ -54,6 +54,8 @@ struct Derived {
   // time
   double dt, dt_s;
   int S, K;
+  // adaptive step size (0 = off): C, C h, the larger phase sound speed, S C h / c_s
+  double cfl, cfl_h, c_max, dt_solid_cap;
 };
 
 int derive(const sphnj_params &p, Derived *out, std::string *err);

@@ -780,6 +780,7 @@ int sphnj_case_write(const sphnj_params *P, const sphnj_particles *ps, const cha
   D("impact_length_hsml_ratio", P->impact_length_hsml_ratio);
   I("thread_num", P->thread_num);
   D("kernel_2d_coef", P->kernel_2d_coef);
+  D("adaptive_cfl", P->adaptive_cfl);
   std::fclose(f);
   return sphnj_particles_write(ps, (d + "/data").c_str());
 }

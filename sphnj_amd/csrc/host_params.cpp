@@ -208,6 +208,11 @@ int derive(const sphnj_params &p, Derived *o, std::string *err) {
   D.S = p.solid_sub_step > 0 ? p.solid_sub_step : 1;
   D.K = p.shift_per_step > 0 ? p.shift_per_step : 0;
   D.dt_s = p.dt / (double)D.S;
+  // adaptive step size (extension): dt_n = min(dt, C h / (c_max + U), S C h / c_s)
+  D.cfl = p.adaptive_cfl > 0.0 ? p.adaptive_cfl : 0.0;
+  D.cfl_h = D.cfl * D.h;
+  D.c_max = std::fmax(c[0], c[1]);
+  D.dt_solid_cap = D.c_s > 0.0 ? (double)D.S * D.cfl_h / D.c_s : 0.0;
   return SPHNJ_OK;
 }
 
@@ -266,6 +271,7 @@ int sphnj_params_default(sphnj_params *o) {
   o->thread_num = 32;
   o->background_pressure = 0.0;
   o->kernel_2d_coef = 0.0;
+  o->adaptive_cfl = 0.0;
   return SPHNJ_OK;
 }
 
@@ -332,6 +338,7 @@ int sphnj_params_from_ini(const char *path, sphnj_params *o) {
   r.i("thread_num", o->thread_num);
   r.d("background_pressure", o->background_pressure);
   r.d("kernel_2d_coef", o->kernel_2d_coef);
+  r.d("adaptive_cfl", o->adaptive_cfl);
   return SPHNJ_OK;
 }
 

@@ -1010,6 +1010,7 @@ __global__ void __launch_bounds__(64) k_umax_final(Dev<Pol> d) {
   const unsigned long long m = wave_max_u64(d.umax_slot[(size_t)(c * kSpread + (int)threadIdx.x) * kSlotStride]);
   if (threadIdx.x == 0) {
     d.umax2[c] = m;
+    d.u2_last[c] = __longlong_as_double((long long)m);
     d.umax[c] = fmin((typename Pol::R)sqrt(__longlong_as_double((long long)m)), d.cases[c].umax_cap);
   }
 }
@@ -1615,9 +1616,10 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
       ax += -0.1 * u0;
       ay += -0.1 * v0;
     }
-    double x = x0 + u0 * cp.dt, y = y0 + v0 * cp.dt;
-    const double u = u0 + (ax + cp.gx_d) * cp.dt, v = v0 + (ay + cp.gy_d) * cp.dt;
-    double rho = r0 + (double)S.drho * cp.dt;
+    const double dt = d.dtv[c];
+    double x = x0 + u0 * dt, y = y0 + v0 * dt;
+    const double u = u0 + (ax + cp.gx_d) * dt, v = v0 + (ay + cp.gy_d) * dt;
+    double rho = r0 + (double)S.drho * dt;
     rho = fmax(rho, cp.rho_min_d[phi]);
     // densityToPressure (src/fluid.cu:15-23)
     const double p = fmax((pow(rho * cp.rho0_inv_d[phi], cp.gam_d[phi]) - 1.0) * cp.rho2p_d[phi] + cp.p_bg_d,
@@ -1852,7 +1854,8 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
     gy += gys * cp.kdc_s;
   }
   const R umax = d.umax[c];
-  const R dxs = -cp.shift_k * gx, dys = -cp.shift_k * gy;
+  const R shift_k = (R)(cp.shift_ch * d.dtv[c]);  // particle_shift_coef h dt
+  const R dxs = -shift_k * gx, dys = -shift_k * gy;
   R sxv = dxs, syv = dys;
   if (cls == 0) {  // SPLASH: no shift
     sxv = 0;
@@ -2260,6 +2263,7 @@ __device__ __forceinline__ void solid_force_body(const Dev<Pol> &d, int sub, int
   }
   const int pk = d.s_cxy[i], cx = cxy_x(pk), cy = cxy_y(pk);
   const double xi = d.sx[i], yi = d.sy[i], ui = d.su[i], vi = d.sv[i];
+  const double dt_s = d.dtv[c] / (double)cp.S;  // solid sub-step (src/solid.cu:456)
   d.s_cp[2 * i] = cpx;
   d.s_cp[2 * i + 1] = cpy;
   double tax = (double)ax + (double)cpx, tay = (double)ay + (double)cpy;
@@ -2267,8 +2271,8 @@ __device__ __forceinline__ void solid_force_body(const Dev<Pol> &d, int sub, int
   if (d.s_bc[i] == 1) {  // FIXED_PARTICLE
     u = cp.fixed_u;
     v = cp.fixed_v;
-    x += u * cp.dt_s;
-    y += v * cp.dt_s;
+    x += u * dt_s;
+    y += v * dt_s;
     tax = 0.0;
     tay = 0.0;
   } else {
@@ -2276,10 +2280,10 @@ __device__ __forceinline__ void solid_force_body(const Dev<Pol> &d, int sub, int
       tax += -0.1 * ui;
       tay += -0.1 * vi;
     }
-    x += u * cp.dt_s;
-    y += v * cp.dt_s;
-    u += (tax + cp.gx_d) * cp.dt_s;
-    v += (tay + cp.gy_d) * cp.dt_s;
+    x += u * dt_s;
+    y += v * dt_s;
+    u += (tax + cp.gx_d) * dt_s;
+    v += (tay + cp.gy_d) * dt_s;
   }
   d.s_a[2 * i] = tax;
   d.s_a[2 * i + 1] = tay;
@@ -2568,6 +2572,23 @@ __global__ void __launch_bounds__(256) k_energy_partial(Dev<Pol> d, int flo, int
     __syncthreads();
   }
   if (t < 5) part[blockIdx.x * 5 + t] = sm[t][0];
+}
+
+// The step size of each case for the step that begins (the first thing of its cell stage),
+// and the simulated time at its end: dt, or with adaptive_cfl C > 0
+// dt_n = min(dt, C h / (c_max + U), S C h / c_s), U = sqrt(max |u|^2 after the last update).
+template <class Pol>
+__global__ void __launch_bounds__(64) k_step_dt(Dev<Pol> d) {
+  for (int c = threadIdx.x; c < d.ncase; c += 64) {
+    const auto &cp = d.cases[c];
+    double dt = cp.dt;
+    if (cp.adaptive) {
+      dt = fmin(dt, cp.cfl_h / (cp.c_max + sqrt(d.u2_last[c])));
+      if (cp.dt_solid_cap > 0.0) dt = fmin(dt, cp.dt_solid_cap);
+    }
+    d.dtv[c] = dt;
+    d.tsim[c] += dt;
+  }
 }
 
 // ---- probes of the reference's global outputs (src/output.cu:283-582) ----------------------

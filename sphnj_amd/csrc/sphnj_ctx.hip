@@ -105,6 +105,7 @@ struct EngineBase {
   virtual int reflist(int c, int *off, int *list, int64_t cap) = 0;
   virtual int energy(int c, double out[6]) = 0;
   virtual int probe_global(int c, double xh, double out[SPHNJ_PROBE_COUNT]) = 0;
+  virtual int sim_time(int c, double *t, double *dt) = 0;
   virtual int probe_solid(int c, double X, double Y, double pos[2], int *index) = 0;
   virtual int stage_time(int st, int *launches, double *ms) = 0;
   virtual int autotune() = 0;
@@ -230,8 +231,11 @@ struct Engine : EngineBase {
     c.gx_d = D.gx;
     c.gy_d = D.gy;
     c.dt = D.dt;
-    c.dt_s = D.dt_s;
-    c.shift_k = (R)(D.shift_coef * D.h * D.dt);
+    c.shift_ch = D.shift_coef * D.h;
+    c.adaptive = D.cfl > 0.0 ? 1 : 0;
+    c.cfl_h = D.cfl_h;
+    c.c_max = D.c_max;
+    c.dt_solid_cap = 0.0;  // set by init for cases with solids
     c.umax_cap = (R)D.umax_cap;
     c.cxmin = D.cxmin;
     c.cymin = D.cymin;
@@ -301,6 +305,7 @@ struct Engine : EngineBase {
     std::vector<int> colc;
     for (int c = 0; c < ncase; ++c) {
       fill_case(D[c], hc[c]);
+      if (!St[c].sx.empty()) hc[c].dt_solid_cap = D[c].dt_solid_cap;
       viscid = viscid || D[c].viscid;
       hc[c].cell_off = total_cells;
       total_cells += hc[c].ncell;
@@ -398,11 +403,24 @@ struct Engine : EngineBase {
         (rc = dalloc(&d.f_sacc, 2 * (size_t)nf)) ||
         (rc = dalloc(&d.f_srange, 4 * (size_t)nc)) || (rc = dalloc(&d.umax2, ncase)) || (rc = dalloc(&d.umax, ncase)) ||
         (rc = dalloc(&d.umax_slot, (size_t)ncase * kSpread * kSlotStride)) || (rc = dalloc(&d.status, 1)) ||
+        (rc = dalloc(&d.u2_last, ncase)) || (rc = dalloc(&d.dtv, ncase)) || (rc = dalloc(&d.tsim, ncase)) ||
         (rc = dalloc(&d.s_moved, 1)) ||
         (rc = dalloc(&d.f_lb, (size_t)lb_total)) || (rc = dalloc(&d.f_swp, (size_t)swp_total)) ||
         (rc = dalloc(&col_case, G)))
       return rc;
     if ((rc = h2d(col_case, colc.data(), G))) return rc;
+    {  // step size state: the input speeds, dt, time 0
+      std::vector<double> u2(ncase, 0.0), dt0(ncase), t0(ncase, 0.0);
+      for (int c = 0; c < ncase; ++c) {
+        for (size_t i = 0; i < St[c].fu.size(); ++i)
+          u2[c] = std::max(u2[c], St[c].fu[i] * St[c].fu[i] + St[c].fv[i] * St[c].fv[i]);
+        dt0[c] = D[c].dt;
+      }
+      if ((rc = h2d(d.u2_last, u2.data(), ncase)) || (rc = h2d(d.dtv, dt0.data(), ncase)) ||
+          (rc = h2d(d.tsim, t0.data(), ncase)))
+        return rc;
+      HIPC(hipStreamSynchronize(stream));
+    }
     for (int b = 0; b < 3; ++b) {
       const int B = 64 << b;
       if ((rc = dalloc(&tile_cnt[b], G)) || (rc = dalloc(&tile_off[b], G + 1)) ||
@@ -839,7 +857,9 @@ struct Engine : EngineBase {
   int stage(int st) override {
     const int nf = d.nf, ns = d.ns, nw = d.nw;
     switch (st) {
-      case SPHNJ_STAGE_CELLS:
+      case SPHNJ_STAGE_CELLS:  // a step begins: its size first
+        k_step_dt<Pol><<<1, 64, 0, stream>>>(d);
+        LAUNCH_CHECK();
         return stage_cells();
       case SPHNJ_STAGE_COUPLE:
         if (nf > 0 && ns > 0) {
@@ -1297,6 +1317,16 @@ struct Engine : EngineBase {
       HIPC(hipStreamSynchronize(stream));
       for (int e = 0; e < tot && e < cap; ++e) list[e] = l[e] - lo;
     }
+    return SPHNJ_OK;
+  }
+
+  int sim_time(int c, double *t, double *dt) override {
+    double v[2];
+    int rc;
+    if ((rc = d2h(v, d.tsim + c, 1)) || (rc = d2h(v + 1, d.dtv + c, 1))) return rc;
+    HIPC(hipStreamSynchronize(stream));
+    if (t) *t = v[0];
+    if (dt) *dt = v[1];
     return SPHNJ_OK;
   }
 
@@ -1919,6 +1949,12 @@ int sphnj_snapshot_load(sphnj_ctx *c, int32_t ci, const char *path) {
   S.sp = std::move(sp);
   S.sn = std::move(sn);
   return SPHNJ_OK;
+}
+
+int sphnj_time(sphnj_ctx *c, int32_t ci, double *time, double *last_dt) {
+  if (!c || ci < 0 || ci >= c->ncase) return SPHNJ_E_ARG;
+  if (!c->inited) return c->fail(SPHNJ_E_STATE, "before init");
+  return c->from_engine(c->eng->sim_time(ci, time, last_dt));
 }
 
 int sphnj_probe_global(sphnj_ctx *c, int32_t ci, double height_x, double out[SPHNJ_PROBE_COUNT]) {

@@ -265,6 +265,12 @@ class ContextBase:
         self._call("energy", case, _dptr(out))
         return out
 
+    def time(self, case: int = 0) -> tuple[float, float]:
+        """(simulated time, size of the last step) of a case."""
+        t, dt = C.c_double(), C.c_double()
+        self._call("time", case, C.byref(t), C.byref(dt))
+        return t.value, dt.value
+
     def probe_global(self, case: int = 0, height_x: float = 0.0) -> dict:
         """The reference's global outputs (src/output.cu:283-560) of the current state:
         force_x/y, front, height (at x = height_x), vm_elastic, vm_auxetic, vm_max, mean_pressure."""

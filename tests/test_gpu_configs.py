@@ -262,3 +262,26 @@ def test_reference_scenarios_against_oracle(gpu, oracle_mod, name, precision):
     if name in ("ref_circle_entry", "ref_wedge_entry"):
         assert np.abs(ref["solid"]["pressure"]).max() > 1e4, "the body must be in the water"
     check(errs, TOL[precision], f"{name} ({nf} fluid, {ns} solid, {nw} wall) {precision}")
+
+
+def adaptive_dam():
+    c = Case.generate("dam_break", tank_w=1.0, tank_h=0.8, col_w=0.4, col_h=0.6, dx=0.01, air=1, K=3)
+    c.params.adaptive_cfl = 0.3
+    c.params.dt *= 4.0   # the cap; the CFL term takes over as the flow speeds up
+    return c
+
+
+@pytest.mark.parametrize("precision", ["p64", "m32"])
+def test_adaptive_step_size_dam_break(gpu, oracle_mod, precision):
+    """A two-phase dam break whose step size follows the flow speed (adaptive_cfl): the time
+    and the fields after 200 steps against the oracle."""
+    ora = oracle_mod.Oracle(adaptive_dam())
+    ora.step(200)
+    ref = ora.download()
+    t_ref, dt_ref = ora.time()
+    sim = Simulation(adaptive_dam(), precision)
+    sim.step(200)
+    t, dt = sim.time()
+    assert dt_ref < adaptive_dam().params.dt, "the CFL term must be in play"
+    assert abs(t - t_ref) <= (1e-12 if precision == "p64" else 1e-6) * t_ref
+    check(errors(sim.download(), ref, components=False), TOL[precision], f"adaptive dt {precision}")

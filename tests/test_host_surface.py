@@ -249,7 +249,7 @@ def test_library_exports_every_declared_symbol():
     lib = library()
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
-    assert lib.sphnj_abi_version() == 1
+    assert lib.sphnj_abi_version() == 2
 
 
 # ---- the reference's six scenarios (cases/<name>/particle_generate.cpp + config/input.ini) ----

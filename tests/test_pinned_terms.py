@@ -509,3 +509,39 @@ def test_probes_closed_forms(backend):
     cp = s2.download()["solid"]["couple_dudt"][0]
     close([g2["force_x"], g2["force_y"]], cp * 636.0 * 0.02 ** 2, 1e-12, "total force")
     assert g2["force_x"] < 0          # the fluid pushes the solid away from it
+
+
+# ---- adaptive step size (extension adaptive_cfl; the reference steps by a fixed dt) ---------
+
+@pytest.mark.parametrize("with_solid", [False, True])
+def test_adaptive_step_size_recurrence(backend, with_solid):
+    """An isolated particle falling from rest: step n takes dt_n = min(dt, C h / (c_max + |v_n-1|))
+    (and at most S C h / c_s when the case has solids), v_n = v_n-1 - g dt_n; the time is the
+    sum of the dt_n.  The solid, far away, only caps the step."""
+    C_, g, c0, dtmax, S = 0.2, 9.81, 20.0, 5e-4, 3
+    p = params(gravity=g, dt=dtmax, liquid_sound_speed=c0, gas_sound_speed=15.0, adaptive_cfl=C_,
+               solid_sub_step=S, youngs_modulus=1.0e6, solid_density=1000.0, poisson_ratio=0.3)
+    s_in = solid([0.1], [0.1]) if with_solid else None
+    c = case(p, fluid([0.5], [0.9]), s=s_in)
+    s = run(backend, c)
+    h = p.smoothing_length
+    cs = math.sqrt(1.0e6 / (3 * (1 - 0.6)) / 1000.0)
+    v, t, dts = 0.0, 0.0, []
+    for n in range(30):
+        dt = min(dtmax, C_ * h / (c0 + abs(v)))
+        if with_solid:
+            dt = min(dt, S * C_ * h / cs)
+        v -= g * dt
+        t += dt
+        dts.append(dt)
+    s.step(30)
+    tt, dt_last = s.time()
+    assert len(set(dts)) > 5                      # the step size changes as the particle speeds up
+    close(tt, t, 1e-13, "time")
+    close(dt_last, dts[-1], 1e-13, "last dt")
+    close(s.download()["fluid"]["v"][0], v, 1e-12, "velocity")
+    # a fixed step without the option
+    p.adaptive_cfl = 0.0
+    s2 = run(backend, case(p, fluid([0.5], [0.9])))
+    s2.step(4)
+    close(s2.time()[0], 4 * dtmax, 1e-15, "fixed-step time")

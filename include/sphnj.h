@@ -336,6 +336,16 @@ int sphnj_probe_global(sphnj_ctx *ctx, int32_t case_index, double height_x, doub
  * among several); *index receives that index.  SPHNJ_E_ARG (index -1) when there is none. */
 int sphnj_probe_solid(sphnj_ctx *ctx, int32_t case_index, double X, double Y, double pos[2], int32_t *index);
 
+/* sphnj_energy without allocating.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture setup, tight serving loops,
+ * harnesses that forbid allocation).  Same reductions and results as sphnj_energy, with its
+ * constants and partial sums in a fixed per-GPU static array of 1,288 doubles (10 KB, held by
+ * each GPU the library runs on until the process ends) instead of memory allocated per call;
+ * the array's size does not depend on the case, and a reduction that needed more would be
+ * refused with SPHNJ_E_UNSUPPORTED.  Every context on one GPU shares that array, so calls that
+ * overlap on one GPU race: serialise them.  Synchronises the context's stream. */
+int sphnj_energy_noalloc(sphnj_ctx *ctx, int32_t case_index, double out[6]);
+
 /* ---- timing (HIP events on the context stream) ------------------------------------------ */
 /* Elapsed time of the last sphnj_step call. */
 int sphnj_last_step_ms(sphnj_ctx *ctx, double *ms);

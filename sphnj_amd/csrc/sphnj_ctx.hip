@@ -12,6 +12,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -59,6 +60,43 @@ struct Staged {
   std::vector<double> sM, sp, sn;
 };
 
+// The FP64 reductions of sphnj_energy: 7 per-case constants and 256 blocks of 5 partials.
+constexpr int kEnergyConsts = 7, kEnergyBlocks = 256;
+
+// ---- the allocation-free entry point's scratch ------------------------------------------
+// sphnj_energy_noalloc keeps its constants and partials here instead of allocating them: one
+// array per GPU (every GPU the library runs on holds its own copy until the process ends),
+// shared by every context on that GPU, so calls that overlap on one GPU race.  Host code
+// never passes `scratch` itself to a kernel -- on the host the name is only a stand-in -- but
+// the device address hipGetSymbolAddress returns for the current device, looked up once per
+// device by scratch_address() when a context is initialised.
+constexpr int kScratchDoubles = 8 + kEnergyBlocks * 5;  // 10,304 bytes
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+std::mutex g_scratch_mu;
+std::map<int, double *> g_scratch_ptr;  // device -> address of its `scratch`
+
+int scratch_address(double **out, std::string *err) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess) {
+    *err = "hipGetDevice failed";
+    return SPHNJ_E_HIP;
+  }
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it == g_scratch_ptr.end()) {
+    double *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      *err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+      return SPHNJ_E_HIP;
+    }
+    it = g_scratch_ptr.emplace(dev, p).first;
+  }
+  *out = it->second;
+  return SPHNJ_OK;
+}
+
 struct EngineBase {
   std::string err;
   hipStream_t stream = nullptr;
@@ -104,6 +142,7 @@ struct EngineBase {
   virtual int neighbors(int c, int *counts, int *list, int64_t cap, int64_t *total) = 0;
   virtual int reflist(int c, int *off, int *list, int64_t cap) = 0;
   virtual int energy(int c, double out[6]) = 0;
+  virtual int energy_in(int c, double out[6], double *consts, double *part) = 0;
   virtual int probe_global(int c, double xh, double out[SPHNJ_PROBE_COUNT]) = 0;
   virtual int sim_time(int c, double *t, double *dt) = 0;
   virtual int probe_solid(int c, double X, double Y, double pos[2], int *index) = 0;
@@ -1389,29 +1428,33 @@ struct Engine : EngineBase {
   }
 
   int energy(int c, double out[6]) override {
-    int rc;
-    const Derived &D = Dh[c];
-    const double consts[7] = {D.d * D.d, D.rho0[0], D.rho0[1], D.rho_s, D.rho_aux, D.lambda, D.mu_s};
-    double *dc, *part;
-    const int G = 256;
     void *q1 = nullptr, *q2 = nullptr;
-    HIPC(hipMalloc(&q1, sizeof(consts)));
-    dc = (double *)q1;
-    if (hipMalloc(&q2, (size_t)G * 5 * sizeof(double)) != hipSuccess) {
+    HIPC(hipMalloc(&q1, kEnergyConsts * sizeof(double)));
+    if (hipMalloc(&q2, (size_t)kEnergyBlocks * 5 * sizeof(double)) != hipSuccess) {
       (void)hipFree(q1);
       err = "hipMalloc";
       return SPHNJ_E_HIP;
     }
-    part = (double *)q2;
+    const int rc = energy_in(c, out, (double *)q1, (double *)q2);
+    (void)hipFree(q1);
+    (void)hipFree(q2);
+    return rc;
+  }
+
+  // The reductions of energy() into caller-provided device memory: kEnergyConsts doubles at
+  // dc, kEnergyBlocks x 5 partials at part (every one rewritten by each call).
+  int energy_in(int c, double out[6], double *dc, double *part) override {
+    int rc;
+    const Derived &D = Dh[c];
+    const double consts[kEnergyConsts] = {D.d * D.d, D.rho0[0], D.rho0[1], D.rho_s, D.rho_aux, D.lambda, D.mu_s};
+    const int G = kEnergyBlocks;
     std::vector<double> hp((size_t)G * 5);
-    rc = h2d(dc, consts, 7);
+    rc = h2d(dc, consts, kEnergyConsts);
     if (!rc) {
       k_energy_partial<Pol><<<G, 256, 0, stream>>>(d, f_off[c], f_off[c + 1], s_off[c], s_off[c + 1], dc, part);
       rc = d2h(hp.data(), part, (size_t)G * 5);
     }
     const hipError_t e = hipStreamSynchronize(stream);
-    (void)hipFree(q1);
-    (void)hipFree(q2);
     if (rc) return rc;
     if (e != hipSuccess) {
       err = hipGetErrorString(e);
@@ -1452,6 +1495,7 @@ struct sphnj_ctx {
   bool inited = false;
   bool use_graph = true;
   bool persistent_solid = false;
+  double *scratch_ptr = nullptr;  // this device's `scratch` (sphnj_energy_noalloc)
   std::string err;
   int fail(int code, const std::string &m) {
     err = m;
@@ -1568,6 +1612,13 @@ int sphnj_init(sphnj_ctx *c) {
   c->eng->stream = c->stream;
   c->eng->use_graph = c->use_graph;
   c->eng->persistent_solid = c->persistent_solid;
+  {
+    std::string e;
+    if (scratch_address(&c->scratch_ptr, &e) != SPHNJ_OK) {
+      c->eng.reset();
+      return c->fail(SPHNJ_E_HIP, e);
+    }
+  }
   const int rc = c->eng->init(c->D, c->staged);
   if (rc) {
     c->err = c->eng->err;
@@ -1949,6 +2000,18 @@ int sphnj_snapshot_load(sphnj_ctx *c, int32_t ci, const char *path) {
   S.sp = std::move(sp);
   S.sn = std::move(sn);
   return SPHNJ_OK;
+}
+
+int sphnj_energy_noalloc(sphnj_ctx *c, int32_t ci, double out[6]) {
+  if (!c || !out || ci < 0 || ci >= c->ncase) return SPHNJ_E_ARG;
+  if (!c->inited) return c->fail(SPHNJ_E_STATE, "before init");
+  const int need = kEnergyConsts + kEnergyBlocks * 5;
+  if (need > kScratchDoubles)
+    return c->fail(SPHNJ_E_UNSUPPORTED, "energy reductions need " + std::to_string(need) +
+                                             " doubles of scratch, more than the " +
+                                             std::to_string(kScratchDoubles) + " the library holds");
+  if (!c->scratch_ptr) return c->fail(SPHNJ_E_STATE, "no scratch address for this device");
+  return c->from_engine(c->eng->energy_in(ci, out, c->scratch_ptr, c->scratch_ptr + 8));
 }
 
 int sphnj_time(sphnj_ctx *c, int32_t ci, double *time, double *last_dt) {

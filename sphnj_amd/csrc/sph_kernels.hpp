@@ -1663,93 +1663,55 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
 // positions are re-based on the tile frame (the tile's cell column, and the cell row
 // cy_ref), in units scaled so that the weight is 2^-r'^2; the frame costs at most a few
 // float ulps of a tile height in the gradient of a small correction.
-template <class Pol, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePlan *plans, const int *ntiles,
-                                                       int pass) {
+// A lane's own state for a shift pass (loaded ahead of the staging so that the loads overlap
+// it): packed sub-cell, class, position (for M32 the masters, which stay at the pair
+// kernel's result until the last pass), path length and the shift accumulated so far.
+template <class Pol>
+struct ShiftOwn {
   using R = typename Pol::R;
   using R2 = typename V2<R>::T;
-  constexpr int CAP = ShiftCap<BLOCK>::value;
-  // walls of the tile's 3x3 cell columns over its cell rows follow the fluid candidates
-  // (a tile's fluid candidates rarely fill CAP, so the walls also use what they leave)
-  constexpr int WCAP = BLOCK / 2, WROWS = TileRows<BLOCK>::value / 2 + 2;
-  // fluid candidates, padding, walls, padding (the padding is far away: read by masked tails)
-  __shared__ R2 s_pos[CAP + WCAP + 8];
-  __shared__ int s_lb[kTileCols][TileRows<BLOCK>::value + 1];
-  __shared__ int s_wlb[3][WROWS + 1];  // LDS index of the first wall of cell row wcr0 + k or later
-  __shared__ float s_dmax[BLOCK / 64];
-  const int nt = *ntiles, tile = xcd_tile(blockIdx.x, nt);
-  if (tile >= nt) return;
-  const TilePlan &P = plans[tile];
-  const int t = threadIdx.x, i = P.start + t, c = P.c;
-  const bool valid = t < P.count;
-  const auto &A = d.fa;
-  const auto &B = d.fb;
-  const auto &cp = d.cases[c];
-  const bool staged = P.total <= CAP && P.nr <= TileRows<BLOCK>::value;
-  const bool active = pass < cp.K, last = pass == cp.K - 1;
-  // the lane's own state, loaded ahead of the staging so that the loads overlap it
   int pk = 0, cls = 3;
   double x0 = 0.0, y0 = 0.0;
   R2 sacc{R(0), R(0)};
   float cum0 = 0.f;
-  if (valid) {
-    pk = d.f_cxy[i];
-    x0 = A.x[i];
-    y0 = A.y[i];
-    cum0 = A.cum[i];
-    if (active) {
-      cls = d.f_cls[i];
-      if (pass > 0) sacc = R2{d.f_sacc[2 * i], d.f_sacc[2 * i + 1]};
+};
+template <class Pol>
+__device__ __forceinline__ ShiftOwn<Pol> shift_own(const Dev<Pol> &d, const TilePlan &P, int pass, int K) {
+  ShiftOwn<Pol> o;
+  const int t = threadIdx.x, i = P.start + t;
+  if (t < P.count) {
+    o.pk = d.f_cxy[i];
+    o.x0 = d.fa.x[i];
+    o.y0 = d.fa.y[i];
+    o.cum0 = d.fa.cum[i];
+    if (pass < K) {
+      o.cls = d.f_cls[i];
+      if (pass > 0) o.sacc = {d.f_sacc[2 * i], d.f_sacc[2 * i + 1]};
     }
   }
-  int dmax = 0;  // bit pattern of the largest path length (non-negative floats order as ints)
-  if (staged) {
-    struct Ld {
-      R2 pos;
-      int pk;
-      int cum;
-    };
-    const R fdy = cp.cdy_r, gs = cp.gscale;
-    // this thread's span lies in one cell column: its x offset in the tile frame, scaled
-    const int sa = min((int)(((unsigned)t * (unsigned)P.gmul) >> 20), P.nsc - 1);
-    const R xo = (R)(((P.sc0 + sa) >> 1) - P.cx) * cp.cdx_r * gs;
-    const int gcy = P.cy_ref;
-    tile_stage<BLOCK, 4>(
-        P, d.f_lb, s_lb,
-        [&](int g) {
-          return Ld{R2{ld32(A.vx, g), ld32(A.vy, g)}, Pol::MIXED ? ld32(d.f_cxy, g) : 0,
-                    ld32(reinterpret_cast<const int *>(A.cum), g)};
-        },
-        [&](int l, const Ld &v, int) {
-          if constexpr (Pol::MIXED)  // tile frame, scaled units
-            s_pos[l] = R2{fmaf(v.pos.x, gs, xo), (v.pos.y + (float)(((v.pk & 0xffff) >> 1) - gcy) * fdy) * gs};
-          else
-            s_pos[l] = v.pos;
-          dmax = max(dmax, v.cum);
-        });
-    if (t < 4) s_pos[P.total + t] = R2{R(1e18), R(1e18)};
-  }
-  // walls: staged after the fluid candidates' padding
-  const WallSpans W = wall_spans(d, P, cp);
-  const int wbase = P.total + 4;
-  const bool wstaged = staged && W.n > 0 && P.total + W.n <= CAP + WCAP && W.nwr <= WROWS;
-  if (wstaged) {
-    const R fdx = cp.cdx_r, fdy = cp.cdy_r, gs = cp.gscale;
-    wall_stage<BLOCK, WROWS>(d, W, wbase, s_wlb, [&](int l, int g) {
-      const R wx = ld32(d.w_vx, g), wy = ld32(d.w_vy, g);
-      if constexpr (Pol::MIXED) {  // tile frame, scaled units
-        const int wk = ld32(d.w_cxy, g);
-        s_pos[l] = R2{(wx + (float)(cxy_x(wk) - P.cx) * fdx) * gs, (wy + (float)(cxy_y(wk) - P.cy_ref) * fdy) * gs};
-      } else {
-        s_pos[l] = R2{wx, wy};
-      }
-    });
-    if (t < 4) s_pos[wbase + W.n + t] = R2{R(1e18), R(1e18)};
-  }
-  for (int o = 32; o > 0; o >>= 1) dmax = max(dmax, __shfl_xor(dmax, o, 64));
-  if ((t & 63) == 0) s_dmax[t >> 6] = __int_as_float(dmax);
-  __syncthreads();
-  if (!valid) return;
+  return o;
+}
+
+// The tile's lanes after the staging barrier: gradient sums over the staged candidates (or
+// global memory for a tile that is not staged), walls and solids, then the shift update.
+// Reads only the LDS it is given; stores only the lanes' own results.
+template <class Pol, int BLOCK, class LB, int WROWS>
+__device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P, int pass, const ShiftOwn<Pol> &own,
+                                           const typename V2<typename Pol::R>::T *s_pos,
+                                           const LB (*s_lb)[TileRows<BLOCK>::value + 1], const int (*s_wlb)[WROWS + 1],
+                                           float D, bool staged, bool wstaged, const WallSpans &W) {
+  using R = typename Pol::R;
+  using R2 = typename V2<R>::T;
+  const int t = threadIdx.x, i = P.start + t, c = P.c;
+  if (t >= P.count) return;
+  const auto &A = d.fa;
+  const auto &B = d.fb;
+  const auto &cp = d.cases[c];
+  const bool active = pass < cp.K, last = pass == cp.K - 1;
+  const int pk = own.pk, cls = own.cls;
+  const double x0 = own.x0, y0 = own.y0;
+  const R2 sacc = own.sacc;
+  const float cum0 = own.cum0;
   if (!active) {  // this case has fewer passes than the batch: carry the moving fields over
     B.vx[i] = A.vx[i];
     B.vy[i] = A.vy[i];
@@ -1772,7 +1734,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
     gx += dx * w;
     gy += dy * w;
   };
-  const float Rp = prune_radius(cp, staged, staged_bound<BLOCK>(s_dmax));
+  const float Rp = prune_radius(cp, staged, D);
   const int *lbc = d.f_lb + cp.lb_off;
   const int ny21 = 2 * cp.ny + 1;
   // staged fluid: M32 in the scaled tile frame (w = 2^-r'^2 inside the cut-off, sums
@@ -1788,17 +1750,16 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
   }
   const R rc2s = cp.rc2s;
   // one staged candidate (fluid or wall); w = 0 outside the cut-off
-  auto staged_add = [&](int l, bool in) {
+  auto staged_add = [&](int l, bool) {
     const R2 pj = s_pos[l];
     const R dx = pj.x - xs, dy = pj.y - ys, r2 = dx * dx + dy * dy;
     if constexpr (Pol::MIXED) {
       const R e = __builtin_amdgcn_exp2f(-r2);
-      const R w = (in & (r2 < rc2s)) ? e : R(0);
+      const R w = r2 < rc2s ? e : R(0);
       gxs += dx * w;
       gys += dy * w;
     } else {
-      const R wk = K.w(r2);
-      const R w = in ? wk : R(0);
+      const R w = K.w(r2);
       gx += dx * w;
       gy += dy * w;
     }
@@ -1905,6 +1866,82 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
     d.f_sacc[2 * i] = sn.x;
     d.f_sacc[2 * i + 1] = sn.y;
   }
+}
+
+
+// correctionCalculate (src/fluid.cu:316-367) + correctionUpdate / shiftDisplacementCalculate
+// (src/fluid.cu:369-427), one tile per block: stage the tile, then shift_tile.
+template <class Pol, int BLOCK>
+__global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePlan *plans, const int *ntiles,
+                                                       int pass) {
+  using R = typename Pol::R;
+  using R2 = typename V2<R>::T;
+  constexpr int CAP = ShiftCap<BLOCK>::value;
+  // walls of the tile's 3x3 cell columns over its cell rows follow the fluid candidates
+  // (a tile's fluid candidates rarely fill CAP, so the walls also use what they leave)
+  constexpr int WCAP = BLOCK / 2, WROWS = TileRows<BLOCK>::value / 2 + 2;
+  // fluid candidates, padding, walls, padding (the padding is far away: read by masked tails)
+  __shared__ R2 s_pos[CAP + WCAP + 8];
+  __shared__ int s_lb[kTileCols][TileRows<BLOCK>::value + 1];
+  __shared__ int s_wlb[3][WROWS + 1];  // LDS index of the first wall of cell row wcr0 + k or later
+  __shared__ float s_dmax[BLOCK / 64];
+  const int nt = *ntiles, tile = xcd_tile(blockIdx.x, nt);
+  if (tile >= nt) return;
+  const TilePlan &P = plans[tile];
+  const int t = threadIdx.x, c = P.c;
+  const auto &A = d.fa;
+  const auto &cp = d.cases[c];
+  const bool staged = P.total <= CAP && P.nr <= TileRows<BLOCK>::value;
+  const ShiftOwn<Pol> own = shift_own(d, P, pass, cp.K);
+  int dmax = 0;  // bit pattern of the largest path length (non-negative floats order as ints)
+  if (staged) {
+    struct Ld {
+      R2 pos;
+      int pk;
+      int cum;
+    };
+    const R fdy = cp.cdy_r, gs = cp.gscale;
+    // this thread's span lies in one cell column: its x offset in the tile frame, scaled
+    const int sa = min((int)(((unsigned)t * (unsigned)P.gmul) >> 20), P.nsc - 1);
+    const R xo = (R)(((P.sc0 + sa) >> 1) - P.cx) * cp.cdx_r * gs;
+    const int gcy = P.cy_ref;
+    tile_stage<BLOCK, 4>(
+        P, d.f_lb, s_lb,
+        [&](int g) {
+          return Ld{R2{ld32(A.vx, g), ld32(A.vy, g)}, Pol::MIXED ? ld32(d.f_cxy, g) : 0,
+                    ld32(reinterpret_cast<const int *>(A.cum), g)};
+        },
+        [&](int l, const Ld &v, int) {
+          if constexpr (Pol::MIXED)  // tile frame, scaled units
+            s_pos[l] = R2{fmaf(v.pos.x, gs, xo), (v.pos.y + (float)(((v.pk & 0xffff) >> 1) - gcy) * fdy) * gs};
+          else
+            s_pos[l] = v.pos;
+          dmax = max(dmax, v.cum);
+        });
+    if (t < 4) s_pos[P.total + t] = R2{R(1e18), R(1e18)};
+  }
+  // walls: staged after the fluid candidates' padding
+  const WallSpans W = wall_spans(d, P, cp);
+  const int wbase = P.total + 4;
+  const bool wstaged = staged && W.n > 0 && P.total + W.n <= CAP + WCAP && W.nwr <= WROWS;
+  if (wstaged) {
+    const R fdx = cp.cdx_r, fdy = cp.cdy_r, gs = cp.gscale;
+    wall_stage<BLOCK, WROWS>(d, W, wbase, s_wlb, [&](int l, int g) {
+      const R wx = ld32(d.w_vx, g), wy = ld32(d.w_vy, g);
+      if constexpr (Pol::MIXED) {  // tile frame, scaled units
+        const int wk = ld32(d.w_cxy, g);
+        s_pos[l] = R2{(wx + (float)(cxy_x(wk) - P.cx) * fdx) * gs, (wy + (float)(cxy_y(wk) - P.cy_ref) * fdy) * gs};
+      } else {
+        s_pos[l] = R2{wx, wy};
+      }
+    });
+    if (t < 4) s_pos[wbase + W.n + t] = R2{R(1e18), R(1e18)};
+  }
+  for (int o = 32; o > 0; o >>= 1) dmax = max(dmax, __shfl_xor(dmax, o, 64));
+  if ((t & 63) == 0) s_dmax[t >> 6] = __int_as_float(dmax);
+  __syncthreads();
+  shift_tile<Pol, BLOCK, int, WROWS>(d, P, pass, own, s_pos, s_lb, s_wlb, staged_bound<BLOCK>(s_dmax), staged, wstaged,
+                                     W);
 }
 
 // ======================================================================================

@@ -37,20 +37,20 @@ HBM_PEAK_GBS = 8000.0
 BYTES = {
     # stage: {precision: bytes per fluid particle per launch}
     "fluid": {"m32": 74 + 90, "p64": 136 + 122},   # pair + update (fused)
-    # gradient pass + shift (fused), per pass, averaged over K = 10 passes: a pass reads the
-    # views, sub-cell, path length, class, position and accumulated shift and writes views,
-    # path length and accumulated shift (61 B); the last pass also reads L, u, v and writes
-    # u, v and the masters (117 B)
-    "shift": {"m32": 66, "p64": 84},
+    # gradient pass + shift (fused), per pass, averaged over K = 10 passes.  M32: a pass reads
+    # the views, sub-cell, path length, class and accumulated shift and writes views, path
+    # length and accumulated shift (45 B); the last pass also reads the double masters, L, u, v
+    # and writes u, v and the masters (117 B).  P64: the positions are the views (84 B).
+    "shift": {"m32": 52, "p64": 84},
     "cells": {"m32": 16 + 142, "p64": 20 + 150},   # hash + sort/permute/cell ranges
 }
 WALL_BYTES = {"m32": 38, "p64": 66}  # wall Shepard, per wall particle
 
 
 def shift_step_bytes_m32(K):
-    """M32 shift bytes per fluid particle per step at K passes: 61 B a pass, plus the last
-    pass's extra 56 B (66 B a pass on average at K = 10)."""
-    return 61 * K + 56
+    """M32 shift bytes per fluid particle per step at K passes: 45 B a pass, plus the last
+    pass's extra 72 B (52 B a pass on average at K = 10)."""
+    return 45 * K + 72
 
 
 def configs():

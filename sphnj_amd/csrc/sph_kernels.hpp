@@ -1664,14 +1664,17 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
 // cy_ref), in units scaled so that the weight is 2^-r'^2; the frame costs at most a few
 // float ulps of a tile height in the gradient of a small correction.
 // A lane's own state for a shift pass (loaded ahead of the staging so that the loads overlap
-// it): packed sub-cell, class, position (for M32 the masters, which stay at the pair
-// kernel's result until the last pass), path length and the shift accumulated so far.
+// it): packed sub-cell, class, position, path length and the shift accumulated so far.  The
+// position is the view the neighbour loops read (for P64 the position itself); M32 reads its
+// double masters, which stay at the pair kernel's result until then, on the case's last pass
+// only.
 template <class Pol>
 struct ShiftOwn {
   using R = typename Pol::R;
   using R2 = typename V2<R>::T;
   int pk = 0, cls = 3;
   double x0 = 0.0, y0 = 0.0;
+  R vx = R(0), vy = R(0);
   R2 sacc{R(0), R(0)};
   float cum0 = 0.f;
 };
@@ -1681,8 +1684,14 @@ __device__ __forceinline__ ShiftOwn<Pol> shift_own(const Dev<Pol> &d, const Tile
   const int t = threadIdx.x, i = P.start + t;
   if (t < P.count) {
     o.pk = d.f_cxy[i];
-    o.x0 = d.fa.x[i];
-    o.y0 = d.fa.y[i];
+    if (!Pol::MIXED || pass == K - 1) {
+      o.x0 = d.fa.x[i];
+      o.y0 = d.fa.y[i];
+    }
+    if constexpr (Pol::MIXED) {
+      o.vx = d.fa.vx[i];
+      o.vy = d.fa.vy[i];
+    }
     o.cum0 = d.fa.cum[i];
     if (pass < K) {
       o.cls = d.f_cls[i];
@@ -1719,14 +1728,13 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
     return;
   }
   const int sy = cxy_y(pk), cx = P.cx, cy = sy >> 1;
-  // current position: the M32 masters stay at the pair kernel's result until the last pass
-  const double xc = Pol::MIXED ? x0 + (double)sacc.x : x0, yc = Pol::MIXED ? y0 + (double)sacc.y : y0;
-  const R xi = view_x<Pol>(xc, cx, cp), yi = view_y<Pol>(yc, cy, cp);  // = the stored views
+  // current position: the stored view (M32: cell-relative float, advanced by each pass)
+  const R xi = Pol::MIXED ? own.vx : (R)x0, yi = Pol::MIXED ? own.vy : (R)y0;
   LanePos lp;
   if constexpr (Pol::MIXED)
     lp = lane_pos_view(xi, yi, cx, cy, P.sx, sy, cp);
   else
-    lp = lane_pos(xc, yc, P.sx, sy, cp);
+    lp = lane_pos(x0, y0, P.sx, sy, cp);
   const LoopConsts<R> K(cp);
   R gx = 0, gy = 0;
   auto add = [&](R dx, R dy) {
@@ -1839,12 +1847,16 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
   sxv *= umax;
   syv *= umax;
   const R2 sn{sacc.x + sxv, sacc.y + syv};
-  double x, y;
+  double x = 0.0, y = 0.0;
   if constexpr (Pol::MIXED) {
-    x = x0 + (double)sn.x;
-    y = y0 + (double)sn.y;
-    B.vx[i] = view_x<Pol>(x, cx, cp);
-    B.vy[i] = view_y<Pol>(y, cy, cp);
+    // the views move by this pass's shift in float; the masters take the accumulated shift
+    // in double on the last pass
+    B.vx[i] = xi + sxv;
+    B.vy[i] = yi + syv;
+    if (last) {
+      x = x0 + (double)sn.x;
+      y = y0 + (double)sn.y;
+    }
   } else {
     x = x0 + sxv;
     y = y0 + syv;

@@ -238,6 +238,13 @@ int sphnj_set_graph(sphnj_ctx *ctx, int enable);
  * launches (default).  Both give the same results bit for bit.  On MI355X a grid barrier must
  * write back and invalidate the L2 of every XCD, which costs more than a launch boundary. */
 int sphnj_set_persistent_solid(sphnj_ctx *ctx, int enable);
+/* Shift passes of a step (fluidCorrection's loop, src/fluid.cu:442-459; no counterpart of its
+ * own in the reference): mode 1 (default) has the step's first pass record, per particle, the
+ * runs of candidates within rc (1 + 0.02), and the later passes walk only those runs while
+ * every path since the first pass stays within 0.02 rc (else the full windows); mode 0 searches
+ * the full pruned windows in every pass; mode 2 records the runs and never walks them.  All
+ * three give the same results bit for bit.  SPHNJ_E_ARG for another mode. */
+int sphnj_set_shift_runs(sphnj_ctx *ctx, int mode);
 /* Autotuner: times the fluid pair kernel (family 0) and the shift kernel (family 1) at block
  * sizes 64, 128 and 256 -- a block owns that many consecutive sorted particles, so the block
  * size also sets the cells each LDS tile spans -- on the current state with HIP events, and

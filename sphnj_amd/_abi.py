@@ -148,6 +148,7 @@ LIB_PROTOS = {
     "sphnj_last_error": (C.c_char_p, [VP]),
     "sphnj_set_graph": (C.c_int, [VP, C.c_int]),
     "sphnj_set_persistent_solid": (C.c_int, [VP, C.c_int]),
+    "sphnj_set_shift_runs": (C.c_int, [VP, C.c_int]),
     "sphnj_energy_noalloc": (C.c_int, [VP, C.c_int32, c_double_p]),
     "sphnj_snapshot_write": (C.c_int, [VP, C.c_int32, C.c_int64, C.c_char_p]),
     "sphnj_snapshot_info": (C.c_int, [C.c_char_p, P(Params), P(C.c_int64), c_int32_p]),

@@ -49,6 +49,12 @@ struct CaseDev {
   R cdx_r, cdy_r;
   // half-cell ("sub-cell") grid of the fluid order: sizes and the pruning cut-off rc (1 + 1e-5)
   double rcut_prune;
+  // shift windows kept over a step's passes (skin s rc, s = kShiftSkin): the recording radius
+  // as the first pass compares it (M32 scaled r'^2, P64 r^2) of rc (1 + s), the widening of
+  // that pass's pruning radius, and the largest path the lane and its candidates may have
+  // taken since that pass for the windows to hold every neighbour within rc
+  R rl2_rec;
+  float skin_grow, skin_ok;
   float ssx, ssy, inv_ssx, inv_ssy;
   double dxmin, dymin, dxmax, dymax;
   int nx, ny, cell_off, ncell;
@@ -108,6 +114,13 @@ struct Dev {
   // shift displacement accumulated over the passes of a step (2 per slot; velocities and
   // M32 masters take it on the case's last pass)
   R *f_sacc;
+  // per slot, the first shift pass of a step records the lane's candidate runs for the later
+  // passes: up to 5 runs of the staged LDS order (11-bit start, 7-bit length) packed in x, y
+  // and z, bit 31 of x set when they are complete, and the lane's path length since the hash
+  // at that pass (float bits) in w; per tile of the shift, the largest such path among its
+  // staged candidates
+  uint4 *f_wc;
+  float *tile_d0;
   int2 *f_srange;  // per global sub-cell (4 per cell): [start, end) slot range, empty (0, 0)
   // per sub-column, 2 ny + 1 entries: first slot of sub-row k or later; the sub-column's end
   int *f_lb;

@@ -1151,6 +1151,22 @@ __device__ __forceinline__ bool span_window(const TilePlan &P, const CP &cp, int
   return ra <= rb;
 }
 
+// Shift windows over the passes of a step.  A step's shift passes move particles by a tiny
+// fraction of rc, and every pass of a step uses the same tiles, so the same staged LDS
+// order.  The first pass (kShiftRecord, when a step has more than one) widens its pruning
+// radius by s rc and records, per lane and candidate sub-column, the run of LDS indices from
+// the first to the last candidate within rc (1 + s).  Its path lengths restart: the lanes
+// store the path since that pass (and keep the one since the hash with the runs).  A later
+// pass (kShiftCached) walks only those runs while the lane's path since the first pass plus
+// the largest one among the tile's staged candidates stays within s rc: every candidate
+// within rc then was within rc (1 + s) at the first pass, so it lies in a run, and the runs
+// keep the sub-column and slot order, so the non-zero terms and their order are those of the
+// full windows and the sums are the same bit for bit.  Any other lane searches the full
+// windows, with the tile's path bound rebuilt from the first pass's bound.  The last pass
+// writes the path since the hash again.  kShiftPlain is the single pass of a one-pass step.
+constexpr int kShiftPlain = 0, kShiftRecord = 1, kShiftCached = 2;
+constexpr double kShiftSkin = 0.02;
+
 // Pruning radius: rc (1 + 1e-5) plus the tile's displacement bound D; a tile that is not
 // staged searches its lanes' whole 3x3 cells.
 template <class CP>
@@ -1200,6 +1216,7 @@ __device__ __forceinline__ void for_range_masked(int lo, int hi, F &&f) {
 template <class F>
 __device__ __forceinline__ void for_range_4_2_1(int lo, int hi, F &&f) {
   int s = lo;
+#pragma unroll 1
   for (const int last = hi - 3; s < last; s += 4) {
     f(s, true);
     f(s + 1, true);
@@ -1668,6 +1685,8 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
 // position is the view the neighbour loops read (for P64 the position itself); M32 reads its
 // double masters, which stay at the pair kernel's result until then, on the case's last pass
 // only.
+// The cached passes also read the runs recorded by the step's first pass, and their path
+// length is the path since that pass.
 template <class Pol>
 struct ShiftOwn {
   using R = typename Pol::R;
@@ -1677,8 +1696,9 @@ struct ShiftOwn {
   R vx = R(0), vy = R(0);
   R2 sacc{R(0), R(0)};
   float cum0 = 0.f;
+  uint4 wc{0u, 0u, 0u, 0u};
 };
-template <class Pol>
+template <class Pol, int MODE>
 __device__ __forceinline__ ShiftOwn<Pol> shift_own(const Dev<Pol> &d, const TilePlan &P, int pass, int K) {
   ShiftOwn<Pol> o;
   const int t = threadIdx.x, i = P.start + t;
@@ -1696,6 +1716,7 @@ __device__ __forceinline__ ShiftOwn<Pol> shift_own(const Dev<Pol> &d, const Tile
     if (pass < K) {
       o.cls = d.f_cls[i];
       if (pass > 0) o.sacc = {d.f_sacc[2 * i], d.f_sacc[2 * i + 1]};
+      if constexpr (MODE == kShiftCached) o.wc = d.f_wc[i];
     }
   }
   return o;
@@ -1703,12 +1724,15 @@ __device__ __forceinline__ ShiftOwn<Pol> shift_own(const Dev<Pol> &d, const Tile
 
 // The tile's lanes after the staging barrier: gradient sums over the staged candidates (or
 // global memory for a tile that is not staged), walls and solids, then the shift update.
-// Reads only the LDS it is given; stores only the lanes' own results.
-template <class Pol, int BLOCK, class LB, int WROWS>
+// Reads only the LDS it is given; stores only the lanes' own results.  D bounds the staged
+// candidates' paths since the hash; a cached pass also passes Dsp, their paths since the
+// step's first pass, and runs whether a staged tile may walk the recorded runs.
+template <class Pol, int BLOCK, int MODE, class LB, int WROWS>
 __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P, int pass, const ShiftOwn<Pol> &own,
                                            const typename V2<typename Pol::R>::T *s_pos,
                                            const LB (*s_lb)[TileRows<BLOCK>::value + 1], const int (*s_wlb)[WROWS + 1],
-                                           float D, bool staged, bool wstaged, const WallSpans &W) {
+                                           float D, float Dsp, bool runs, bool staged, bool wstaged,
+                                           const WallSpans &W) {
   using R = typename Pol::R;
   using R2 = typename V2<R>::T;
   const int t = threadIdx.x, i = P.start + t, c = P.c;
@@ -1742,7 +1766,7 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
     gx += dx * w;
     gy += dy * w;
   };
-  const float Rp = prune_radius(cp, staged, D);
+  const float Rp = prune_radius(cp, staged, D) + ((MODE == kShiftRecord && staged) ? cp.skin_grow : 0.f);
   const int *lbc = d.f_lb + cp.lb_off;
   const int ny21 = 2 * cp.ny + 1;
   // staged fluid: M32 in the scaled tile frame (w = 2^-r'^2 inside the cut-off, sums
@@ -1757,8 +1781,8 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
     ys = o.y;
   }
   const R rc2s = cp.rc2s;
-  // one staged candidate (fluid or wall); w = 0 outside the cut-off
-  auto staged_add = [&](int l, bool) {
+  // one staged candidate (fluid or wall); w = 0 outside the cut-off; returns r^2 (scaled)
+  auto staged_term = [&](int l) {
     const R2 pj = s_pos[l];
     const R dx = pj.x - xs, dy = pj.y - ys, r2 = dx * dx + dy * dy;
     if constexpr (Pol::MIXED) {
@@ -1771,20 +1795,69 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
       gx += dx * w;
       gy += dy * w;
     }
+    return r2;
   };
-  for_windows<false>(P, cp, sy, cy, lp, Rp, [&](int sc, int ra, int rb) {
-    if (staged) {
-      const int sa = sc - P.sc0;
-      for_range_4_2_1(s_lb[sa][ra - P.r0], s_lb[sa][rb + 1 - P.r0], staged_add);
-    } else {  // global views, relative to their own cell: one run per cell row
-      const R bx = xi - cell_shift_x<Pol>((sc >> 1) - cx, cp);
-      for (int cr = ra >> 1; cr <= (rb >> 1); ++cr) {
-        const R by = yi - cell_shift_y<Pol>(cr - cy, cp);
-        const int lo = lbc[sc * ny21 + max(ra, 2 * cr)], hi = lbc[sc * ny21 + min(rb, 2 * cr + 1) + 1];
-        for_range<4>(lo, hi, [&](int s) { add(A.vx[s] - bx, A.vy[s] - by); });
-      }
+  auto staged_add = [&](int l, bool) { (void)staged_term(l); };
+  // first pass of several: the runs to record (see kShiftRecord)
+  unsigned wc0 = 0u, wc1 = 0u, wc2 = 0u;
+  int nrun = 0;
+  bool runs_ok = staged;
+  auto put_run = [&](int f, int n) {
+    if (nrun >= 5 || n > 127) {
+      runs_ok = false;
+      return;
     }
-  });
+    const unsigned st = (unsigned)f, ln = (unsigned)n;
+    if (nrun == 0) wc0 |= st | ln << 22;
+    if (nrun == 1) {
+      wc0 |= st << 11;
+      wc1 |= ln << 22;
+    }
+    if (nrun == 2) wc1 |= st, wc2 |= ln << 11;
+    if (nrun == 3) wc1 |= st << 11, wc2 |= ln << 18;
+    if (nrun == 4) wc2 |= st | ln << 25;
+    ++nrun;
+  };
+  bool cached = false;
+  if constexpr (MODE == kShiftCached) cached = runs && (own.wc.x >> 31) && cum0 + Dsp <= cp.skin_ok;
+  if (cached) {
+    const unsigned x = own.wc.x, y = own.wc.y, z = own.wc.z;
+    const int s0 = x & 0x7ff, s1 = (x >> 11) & 0x7ff, s2 = y & 0x7ff, s3 = (y >> 11) & 0x7ff, s4 = z & 0x7ff;
+    for_range_4_2_1(s0, s0 + (int)((x >> 22) & 0x7f), staged_add);
+    for_range_4_2_1(s1, s1 + (int)((y >> 22) & 0x7f), staged_add);
+    for_range_4_2_1(s2, s2 + (int)((z >> 11) & 0x7f), staged_add);
+    for_range_4_2_1(s3, s3 + (int)((z >> 18) & 0x7f), staged_add);
+    for_range_4_2_1(s4, s4 + (int)(z >> 25), staged_add);
+  } else {
+    const R rl2 = cp.rl2_rec;
+    for_windows<false>(P, cp, sy, cy, lp, Rp, [&](int sc, int ra, int rb) {
+      if (staged) {
+        const int sa = sc - P.sc0;
+        const int lo = s_lb[sa][ra - P.r0], hi = s_lb[sa][rb + 1 - P.r0];
+        if constexpr (MODE == kShiftRecord) {
+          int f = hi, la = lo - 1;  // first and last candidate within rc (1 + s)
+          for_range_4_2_1(lo, hi, [&](int l, bool) {
+            if (staged_term(l) < rl2) {
+              f = min(f, l);
+              la = l;
+            }
+          });
+          if (la >= f) put_run(f, la - f + 1);
+        } else {
+          for_range_4_2_1(lo, hi, staged_add);
+        }
+      } else {  // global views, relative to their own cell: one run per cell row
+        const R bx = xi - cell_shift_x<Pol>((sc >> 1) - cx, cp);
+        for (int cr = ra >> 1; cr <= (rb >> 1); ++cr) {
+          const R by = yi - cell_shift_y<Pol>(cr - cy, cp);
+          const int lo = lbc[sc * ny21 + max(ra, 2 * cr)], hi = lbc[sc * ny21 + min(rb, 2 * cr + 1) + 1];
+          for_range<4>(lo, hi, [&](int s) { add(A.vx[s] - bx, A.vy[s] - by); });
+        }
+      }
+    });
+  }
+  if constexpr (MODE == kShiftRecord)
+    d.f_wc[i] = uint4{wc0 | (runs_ok ? 0x80000000u : 0u), wc1, wc2, __float_as_uint(cum0)};
   // walls of the 3x3 cells (cell column by cell column, rows in order), then solids
   if (wstaged) {
     const int lr0 = max(cy - 1, 0) - W.wcr0, lr1 = min(cy + 1, cp.ny - 1) - W.wcr0;
@@ -1863,9 +1936,15 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
     B.x[i] = x;
     B.y[i] = y;
   }
-  // path length: this pass moved the particle by (sxv, syv)
-  const float sx2 = (float)sxv, sy2 = (float)syv;
-  B.cum[i] = cum0 + __builtin_amdgcn_sqrtf(sx2 * sx2 + sy2 * sy2);
+  // path length: this pass moved the particle by (sxv, syv); the passes after a recording
+  // one count from it and the last adds the path since the hash it recorded
+  const float sx2 = (float)sxv, sy2 = (float)syv, step = __builtin_amdgcn_sqrtf(sx2 * sx2 + sy2 * sy2);
+  if constexpr (MODE == kShiftPlain)
+    B.cum[i] = cum0 + step;
+  else if constexpr (MODE == kShiftRecord)
+    B.cum[i] = last ? cum0 + step : step;
+  else
+    B.cum[i] = last ? (__uint_as_float(own.wc.w) + cum0) + step : cum0 + step;
   if (last) {
     if constexpr (Pol::MIXED) {
       A.x[i] = x;
@@ -1882,10 +1961,12 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
 
 
 // correctionCalculate (src/fluid.cu:316-367) + correctionUpdate / shiftDisplacementCalculate
-// (src/fluid.cu:369-427), one tile per block: stage the tile, then shift_tile.
-template <class Pol, int BLOCK>
+// (src/fluid.cu:369-427), one tile per block: stage the tile, then shift_tile.  MODE is
+// kShiftPlain, kShiftRecord or kShiftCached (the step's one pass, its first of several, a
+// later one).
+template <class Pol, int BLOCK, int MODE>
 __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePlan *plans, const int *ntiles,
-                                                       int pass) {
+                                                       int pass, int use_runs) {
   using R = typename Pol::R;
   using R2 = typename V2<R>::T;
   constexpr int CAP = ShiftCap<BLOCK>::value;
@@ -1904,7 +1985,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
   const auto &A = d.fa;
   const auto &cp = d.cases[c];
   const bool staged = P.total <= CAP && P.nr <= TileRows<BLOCK>::value;
-  const ShiftOwn<Pol> own = shift_own(d, P, pass, cp.K);
+  const ShiftOwn<Pol> own = shift_own<Pol, MODE>(d, P, pass, cp.K);
   int dmax = 0;  // bit pattern of the largest path length (non-negative floats order as ints)
   if (staged) {
     struct Ld {
@@ -1952,8 +2033,15 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePla
   for (int o = 32; o > 0; o >>= 1) dmax = max(dmax, __shfl_xor(dmax, o, 64));
   if ((t & 63) == 0) s_dmax[t >> 6] = __int_as_float(dmax);
   __syncthreads();
-  shift_tile<Pol, BLOCK, int, WROWS>(d, P, pass, own, s_pos, s_lb, s_wlb, staged_bound<BLOCK>(s_dmax), staged, wstaged,
-                                     W);
+  float D = staged_bound<BLOCK>(s_dmax), Dsp = 0.f;
+  if constexpr (MODE == kShiftRecord) {
+    if (t == 0) d.tile_d0[tile] = D;
+  } else if constexpr (MODE == kShiftCached) {  // the staged paths count from the first pass
+    Dsp = D;
+    D = d.tile_d0[tile] + Dsp;
+  }
+  shift_tile<Pol, BLOCK, MODE, int, WROWS>(d, P, pass, own, s_pos, s_lb, s_wlb, D, Dsp, staged && use_runs, staged,
+                                           wstaged, W);
 }
 
 // ======================================================================================

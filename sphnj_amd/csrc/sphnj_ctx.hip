@@ -102,6 +102,7 @@ struct EngineBase {
   hipStream_t stream = nullptr;
   bool use_graph = true;
   bool persistent_solid = false;  // S solid sub-steps as one launch (k_solid_substeps)
+  int shift_runs = 1;             // shift passes after the first walk its recorded runs
   int blk_pair = 256, blk_shift = 256, blk_stream = 256;
   // per-stage timing: HIP events recorded on the context stream around each stage while on
   bool timing = false;
@@ -285,6 +286,12 @@ struct Engine : EngineBase {
     c.cdx_r = (R)D.cdx;
     c.cdy_r = (R)D.cdy;
     c.rcut_prune = std::sqrt(D.rc2) * (1.0 + 1e-5);
+    {
+      const double g = (1.0 + kShiftSkin) * (1.0 + kShiftSkin), rc = std::sqrt(D.rc2);
+      c.rl2_rec = Pol::MIXED ? (R)(D.rc2 * D.hinv2 * 1.4426950408889634 * g) : (R)(D.rc2 * g);
+      c.skin_grow = (float)(kShiftSkin * rc);
+      c.skin_ok = (float)(kShiftSkin * rc * (1.0 - 1e-3));
+    }
     c.ssx = (float)(0.5 * D.cdx);
     c.ssy = (float)(0.5 * D.cdy);
     c.inv_ssx = (float)(2.0 * D.inv_cdx);
@@ -439,7 +446,8 @@ struct Engine : EngineBase {
         (rc = dalloc(&cnt_tsum, (size_t)nblk(4 * nc, kScanTile) + 1)) ||
         (rc = dalloc(&cnt_toff, (size_t)nblk(4 * nc, kScanTile) + 1)) ||
         (rc = dalloc(&d.f_col, nf)) || (rc = dalloc(&d.f_near, nf)) || (rc = dalloc(&d.f_L, 4 * (size_t)nf)) ||
-        (rc = dalloc(&d.f_sacc, 2 * (size_t)nf)) ||
+        (rc = dalloc(&d.f_sacc, 2 * (size_t)nf)) || (rc = dalloc(&d.f_wc, nf)) ||
+        (rc = dalloc(&d.tile_d0, (size_t)max_tiles(64))) ||
         (rc = dalloc(&d.f_srange, 4 * (size_t)nc)) || (rc = dalloc(&d.umax2, ncase)) || (rc = dalloc(&d.umax, ncase)) ||
         (rc = dalloc(&d.umax_slot, (size_t)ncase * kSpread * kSlotStride)) || (rc = dalloc(&d.status, 1)) ||
         (rc = dalloc(&d.u2_last, ncase)) || (rc = dalloc(&d.dtv, ncase)) || (rc = dalloc(&d.tsim, ncase)) ||
@@ -806,10 +814,19 @@ struct Engine : EngineBase {
     LAUNCH_CHECK();
     return SPHNJ_OK;
   }
+  // the step's first pass records the candidate runs of the later ones (kShiftRecord);
+  // shift_runs 0 searches the full windows in every pass, 2 records but never walks the runs
   template <int B>
   void shift_at(int pass) {
     const int b = bidx(B);
-    k_fluid_shift<Pol, B><<<grid_tiles(B), B, 0, stream>>>(d, plans[b], tile_off[b] + G, pass);
+    const int *nt = tile_off[b] + G;
+    const dim3 g(grid_tiles(B));
+    if (shift_runs == 0 || K == 1)
+      k_fluid_shift<Pol, B, kShiftPlain><<<g, B, 0, stream>>>(d, plans[b], nt, pass, 0);
+    else if (pass == 0)
+      k_fluid_shift<Pol, B, kShiftRecord><<<g, B, 0, stream>>>(d, plans[b], nt, pass, 0);
+    else
+      k_fluid_shift<Pol, B, kShiftCached><<<g, B, 0, stream>>>(d, plans[b], nt, pass, shift_runs == 1);
   }
   int launch_shift(int blk, int pass) {
     int rc;
@@ -1495,6 +1512,7 @@ struct sphnj_ctx {
   bool inited = false;
   bool use_graph = true;
   bool persistent_solid = false;
+  int shift_runs = 1;
   double *scratch_ptr = nullptr;  // this device's `scratch` (sphnj_energy_noalloc)
   std::string err;
   int fail(int code, const std::string &m) {
@@ -1612,6 +1630,7 @@ int sphnj_init(sphnj_ctx *c) {
   c->eng->stream = c->stream;
   c->eng->use_graph = c->use_graph;
   c->eng->persistent_solid = c->persistent_solid;
+  c->eng->shift_runs = c->shift_runs;
   {
     std::string e;
     if (scratch_address(&c->scratch_ptr, &e) != SPHNJ_OK) {
@@ -1726,6 +1745,16 @@ int sphnj_set_persistent_solid(sphnj_ctx *c, int enable) {
   if (c->eng) {
     if (c->eng->persistent_solid != c->persistent_solid) c->eng->drop_graphs();
     c->eng->persistent_solid = c->persistent_solid;
+  }
+  return SPHNJ_OK;
+}
+
+int sphnj_set_shift_runs(sphnj_ctx *c, int mode) {
+  if (!c || mode < 0 || mode > 2) return SPHNJ_E_ARG;
+  c->shift_runs = mode;
+  if (c->eng) {
+    if (c->eng->shift_runs != mode) c->eng->drop_graphs();
+    c->eng->shift_runs = mode;
   }
   return SPHNJ_OK;
 }

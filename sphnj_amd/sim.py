@@ -293,7 +293,7 @@ class Simulation(ContextBase):
     prefix = "sphnj_"
 
     def __init__(self, cases: Case | list[Case], precision: str = "p64", graph: bool = True,
-                 persistent_solid: bool = False):
+                 persistent_solid: bool = False, shift_runs: int = 1):
         lib = library()
         cases = [cases] if isinstance(cases, Case) else list(cases)
         prec = {"p64": A.PREC_P64, "m32": A.PREC_M32}[precision]
@@ -304,6 +304,7 @@ class Simulation(ContextBase):
         self.precision = precision
         _check(lib.sphnj_set_graph(h, int(graph)), "set_graph")
         _check(lib.sphnj_set_persistent_solid(h, int(persistent_solid)), "set_persistent_solid")
+        _check(lib.sphnj_set_shift_runs(h, int(shift_runs)), "set_shift_runs")
         self._upload_all()
 
     def _err(self) -> str:
@@ -361,6 +362,11 @@ class Simulation(ContextBase):
 
     def set_stage_timing(self, on: bool) -> None:
         self._call("set_stage_timing", int(on))
+
+    def set_shift_runs(self, mode: int) -> None:
+        """Shift passes after a step's first: 1 walk the runs it recorded (default), 0 search
+        the full windows, 2 record and never walk them (sphnj_set_shift_runs)."""
+        self._call("set_shift_runs", int(mode))
 
     def stage_time(self, name: str) -> tuple[int, float]:
         """(kernel launches, total ms) of a stage since timing was enabled."""

@@ -180,6 +180,48 @@ def test_dense_tiles_read_global_candidates(gpu, oracle_mod):
             assert np.array_equal(out["p64", 64][k], out["p64", blk][k]), (blk, k)
 
 
+@pytest.mark.parametrize("precision", ["p64", "m32"])
+def test_shift_runs_equal_full_windows(gpu, precision):
+    """The shift passes after a step's first walk the candidate runs it recorded (mode 1),
+    search the full pruned windows (mode 0), or record the runs and always fall back to the
+    windows rebuilt from the first pass's path bound (mode 2): the same non-zero terms in the
+    same order, so every field is bit-identical, through graph replays, on a batch whose cases
+    take 10 and 3 passes per step (the one with fewer carries its state over)."""
+    cases = [Case.generate("dam_break", dx=0.02, air=1, K=10),
+             Case.generate("dam_break", dx=0.025, K=3)]
+    sims = {m: Simulation(cases, precision, shift_runs=m) for m in (0, 1, 2)}
+    for n in (1, 9, 30):
+        out = {}
+        for m, s in sims.items():
+            s.step(n)
+            out[m] = [s.download(c)["fluid"] for c in range(2)]
+        for m in (1, 2):
+            for c in range(2):
+                for k in ("x", "y", "u", "v", "rho", "p", "boundary_type"):
+                    assert np.array_equal(out[m][c][k], out[0][c][k]), (n, m, c, k)
+    for s in sims.values():
+        assert s.status() == 0
+    with pytest.raises(Exception):
+        sims[1].set_shift_runs(3)
+
+
+def test_shift_runs_on_global_tiles(gpu):
+    """Tiles too dense for LDS read global memory and record no runs; beside them staged
+    tiles walk theirs: P64 with and without runs is bit-identical over several steps."""
+    p = params(shift_per_step=4)
+    xd, yd = lattice(40, 40, 0.005, 0.3, 0.3)
+    xn, yn = lattice(20, 20, 0.02, 0.6, 0.2)
+    x, y = np.r_[xd, xn], np.r_[yd, yn]
+    c = case(p, f=fluid(x, y, u=0.5 * (y - 0.4), v=0.3 * (x - 0.5), p=100.0 * x, rho=1000.0 + 0.5 * y))
+    a = Simulation(c, "p64", shift_runs=1)
+    b = Simulation(c, "p64", shift_runs=0)
+    a.step(3)
+    b.step(3)
+    da, db = a.download()["fluid"], b.download()["fluid"]
+    for k in ("x", "y", "u", "v", "rho", "p", "boundary_type"):
+        assert np.array_equal(da[k], db[k]), k
+
+
 def test_solid_sort_follows_cell_changes(gpu, oracle_mod):
     """A FIXED solid block driven at 4 m/s through a fluid block: its particles cross cell
     boundaries during the run, so the solids' sort, which is skipped on the steps where no solid

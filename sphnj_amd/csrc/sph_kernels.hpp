@@ -1681,12 +1681,11 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
 // cy_ref), in units scaled so that the weight is 2^-r'^2; the frame costs at most a few
 // float ulps of a tile height in the gradient of a small correction.
 // A lane's own state for a shift pass (loaded ahead of the staging so that the loads overlap
-// it): packed sub-cell, class, position, path length and the shift accumulated so far.  The
-// position is the view the neighbour loops read (for P64 the position itself); M32 reads its
-// double masters, which stay at the pair kernel's result until then, on the case's last pass
-// only.
-// The cached passes also read the runs recorded by the step's first pass, and their path
-// length is the path since that pass.
+// it): packed sub-cell, class, position, path length and the shift accumulated so far.  The position is the view the neighbour loops read (for P64 the position itself); M32
+// reads its double masters, which stay at the pair kernel's result until then, on the case's
+// last pass only.  A pass after a recording one also reads the lane's runs (the path since the
+// hash on the last pass only), and the sub-cell only where it needs it before the loops:
+// walls or solids in the tile, or the last pass (pk = -1 marks it unread).
 template <class Pol>
 struct ShiftOwn {
   using R = typename Pol::R;
@@ -1695,16 +1694,17 @@ struct ShiftOwn {
   double x0 = 0.0, y0 = 0.0;
   R vx = R(0), vy = R(0);
   R2 sacc{R(0), R(0)};
-  float cum0 = 0.f;
-  uint4 wc{0u, 0u, 0u, 0u};
+  float cum0 = 0.f, cum_hash = 0.f;
+  uint3 wc{0u, 0u, 0u};
 };
 template <class Pol, int MODE>
 __device__ __forceinline__ ShiftOwn<Pol> shift_own(const Dev<Pol> &d, const TilePlan &P, int pass, int K) {
   ShiftOwn<Pol> o;
   const int t = threadIdx.x, i = P.start + t;
   if (t < P.count) {
-    o.pk = d.f_cxy[i];
-    if (!Pol::MIXED || pass == K - 1) {
+    const bool lastp = pass == K - 1;
+    o.pk = (MODE != kShiftCached || lastp || P.sw) ? d.f_cxy[i] : -1;
+    if (!Pol::MIXED || lastp) {
       o.x0 = d.fa.x[i];
       o.y0 = d.fa.y[i];
     }
@@ -1716,7 +1716,10 @@ __device__ __forceinline__ ShiftOwn<Pol> shift_own(const Dev<Pol> &d, const Tile
     if (pass < K) {
       o.cls = d.f_cls[i];
       if (pass > 0) o.sacc = {d.f_sacc[2 * i], d.f_sacc[2 * i + 1]};
-      if constexpr (MODE == kShiftCached) o.wc = d.f_wc[i];
+      if constexpr (MODE == kShiftCached) {
+        o.wc = *reinterpret_cast<const uint3 *>(d.f_wc + i);
+        if (lastp) o.cum_hash = __uint_as_float(d.f_wc[i].w);
+      }
     }
   }
   return o;
@@ -1741,7 +1744,7 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
   const auto &B = d.fb;
   const auto &cp = d.cases[c];
   const bool active = pass < cp.K, last = pass == cp.K - 1;
-  const int pk = own.pk, cls = own.cls;
+  const int cls = own.cls;
   const double x0 = own.x0, y0 = own.y0;
   const R2 sacc = own.sacc;
   const float cum0 = own.cum0;
@@ -1751,6 +1754,9 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
     B.cum[i] = cum0;
     return;
   }
+  bool cached = false;
+  if constexpr (MODE == kShiftCached) cached = runs && (own.wc.x >> 31) && cum0 + Dsp <= cp.skin_ok;
+  const int pk = own.pk >= 0 ? own.pk : (cached ? 0 : d.f_cxy[i]);
   const int sy = cxy_y(pk), cx = P.cx, cy = sy >> 1;
   // current position: the stored view (M32: cell-relative float, advanced by each pass)
   const R xi = Pol::MIXED ? own.vx : (R)x0, yi = Pol::MIXED ? own.vy : (R)y0;
@@ -1818,8 +1824,6 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
     if (nrun == 4) wc2 |= st | ln << 25;
     ++nrun;
   };
-  bool cached = false;
-  if constexpr (MODE == kShiftCached) cached = runs && (own.wc.x >> 31) && cum0 + Dsp <= cp.skin_ok;
   if (cached) {
     const unsigned x = own.wc.x, y = own.wc.y, z = own.wc.z;
     const int s0 = x & 0x7ff, s1 = (x >> 11) & 0x7ff, s2 = y & 0x7ff, s3 = (y >> 11) & 0x7ff, s4 = z & 0x7ff;
@@ -1835,14 +1839,16 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
         const int sa = sc - P.sc0;
         const int lo = s_lb[sa][ra - P.r0], hi = s_lb[sa][rb + 1 - P.r0];
         if constexpr (MODE == kShiftRecord) {
-          int f = hi, la = lo - 1;  // first and last candidate within rc (1 + s)
+          // first and last candidate within rc (1 + s): f follows l until one is seen
+          int f = lo, la = lo;
+          bool seen = false;
           for_range_4_2_1(lo, hi, [&](int l, bool) {
-            if (staged_term(l) < rl2) {
-              f = min(f, l);
-              la = l;
-            }
+            const bool in = staged_term(l) < rl2;
+            f = seen ? f : l;
+            seen = seen || in;
+            la = in ? l : la;
           });
-          if (la >= f) put_run(f, la - f + 1);
+          if (seen) put_run(f, la - f + 1);
         } else {
           for_range_4_2_1(lo, hi, staged_add);
         }
@@ -1944,7 +1950,7 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
   else if constexpr (MODE == kShiftRecord)
     B.cum[i] = last ? cum0 + step : step;
   else
-    B.cum[i] = last ? (__uint_as_float(own.wc.w) + cum0) + step : cum0 + step;
+    B.cum[i] = last ? (own.cum_hash + cum0) + step : cum0 + step;
   if (last) {
     if constexpr (Pol::MIXED) {
       A.x[i] = x;

@@ -37,20 +37,24 @@ HBM_PEAK_GBS = 8000.0
 BYTES = {
     # stage: {precision: bytes per fluid particle per launch}
     "fluid": {"m32": 74 + 90, "p64": 136 + 122},   # pair + update (fused)
-    # gradient pass + shift (fused), per pass, averaged over K = 10 passes.  M32: a pass reads
-    # the views, sub-cell, path length, class and accumulated shift and writes views, path
-    # length and accumulated shift (45 B); the last pass also reads the double masters, L, u, v
-    # and writes u, v and the masters (117 B).  P64: the positions are the views (84 B).
-    "shift": {"m32": 52, "p64": 84},
+    # gradient pass + shift (fused), per pass, averaged over K = 10 passes.  M32: every pass
+    # reads the views, path length and class and writes the views and path length (25 B); the
+    # first also reads the sub-cell and writes the accumulated shift and the candidate runs
+    # (53 B); a middle pass reads the runs (12 B) and reads and writes the accumulated shift
+    # (53 B); the last also reads the sub-cell, the path since the hash kept with the runs, the
+    # double masters, L, u, v and writes the masters, u, v (133 B).  P64: the positions are
+    # the views, the accumulated shift and L are double (77, 85 and 141 B).
+    "shift": {"m32": 61, "p64": 90},
     "cells": {"m32": 16 + 142, "p64": 20 + 150},   # hash + sort/permute/cell ranges
 }
 WALL_BYTES = {"m32": 38, "p64": 66}  # wall Shepard, per wall particle
 
 
 def shift_step_bytes_m32(K):
-    """M32 shift bytes per fluid particle per step at K passes: 45 B a pass, plus the last
-    pass's extra 72 B (52 B a pass on average at K = 10)."""
-    return 45 * K + 72
+    """M32 shift bytes per fluid particle per step at K passes: 53 B for the first and each
+    middle pass and 133 B for the last (61 B a pass on average at K = 10); a single pass reads
+    the sub-cell, views, path, class, masters, L, u, v and writes views, path, masters, u, v."""
+    return 109 if K == 1 else 53 * K + 80
 
 
 def configs():
@@ -318,12 +322,14 @@ def main() -> int:
     # roofline of the dominant kernel (largest share of the step)
     dom = max(("fluid", "shift", "cells"), key=lambda s: prof[s]["ms_per_step"]) if args.profile_steps else "shift"
     per_launch_ms = prof[dom]["ms_per_launch"]
-    bytes_launch = BYTES[dom]["m32"] * nf
+    K = case.params.shift_per_step
+    bytes_pp = shift_step_bytes_m32(K) / K if dom == "shift" else BYTES[dom]["m32"]
+    bytes_launch = bytes_pp * nf
     achieved = bytes_launch / (per_launch_ms * 1e-3) / 1e9 if per_launch_ms else 0.0
     blocks = {"fluid": (tune or {}).get("pair", {}).get("block", 256),
               "shift": (tune or {}).get("shift", {}).get("block", 256), "cells": 256}
     kernel_name = {"fluid": f"k_fluid_pair_update<M32, {blocks['fluid']}>",
-                   "shift": f"k_fluid_shift<M32, {blocks['shift']}>",
+                   "shift": f"k_fluid_shift<M32, {blocks['shift']}> (average of the step's {K} passes)",
                    "cells": "cells stage (hash + counting sort + permute + ranges)"}[dom]
     # The neighbour kernels issue vector-ALU instructions at the CU's rate (about one wave64
     # instruction per cycle), so VALU issue, not HBM, is what binds them; achieved / peak / frac
@@ -337,11 +343,10 @@ def main() -> int:
                 "counters_from": "committed rocprofv3 passes: profiles/pmc_traffic_two_phase_3M.json "
                                  "(FETCH_SIZE / WRITE_SIZE, copy-calibrated), profiles/pmc_valu_two_phase_3M.json "
                                  "(SQ_INSTS_VALU / GRBM_GUI_ACTIVE)",
-                "bytes_per_particle": BYTES[dom]["m32"], "ms_per_launch": round(per_launch_ms, 5)}
+                "bytes_per_particle": round(bytes_pp, 2), "ms_per_launch": round(per_launch_ms, 5)}
 
     # the whole step against HBM: algorithmic bytes of every fluid and wall stage (solids, which
     # are at most 0.3% of the particles of these workloads, are not counted) over the step time
-    K = case.params.shift_per_step
     step_bytes = nf * (BYTES["cells"]["m32"] + BYTES["fluid"]["m32"] + shift_step_bytes_m32(K)) \
         + nw * WALL_BYTES["m32"]
     step_gbs = step_bytes * args.steps / t / 1e9

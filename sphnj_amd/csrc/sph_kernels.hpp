@@ -591,13 +591,22 @@ __global__ void __launch_bounds__(256) k_wall_update(Dev<Pol> d) {
 // ======================================================================================
 // Tiles over the half-cell grid
 // ======================================================================================
-// Once per step (after the hash), the fluid slots of every sub-column are cut into chunks
-// of BLOCK consecutive slots, and a chunk whose slots span more than plan_span(BLOCK)
-// sub-rows (a sparse sub-column, whose neighbours would overflow the LDS budget) is split
-// into kMaxPieces tiles of equal slot counts.  Every tile gets a plan: the
-// candidate sub-columns [sc0, sc0 + nsc) and sub-rows [r0, r0 + nr) of its lanes -- the 3x3
-// cells of the tile's cell column over its rows -- as slot spans, and whether any solid or
-// wall particle lies in those cells.  The row table f_lb gives, per sub-column, the first
+// Once per step (after the hash), the fluid slots of every cell column are cut into tiles
+// of at most BLOCK lanes.  Both sub-columns of a cell column have the same candidate
+// sub-columns (the 3x3 cells' six), so a tile takes the lanes of both over a band of R
+// sub-rows: half the rows of a one-sub-column tile of as many lanes, so about half the
+// candidates to stage per lane.  R is the cell column's largest band height (at most
+// kBandMax) whose bands all hold at most BLOCK lanes; a band with none gets no tile, and one
+// with more (a single overfull sub-row pair) is split into pieces of BLOCK lanes.  A cell
+// column whose bands would need more tiles than its budget (2 per BLOCK slots of each
+// sub-column) is cut per sub-column instead: chunks of BLOCK consecutive slots, a chunk whose
+// slots span more than plan_span(BLOCK) sub-rows (a sparse sub-column, whose neighbours would
+// overflow the LDS budget) split into kMaxPieces tiles of equal slot counts.  A tile's lanes
+// are at most two slot runs: count0 lanes of the even sub-column from start, then the rest of
+// the odd one from start1.  Every tile gets a plan: the candidate sub-columns
+// [sc0, sc0 + nsc) and sub-rows [r0, r0 + nr) of its lanes -- the 3x3 cells of the tile's
+// cell column over its rows -- as slot spans, and whether any solid or wall particle lies in
+// those cells.  The row table f_lb gives, per sub-column, the first
 // slot of every sub-row or later (2 ny + 1 entries, the last one the sub-column's end), so a
 // row window [ra, rb] of a sub-column is the slot range [lb(ra), lb(rb + 1)).  The pair
 // kernel and every shift pass reuse the plans: a block loads its plan, copies the spans and
@@ -612,6 +621,7 @@ __global__ void __launch_bounds__(256) k_wall_update(Dev<Pol> d) {
 // positions -- by a margin far above the rounding of either precision -- and would have added
 // exact zeros to every sum.
 constexpr int kTileCols = 6;  // sub-columns of the 3x3 cells of a sub-column's cell column
+constexpr int kBandMax = 48;  // sub-rows of a band at most (bounds the rows a tile stages)
 // LDS budget of a tile of B slots: candidate particles and sub-rows.  A dense tile in one
 // sub-column has about 6 B + 140 candidates (six sub-columns over its rows and three more on
 // either side) over about B / 3.75 + 6 rows; a tile beyond the budget reads global memory.
@@ -627,6 +637,36 @@ constexpr int kMaxPieces = 2;
 __device__ __forceinline__ int chunk_pieces(const int *f_cxy, int start, int count, int B) {
   const int rows = cxy_y(f_cxy[start + count - 1]) - cxy_y(f_cxy[start]) + 1;
   return min(min((rows + plan_span(B) - 1) / plan_span(B), kMaxPieces), count);
+}
+
+// Lanes of a cell column in its sub-rows [a, b), from the row tables of its two sub-columns.
+__device__ __forceinline__ int band_lanes(const int *lb0, const int *lb1, int a, int b) {
+  return (lb0[b] - lb0[a]) + (lb1[b] - lb1[a]);
+}
+__device__ __forceinline__ int wave_max_int(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+// Band height of a cell column of ny2 sub-rows for tiles of B lanes (all lanes of a wave call
+// it and get it): from the densest sub-row pair up while every band still holds at most B.
+__device__ __forceinline__ int band_height(const int *lb0, const int *lb1, int ny2, int B, int lane) {
+  int dm = 0;
+  for (int r = lane; r < ny2; r += 64) dm = max(dm, band_lanes(lb0, lb1, r, r + 1));
+  dm = wave_max_int(dm);
+  int R = max(1, min(kBandMax, B / max(dm, 1)));
+  for (int it = 0; it < 8 && R < kBandMax; ++it) {
+    int m = 0;
+    for (int a = lane * (R + 1); a < ny2; a += 64 * (R + 1)) m = max(m, band_lanes(lb0, lb1, a, min(a + R + 1, ny2)));
+    if (wave_max_int(m) > B) break;
+    ++R;
+  }
+  return R;
+}
+// Tiles of the bands of R sub-rows (all lanes of a wave; lane 0's sum is the count).
+__device__ __forceinline__ int band_tiles(const int *lb0, const int *lb1, int ny2, int R, int B, int lane) {
+  int t = 0;
+  for (int a = lane * R; a < ny2; a += 64 * R) t += (band_lanes(lb0, lb1, a, min(a + R, ny2)) + B - 1) / B;
+  return wave_sum(t);
 }
 template <int BLOCK>
 struct TileRows {
@@ -645,8 +685,9 @@ struct V2<double> {
 };
 
 struct TilePlan {
-  int c, start, count;   // case, first slot, lanes
-  int sx, cx;            // sub-column and cell column of the tile
+  int c, start, count;   // case, slot of lane 0, lanes
+  int start1, count0;    // lanes [count0, count) are the slots from start1 on (odd sub-column)
+  int sx, cx;            // even sub-column and cell column of the tile
   int sc0, nsc, r0, nr;  // candidate sub-columns and sub-rows
   int cy_ref;            // cell row of the M32 shift frame
   int sw;                // wall (bit 0) / solid (bit 1) particles lie in the tile's 3x3 cells
@@ -655,27 +696,51 @@ struct TilePlan {
   int gsz, gmul;         // staging: threads per span B / nsc; ceil(2^20 / gsz) (t / gsz = t gmul >> 20)
 };
 
-// The tiles of one block size: counts per sub-column, their exclusive scan (G + 1 entries, the
-// total last) and the plans.  The planning kernels take two jobs (the count in k_subcol_rows,
-// the others by blockIdx.y), so the tiles of the pair and the shift block sizes are planned
-// by the same launches.
+// Slot and sub-column of lane t of a tile.
+__device__ __forceinline__ int lane_slot(const TilePlan &P, int t) {
+  return t < P.count0 ? P.start + t : P.start1 + (t - P.count0);
+}
+__device__ __forceinline__ int lane_subcol(const TilePlan &P, int t) { return P.sx + (t < P.count0 ? 0 : 1); }
+
+// The tiles of one block size: counts per global sub-column (a cell column's at its even
+// one), their exclusive scan (G + 1 entries, the total last), how each cell column is cut and
+// the plans.  The planning kernels take two jobs (by blockIdx.y), so the tiles of the pair and
+// the shift block sizes are planned by the same launches.
 struct PlanJob {
   int B;
   int *cnt, *off;
+  int *band;  // per even global sub-column: the cell column's band height, 0 when cut per sub-column
   TilePlan *plans;
 };
 struct PlanJobs {
   PlanJob j[2];
 };
 
-// Tiles of block size B in the sub-column whose slots are [s0, s0 + n) (all lanes of a wave
-// call it; lane 0 gets the count).
+// Tiles of block size B in the sub-column whose slots are [s0, s0 + n) when it is cut on its
+// own (all lanes of a wave call it and get the count).
 template <class Pol>
 __device__ __forceinline__ int count_tiles(const Dev<Pol> &d, int s0, int n, int B, int lane) {
   const int nch = (n + B - 1) / B;
   int t = 0;
   for (int k = lane; k < nch; k += 64) t += chunk_pieces(d.f_cxy, s0 + k * B, min(B, n - k * B), B);
   return wave_sum(t);
+}
+// Tiles of block size B of the cell column whose sub-columns have the row tables lb0, lb1 (ny2
+// sub-rows): in bands (*R = their height) while that needs at most 2 tiles per B slots of each
+// sub-column, else per sub-column (*R = 0).  All lanes of a wave call it and get the count.
+template <class Pol>
+__device__ __forceinline__ int cell_column_tiles(const Dev<Pol> &d, const int *lb0, const int *lb1, int ny2, int B,
+                                                 int lane, int *R) {
+  const int n0 = lb0[ny2] - lb0[0], n1 = lb1[ny2] - lb1[0];
+  const int budget = kMaxPieces * ((n0 + B - 1) / B + (n1 + B - 1) / B);
+  const int h = band_height(lb0, lb1, ny2, B, lane);
+  const int nb = band_tiles(lb0, lb1, ny2, h, B, lane);
+  if (nb <= budget) {
+    *R = h;
+    return nb;
+  }
+  *R = 0;
+  return count_tiles(d, lb0[0], n0, B, lane) + count_tiles(d, lb1[0], n1, B, lane);
 }
 
 // Wall row table w_lb of every cell column (once, after the walls are sorted; a thread per
@@ -706,20 +771,18 @@ __global__ void __launch_bounds__(256) k_wall_rows(Dev<Pol> d, int G, const int 
 
 // Row table of every sub-column (one block of NW waves per global sub-column): f_lb[k] for row
 // k = the suffix minimum of the non-empty rows' starts, with the sub-column's end past the last
-// non-empty row; and the sub-column's tile counts for the njobs block sizes.  The blocks of
-// even sub-columns also audit their cell column's cells against the reference's 64-entry cell
-// capacity (include/global.h:9) and build its f_swp (when the batch has solids).  With four
-// waves (a few thousand sub-columns) the waves build the row table together, then wave 0 counts
-// the tiles while waves 1 and 2 audit and index the cell column; with one (many small
+// non-empty row.  The blocks of even sub-columns also audit their cell column's cells against
+// the reference's 64-entry cell capacity (include/global.h:9) and build its f_swp (when the
+// batch has solids).  With four waves (a few thousand sub-columns) the waves build the row
+// table together, then waves 1 and 2 audit and index the cell column; with one (many small
 // sub-columns, a batch of cases) the wave does all of it.
 template <class Pol, int NW>
-__global__ void __launch_bounds__(64 * NW) k_subcol_rows(Dev<Pol> d, int G, const int *col_case, PlanJobs jobs,
-                                                         int njobs, int *reset_moved) {
+__global__ void __launch_bounds__(64 * NW) k_subcol_rows(Dev<Pol> d, int G, const int *col_case, int *reset_moved) {
   constexpr int NT = 64 * NW;
   const int g = blockIdx.x, t = threadIdx.x, w = t >> 6, lane = t & 63;
   if (g >= G) return;
   if (reset_moved != nullptr && g == 0 && t == 0) *reset_moved = 0;  // read by the earlier kernels only
-  __shared__ int s_min[NW], s_end[NW], s_s0;
+  __shared__ int s_min[NW], s_end[NW];
   const auto &cp = d.cases[col_case[g]];
   const int sx = g - cp.col_off, ny2 = 2 * cp.ny;
   const int kb = 4 * cp.cell_off + sx * ny2;
@@ -771,18 +834,7 @@ __global__ void __launch_bounds__(64 * NW) k_subcol_rows(Dev<Pol> d, int G, cons
       lb[ka] = run;
     }
   }
-  if (t == 0) {
-    lb[ny2] = end;
-    s_s0 = run;  // = lb[0]
-  }
-  __syncthreads();
-  if (w == 0) {  // the sub-column's tile counts
-    const int s0 = s_s0;
-    for (int j = 0; j < njobs; ++j) {
-      const int nt = count_tiles(d, s0, end - s0, jobs.j[j].B, lane);
-      if (lane == 0) jobs.j[j].cnt[g] = nt;
-    }
-  }
+  if (t == 0) lb[ny2] = end;
   if (w == 1 % NW && (sx & 1) == 0) {  // the block of an even sub-column audits its cell column
     const int qc = cp.cell_off + (sx >> 1) * cp.ny, kc = 4 * cp.cell_off + sx * ny2;
     bool over = false;
@@ -825,21 +877,26 @@ __global__ void __launch_bounds__(64 * NW) k_subcol_rows(Dev<Pol> d, int G, cons
   }
 }
 
-// Tiles per sub-column (one wave per global sub-column, a lane per chunk).
+// Tiles per cell column, counted at its even global sub-column (0 at the odd one), and how it
+// is cut (one wave per global sub-column).
 template <class Pol>
 __global__ void __launch_bounds__(256) k_tile_count(Dev<Pol> d, int G, const int *col_case, PlanJobs jobs) {
   const int g = blockIdx.x * 4 + (int)(threadIdx.x >> 6), lane = threadIdx.x & 63;
   if (g >= G) return;
-  const int B = jobs.j[blockIdx.y].B;
-  int *cnt = jobs.j[blockIdx.y].cnt;
+  const PlanJob &J = jobs.j[blockIdx.y];
   const auto &cp = d.cases[col_case[g]];
-  const int ny2 = 2 * cp.ny;
-  const int *lb = d.f_lb + cp.lb_off + (g - cp.col_off) * (ny2 + 1);
-  const int s0 = lb[0], n = lb[ny2] - s0, nch = (n + B - 1) / B;
-  int t = 0;
-  for (int k = lane; k < nch; k += 64) t += chunk_pieces(d.f_cxy, s0 + k * B, min(B, n - k * B), B);
-  t = wave_sum(t);
-  if (lane == 0) cnt[g] = t;
+  const int sx = g - cp.col_off, ny2 = 2 * cp.ny;
+  if (sx & 1) {
+    if (lane == 0) J.cnt[g] = J.band[g] = 0;
+    return;
+  }
+  const int *lb0 = d.f_lb + cp.lb_off + sx * (ny2 + 1), *lb1 = lb0 + (ny2 + 1);
+  int R;
+  const int t = cell_column_tiles(d, lb0, lb1, ny2, J.B, lane, &R);
+  if (lane == 0) {
+    J.cnt[g] = t;
+    J.band[g] = R;
+  }
 }
 
 // Exclusive scan of the per-sub-column tile counts (one block of 1024 threads per job); the
@@ -886,7 +943,8 @@ __global__ void __launch_bounds__(1024) k_tile_scan(PlanJobs jobs, int G) {
   if (t == 0) off[G] = total;
 }
 
-// Plans of the tiles of size B (one wave per global sub-column, a lane per chunk).
+// Plans of the tiles of size B (one wave per global sub-column; the wave of a cell column's even
+// sub-column plans it, a lane per band or chunk).
 template <class Pol>
 __global__ void __launch_bounds__(256) k_tile_plan(Dev<Pol> d, int G, const int *col_case, PlanJobs jobs,
                                                    int scanned) {
@@ -919,60 +977,86 @@ __global__ void __launch_bounds__(256) k_tile_plan(Dev<Pol> d, int G, const int 
   const int c = col_case[g];
   const auto &cp = d.cases[c];
   const int sx = g - cp.col_off, cx = sx >> 1, ny2 = 2 * cp.ny;
+  if (sx & 1) return;  // a cell column is planned at its even sub-column
   const int *lbc = d.f_lb + cp.lb_off;
-  const int s0 = lbc[sx * (ny2 + 1)], n = lbc[sx * (ny2 + 1) + ny2] - s0, nch = (n + B - 1) / B;
+  const int *lb0 = lbc + sx * (ny2 + 1), *lb1 = lb0 + (ny2 + 1);
   const int sc0 = max(2 * cx - 2, 0), sc1 = min(2 * cx + 3, 2 * cp.nx - 1), nsc = sc1 - sc0 + 1;
   const int wc0 = max(cx - 1, 0), wc1 = min(cx + 1, cp.nx - 1);
   const int *swp = d.f_swp + cp.swp_off, *wlb = d.w_lb + cp.swp_off;
-  for (int k0 = 0; k0 < nch; k0 += 64) {
-    const int k = k0 + lane, cstart = s0 + k * B, ccount = k < nch ? min(B, n - k * B) : 0;
-    const int np = k < nch ? chunk_pieces(d.f_cxy, cstart, ccount, B) : 0;
-    const int incl = wave_incl_sum(np, lane);
-    for (int q = 0; q < np; ++q) {
-      TilePlan &P = plans[tile + incl - np + q];
-      const int start = cstart + q * ccount / np, count = cstart + (q + 1) * ccount / np - start;
-      const int sy0 = cxy_y(d.f_cxy[start]), sy1 = cxy_y(d.f_cxy[start + count - 1]);
-      const int r0 = max(2 * (sy0 >> 1) - 2, 0), r1 = min(2 * (sy1 >> 1) + 3, ny2 - 1), nr = r1 - r0 + 1;
-      int total = 0;
+  // the plan of a tile whose lanes lie in sub-rows [sy0, sy1]
+  auto plan = [&](TilePlan &P, int start, int count0, int start1, int count, int sy0, int sy1) {
+    const int r0 = max(2 * (sy0 >> 1) - 2, 0), r1 = min(2 * (sy1 >> 1) + 3, ny2 - 1), nr = r1 - r0 + 1;
+    int total = 0;
 #pragma unroll
-      for (int a = 0; a < kTileCols; ++a) {
-        int lo = 0, len = 0, row = 0;
-        if (a < nsc) {
-          row = (sc0 + a) * (ny2 + 1) + r0;
-          lo = lbc[row];
-          len = lbc[row + nr] - lo;
-          row += cp.lb_off;
-        }
-        P.lo[a] = lo;
-        P.len[a] = len;
-        P.row[a] = row;
-        total += len;
+    for (int a = 0; a < kTileCols; ++a) {
+      int lo = 0, len = 0, row = 0;
+      if (a < nsc) {
+        row = (sc0 + a) * (ny2 + 1) + r0;
+        lo = lbc[row];
+        len = lbc[row + nr] - lo;
+        row += cp.lb_off;
       }
-      // wall (bit 0) and solid (bit 1) particles in the 3x3 cells of the tile's lanes
-      const int wr0 = max((sy0 >> 1) - 1, 0), wr1 = min((sy1 >> 1) + 1, cp.ny - 1);
-      int nw = 0, ns = 0;
-      for (int col = wc0; col <= wc1; ++col) {
-        const int q = col * (cp.ny + 1);
-        nw += wlb[q + wr1 + 1] - wlb[q + wr0];
-        if (d.ns > 0) ns += swp[q + wr1 + 1] - swp[q + wr0];
-      }
-      const int sw = (nw > 0 ? 1 : 0) | (ns > 0 ? 2 : 0);
-      P.c = c;
-      P.start = start;
-      P.count = count;
-      P.sx = sx;
-      P.cx = cx;
-      P.sc0 = sc0;
-      P.nsc = nsc;
-      P.r0 = r0;
-      P.nr = nr;
-      P.cy_ref = (sy0 + sy1) >> 2;
-      P.sw = sw;
-      P.total = total;
-      P.gsz = B / nsc;
-      P.gmul = ((1 << 20) + P.gsz - 1) / P.gsz;
+      P.lo[a] = lo;
+      P.len[a] = len;
+      P.row[a] = row;
+      total += len;
     }
-    tile += __shfl(incl, 63, 64);
+    // wall (bit 0) and solid (bit 1) particles in the 3x3 cells of the tile's lanes
+    const int wr0 = max((sy0 >> 1) - 1, 0), wr1 = min((sy1 >> 1) + 1, cp.ny - 1);
+    int nw = 0, ns = 0;
+    for (int col = wc0; col <= wc1; ++col) {
+      const int q = col * (cp.ny + 1);
+      nw += wlb[q + wr1 + 1] - wlb[q + wr0];
+      if (d.ns > 0) ns += swp[q + wr1 + 1] - swp[q + wr0];
+    }
+    P.c = c;
+    P.start = start;
+    P.count = count;
+    P.start1 = start1;
+    P.count0 = count0;
+    P.sx = sx;
+    P.cx = cx;
+    P.sc0 = sc0;
+    P.nsc = nsc;
+    P.r0 = r0;
+    P.nr = nr;
+    P.cy_ref = (sy0 + sy1) >> 2;
+    P.sw = (nw > 0 ? 1 : 0) | (ns > 0 ? 2 : 0);
+    P.total = total;
+    P.gsz = B / nsc;
+    P.gmul = ((1 << 20) + P.gsz - 1) / P.gsz;
+  };
+  const int R = jobs.j[blockIdx.y].band[g];
+  if (R > 0) {  // bands of R sub-rows over both sub-columns, a lane per band
+    const int nb = (ny2 + R - 1) / R;
+    for (int k0 = 0; k0 < nb; k0 += 64) {
+      const int k = k0 + lane, a = min(k * R, ny2), b = min(a + R, ny2);
+      const int n = k < nb ? band_lanes(lb0, lb1, a, b) : 0, n0 = k < nb ? lb0[b] - lb0[a] : 0;
+      const int np = (n + B - 1) / B;
+      const int incl = wave_incl_sum(np, lane);
+      for (int q = 0; q < np; ++q) {  // lanes [q B, q B + cnt) of the band: even sub-column first
+        const int u0 = q * B, cnt = min(B, n - u0), c0 = max(0, min(n0 - u0, cnt));
+        const int s1 = lb1[a] + max(0, u0 - n0);
+        plan(plans[tile + incl - np + q], u0 < n0 ? lb0[a] + u0 : s1, c0, s1, cnt, a, b - 1);
+      }
+      tile += __shfl(incl, 63, 64);
+    }
+  } else {  // each sub-column on its own, a lane per chunk
+    for (int h = 0; h < 2; ++h) {
+      const int *lb = h ? lb1 : lb0;
+      const int s0 = lb[0], n = lb[ny2] - s0, nch = (n + B - 1) / B;
+      for (int k0 = 0; k0 < nch; k0 += 64) {
+        const int k = k0 + lane, cstart = s0 + k * B, ccount = k < nch ? min(B, n - k * B) : 0;
+        const int np = k < nch ? chunk_pieces(d.f_cxy, cstart, ccount, B) : 0;
+        const int incl = wave_incl_sum(np, lane);
+        for (int q = 0; q < np; ++q) {
+          const int start = cstart + q * ccount / np, count = cstart + (q + 1) * ccount / np - start;
+          plan(plans[tile + incl - np + q], start, h ? 0 : count, start, count, cxy_y(d.f_cxy[start]),
+               cxy_y(d.f_cxy[start + count - 1]));
+        }
+        tile += __shfl(incl, 63, 64);
+      }
+    }
   }
 }
 
@@ -1110,14 +1194,13 @@ __device__ __forceinline__ LanePos lane_pos_view(float xi, float yi, int cx, int
 constexpr int kWindowOrder = 7;
 __device__ __forceinline__ int window_offset(int q) { return (q & 1) ? -((q + 1) >> 1) : (q >> 1); }
 
-// Visits the candidate row windows of a lane: f(sc, ra, rb) for each sub-column sc that meets
-// the lane's disc of radius R, with the sub-rows [ra, rb] that do, clipped to the 3x3 cells
-// of the lane's cell (the tile's cell column, row cy) and to the tile's candidate range;
-// sub-columns in window_offset order (PAIRED) or from left to right.
+// Visits the candidate row windows of a lane of sub-column sx: f(sc, ra, rb) for each
+// sub-column sc that meets the lane's disc of radius R, with the sub-rows [ra, rb] that do,
+// clipped to the 3x3 cells of the lane's cell (the tile's cell column, row cy) and to the
+// tile's candidate range; sub-columns in window_offset order (PAIRED) or from left to right.
 template <bool PAIRED, class CP, class F>
-__device__ __forceinline__ void for_windows(const TilePlan &P, const CP &cp, int sy, int cy, LanePos lp, float R,
-                                            F &&f) {
-  const int sx = P.sx;
+__device__ __forceinline__ void for_windows(const TilePlan &P, const CP &cp, int sx, int sy, int cy, LanePos lp,
+                                            float R, F &&f) {
   const float R2 = R * R;
   const int a_lo = max((int)floorf(lp.fx - R * cp.inv_ssx), P.sc0 - sx);
   const int a_hi = min((int)floorf(lp.fx + R * cp.inv_ssx), P.sc0 + P.nsc - 1 - sx);
@@ -1134,13 +1217,13 @@ __device__ __forceinline__ void for_windows(const TilePlan &P, const CP &cp, int
   }
 }
 
-// The row window [ra, rb] of a lane in candidate sub-column sc0 + sa (for_windows for one
-// sub-column); false when the sub-column holds no candidate of the lane.
+// The row window [ra, rb] of a lane of sub-column sx in candidate sub-column sc0 + sa
+// (for_windows for one sub-column); false when the sub-column holds no candidate of the lane.
 template <class CP>
-__device__ __forceinline__ bool span_window(const TilePlan &P, const CP &cp, int sa, int sy, int cy, LanePos lp,
-                                            float R, int &ra, int &rb) {
+__device__ __forceinline__ bool span_window(const TilePlan &P, const CP &cp, int sx, int sa, int sy, int cy,
+                                            LanePos lp, float R, int &ra, int &rb) {
   if (sa < 0 || sa >= P.nsc) return false;
-  const int a = P.sc0 + sa - P.sx;
+  const int a = P.sc0 + sa - sx;
   const float R2 = R * R;
   if (a < (int)floorf(lp.fx - R * cp.inv_ssx) || a > (int)floorf(lp.fx + R * cp.inv_ssx)) return false;
   const float g = fmaxf(fmaxf((float)a - lp.fx, lp.fx - (float)(a + 1)), 0.f) * cp.ssx;
@@ -1342,8 +1425,9 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
   const int nt = *ntiles, tile = xcd_tile(blockIdx.x, nt);
   if (tile >= nt) return;
   const TilePlan &P = plans[tile];
-  const int t = threadIdx.x, i = P.start + t, c = P.c;
+  const int t = threadIdx.x, c = P.c;
   const bool valid = t < P.count;
+  const int i = valid ? lane_slot(P, t) : 0, lsx = lane_subcol(P, t);  // the lane's slot, sub-column
   const auto &A = d.fa;
   const bool staged = P.total <= CAP && P.nr <= TileRows<BLOCK>::value;
   int dmax = 0;  // bit pattern of the largest path length (non-negative floats order as ints)
@@ -1420,21 +1504,21 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
     const double x0 = A.x[i], y0 = A.y[i];
     LanePos lp;
     if constexpr (Pol::MIXED)
-      lp = lane_pos_view(xi, yi, cx, cy, P.sx, sy, cp);
+      lp = lane_pos_view(xi, yi, cx, cy, lsx, sy, cp);
     else
-      lp = lane_pos(x0, y0, P.sx, sy, cp);
+      lp = lane_pos(x0, y0, lsx, sy, cp);
     const int phi = d.f_phase[i];
     const bool liq = phi == 0;
     const R cs = cp.c[phi], csinv = cp.cinv[phi], mu8 = cp.mu8[phi];
     const R rhoi_inv = krcp<R>(rhoi);
     const LoopConsts<R> K(cp);
     const R eps_h2 = cp.eps_h2, qrc2 = cp.quarter_rc2;
-    // LDS index of the lane's own slot (its span is the tile's own sub-column)
+    // LDS index of the lane's own slot (in the span of its sub-column)
     int own = i;
     if (staged) {
       int b = 0;
-      for (int a = 0; a < P.sx - P.sc0; ++a) b += P.len[a];
-      own = b + (i - P.lo[P.sx - P.sc0]);
+      for (int a = 0; a < lsx - P.sc0; ++a) b += P.len[a];
+      own = b + (i - P.lo[lsx - P.sc0]);
     }
     PairSums<R> S;
     // fluidFluidCalculate on one candidate (w = 0 outside the cut-off: exact zero terms);
@@ -1531,14 +1615,14 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
         use(cur);
       };
       auto eval = [&](unsigned m, int s0) { eval2(m, s0, 0u, 0); };
-      const int off = P.sx - P.sc0;  // tile sub-column of the lanes' own sub-column
+      const int off = lsx - P.sc0;  // tile sub-column of the lane's own sub-column
 #pragma unroll
       for (int q = 0; q < kWindowOrder; ++q) {
         mk[q] = 0u;
         w0[q] = 0;
         const int sa = off + window_offset(q);
         int ra, rb;
-        if (!span_window(P, cp, sa, sy, cy, lp, Rp, ra, rb)) continue;
+        if (!span_window(P, cp, lsx, sa, sy, cy, lp, Rp, ra, rb)) continue;
         const int s0 = s_lb[sa][ra - P.r0], s1 = s_lb[sa][rb + 1 - P.r0];
         w0[q] = s0;
         mk[q] = mark(s0, s1);
@@ -1555,7 +1639,7 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
         }
       }
     }
-    for_windows<true>(P, cp, sy, cy, lp, Rp, [&](int sc, int ra, int rb) {
+    for_windows<true>(P, cp, lsx, sy, cy, lp, Rp, [&](int sc, int ra, int rb) {
       if (staged) return;
       const R bx = xi - cell_shift_x<Pol>((sc >> 1) - cx, cp);
       if constexpr (Pol::MIXED) {
@@ -1681,11 +1765,12 @@ __global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const T
 // cy_ref), in units scaled so that the weight is 2^-r'^2; the frame costs at most a few
 // float ulps of a tile height in the gradient of a small correction.
 // A lane's own state for a shift pass (loaded ahead of the staging so that the loads overlap
-// it): packed sub-cell, class, position, path length and the shift accumulated so far.  The position is the view the neighbour loops read (for P64 the position itself); M32
-// reads its double masters, which stay at the pair kernel's result until then, on the case's
-// last pass only.  A pass after a recording one also reads the lane's runs (the path since the
-// hash on the last pass only), and the sub-cell only where it needs it before the loops:
-// walls or solids in the tile, or the last pass (pk = -1 marks it unread).
+// it): packed sub-cell, class, position, path length and the shift accumulated so far.  The
+// position is the view the neighbour loops read (for P64 the position itself); M32 reads its
+// double masters, which stay at the pair kernel's result until then, on the case's last pass
+// only.  A pass after a recording one also reads the lane's runs (the path since the hash on
+// the last pass only), and the sub-cell only where it needs it before the loops: walls or
+// solids in the tile, or the last pass (pk = -1 marks it unread).
 template <class Pol>
 struct ShiftOwn {
   using R = typename Pol::R;
@@ -1700,8 +1785,9 @@ struct ShiftOwn {
 template <class Pol, int MODE>
 __device__ __forceinline__ ShiftOwn<Pol> shift_own(const Dev<Pol> &d, const TilePlan &P, int pass, int K) {
   ShiftOwn<Pol> o;
-  const int t = threadIdx.x, i = P.start + t;
+  const int t = threadIdx.x;
   if (t < P.count) {
+    const int i = lane_slot(P, t);
     const bool lastp = pass == K - 1;
     o.pk = (MODE != kShiftCached || lastp || P.sw) ? d.f_cxy[i] : -1;
     if (!Pol::MIXED || lastp) {
@@ -1738,8 +1824,9 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
                                            const WallSpans &W) {
   using R = typename Pol::R;
   using R2 = typename V2<R>::T;
-  const int t = threadIdx.x, i = P.start + t, c = P.c;
+  const int t = threadIdx.x, c = P.c;
   if (t >= P.count) return;
+  const int i = lane_slot(P, t), lsx = lane_subcol(P, t);  // the lane's slot and sub-column
   const auto &A = d.fa;
   const auto &B = d.fb;
   const auto &cp = d.cases[c];
@@ -1762,9 +1849,9 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
   const R xi = Pol::MIXED ? own.vx : (R)x0, yi = Pol::MIXED ? own.vy : (R)y0;
   LanePos lp;
   if constexpr (Pol::MIXED)
-    lp = lane_pos_view(xi, yi, cx, cy, P.sx, sy, cp);
+    lp = lane_pos_view(xi, yi, cx, cy, lsx, sy, cp);
   else
-    lp = lane_pos(x0, y0, P.sx, sy, cp);
+    lp = lane_pos(x0, y0, lsx, sy, cp);
   const LoopConsts<R> K(cp);
   R gx = 0, gy = 0;
   auto add = [&](R dx, R dy) {
@@ -1780,8 +1867,8 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
   R gxs = 0, gys = 0;
   R xs = xi, ys = yi;
   if (staged) {
-    int own = i - P.lo[P.sx - P.sc0];
-    for (int a = 0; a < P.sx - P.sc0; ++a) own += P.len[a];
+    int own = i - P.lo[lsx - P.sc0];
+    for (int a = 0; a < lsx - P.sc0; ++a) own += P.len[a];
     const R2 o = s_pos[own];
     xs = o.x;
     ys = o.y;
@@ -1834,7 +1921,7 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
     for_range_4_2_1(s4, s4 + (int)(z >> 25), staged_add);
   } else {
     const R rl2 = cp.rl2_rec;
-    for_windows<false>(P, cp, sy, cy, lp, Rp, [&](int sc, int ra, int rb) {
+    for_windows<false>(P, cp, lsx, sy, cy, lp, Rp, [&](int sc, int ra, int rb) {
       if (staged) {
         const int sa = sc - P.sc0;
         const int lo = s_lb[sa][ra - P.r0], hi = s_lb[sa][rb + 1 - P.r0];

@@ -168,6 +168,7 @@ struct Engine : EngineBase {
   int64_t lb_total = 0, swp_total = 0;
   int *col_case = nullptr;
   int *tile_cnt[3] = {nullptr, nullptr, nullptr}, *tile_off[3] = {nullptr, nullptr, nullptr};
+  int *tile_band[3] = {nullptr, nullptr, nullptr};
   TilePlan *plans[3] = {nullptr, nullptr, nullptr};
   bool plan_ok[3] = {false, false, false};
   static int bidx(int B) { return B == 64 ? 0 : B == 128 ? 1 : 2; }
@@ -470,7 +471,7 @@ struct Engine : EngineBase {
     }
     for (int b = 0; b < 3; ++b) {
       const int B = 64 << b;
-      if ((rc = dalloc(&tile_cnt[b], G)) || (rc = dalloc(&tile_off[b], G + 1)) ||
+      if ((rc = dalloc(&tile_cnt[b], G)) || (rc = dalloc(&tile_off[b], G + 1)) || (rc = dalloc(&tile_band[b], G)) ||
           (rc = dalloc(&plans[b], max_tiles(B))))
         return rc;
     }
@@ -748,16 +749,14 @@ struct Engine : EngineBase {
       k_cell_bounds<<<nblk(ns, 256), 256, 0, stream>>>(s_sorted_keys, ns, d.s_range, moved);
     }
     plan_ok[0] = plan_ok[1] = plan_ok[2] = false;
-    if (nf > 0) {  // row tables, the cell audit and the tile counts of both block sizes
-      PlanJobs jobs{};
-      const int nj = plan_jobs(blk_pair, blk_shift, jobs);
+    if (nf > 0) {  // row tables and the cell audit, then the tiles of both block sizes
       if (G <= kPlanPrefixMax)
-        k_subcol_rows<Pol, 4><<<G, 256, 0, stream>>>(d, G, col_case, jobs, nj, moved);
+        k_subcol_rows<Pol, 4><<<G, 256, 0, stream>>>(d, G, col_case, moved);
       else
-        k_subcol_rows<Pol, 1><<<G, 64, 0, stream>>>(d, G, col_case, jobs, nj, moved);
+        k_subcol_rows<Pol, 1><<<G, 64, 0, stream>>>(d, G, col_case, moved);
       LAUNCH_CHECK();
       int rc;
-      if ((rc = ensure_plans(blk_pair, blk_shift, true))) return rc;
+      if ((rc = ensure_plans(blk_pair, blk_shift))) return rc;
     } else {
       k_cell_audit<Pol><<<nblk(nc, 256), 256, 0, stream>>>(d, d.status);
       LAUNCH_CHECK();
@@ -774,17 +773,16 @@ struct Engine : EngineBase {
       if (B == 0) continue;
       const int b = bidx(B);
       if (plan_ok[b] || (nj == 1 && jobs.j[0].B == B)) continue;
-      jobs.j[nj++] = PlanJob{B, tile_cnt[b], tile_off[b], plans[b]};
+      jobs.j[nj++] = PlanJob{B, tile_cnt[b], tile_off[b], tile_band[b], plans[b]};
     }
     return nj;
   }
-  // counted: k_subcol_rows has just written these jobs' tile counts
-  int ensure_plans(int B1, int B2 = 0, bool counted = false) {
+  int ensure_plans(int B1, int B2 = 0) {
     if (d.nf == 0) return SPHNJ_OK;
     PlanJobs jobs{};
     const int nj = plan_jobs(B1, B2, jobs);
     if (nj == 0) return SPHNJ_OK;
-    if (!counted) k_tile_count<Pol><<<dim3(nblk(G, 4), nj), 256, 0, stream>>>(d, G, col_case, jobs);
+    k_tile_count<Pol><<<dim3(nblk(G, 4), nj), 256, 0, stream>>>(d, G, col_case, jobs);
     const int scanned = G > kPlanPrefixMax;
     if (scanned) k_tile_scan<<<nj, 1024, 0, stream>>>(jobs, G);
     k_tile_plan<Pol><<<dim3(nblk(G, 4), nj), 256, 0, stream>>>(d, G, col_case, jobs, scanned);

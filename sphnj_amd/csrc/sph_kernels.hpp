@@ -1395,9 +1395,12 @@ template <class Pol, int BLOCK>
 struct PairCap {
   static constexpr int value = plan_cap(BLOCK) - ((Pol::MIXED && BLOCK == 256) ? 104 : 0);
 };
+// The shift stages three quarters of the pair's budget: tiles over both sub-columns of a cell
+// column stage about 3.4 candidates a lane (a dense 256-lane tile ~870), and the smaller LDS
+// lets 8 blocks of 256 share a CU.
 template <int BLOCK>
 struct ShiftCap {
-  static constexpr int value = plan_cap(BLOCK);
+  static constexpr int value = plan_cap(BLOCK) * 3 / 4;
 };
 
 // findFluidPair (src/fluid.cu:193-254) + fluidUpdate (src/fluid.cu:256-287), one tile per
@@ -2056,10 +2059,11 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
 // correctionCalculate (src/fluid.cu:316-367) + correctionUpdate / shiftDisplacementCalculate
 // (src/fluid.cu:369-427), one tile per block: stage the tile, then shift_tile.  MODE is
 // kShiftPlain, kShiftRecord or kShiftCached (the step's one pass, its first of several, a
-// later one).
+// later one).  The cached passes are held to 8 waves per SIMD (64 registers, a few spilled:
+// 6% faster than 7), the others to 7.
 template <class Pol, int BLOCK, int MODE>
-__global__ void __launch_bounds__(BLOCK) k_fluid_shift(Dev<Pol> d, const TilePlan *plans, const int *ntiles,
-                                                       int pass, int use_runs) {
+__global__ void __launch_bounds__(BLOCK, (MODE == kShiftCached ? 8 : 7))
+    k_fluid_shift(Dev<Pol> d, const TilePlan *plans, const int *ntiles, int pass, int use_runs) {
   using R = typename Pol::R;
   using R2 = typename V2<R>::T;
   constexpr int CAP = ShiftCap<BLOCK>::value;

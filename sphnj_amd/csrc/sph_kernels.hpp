@@ -1386,14 +1386,13 @@ __device__ __forceinline__ void riemann_pair(R dx, R dy, R r2, R w, R ui, R vi, 
 }
 
 // LDS capacity (particles) of a tile of BLOCK slots: the planner's budget.  The M32 pair kernel
-// at 256 slots (25 B of LDS per candidate) gives up 104 of them so that three blocks share a
-// CU's 160 KB (53.7 KB each, 12 waves instead of 8); a 256-slot tile of a dense sub-column
-// stages about 1,650 candidates, well inside the 1,944 left.  FP64 candidates take 49 B, so
-// the next block count per CU is out of reach for P64 at any size without falling below a
-// typical tile.
+// (25 B of LDS per candidate) stages five eighths of it: a dense 256-lane tile over both
+// sub-columns of a cell column stages about 870 candidates, and at 37 KB per block with its
+// registers held to 104 (no spills) four blocks share a CU (16 waves instead of 12: 6%
+// faster).  FP64 candidates take 49 B, so P64 keeps the full budget.
 template <class Pol, int BLOCK>
 struct PairCap {
-  static constexpr int value = plan_cap(BLOCK) - ((Pol::MIXED && BLOCK == 256) ? 104 : 0);
+  static constexpr int value = Pol::MIXED ? plan_cap(BLOCK) * 5 / 8 : plan_cap(BLOCK);
 };
 // The shift stages three quarters of the pair's budget: tiles over both sub-columns of a cell
 // column stage about 3.4 candidates a lane (a dense 256-lane tile ~870), and the smaller LDS
@@ -1409,7 +1408,8 @@ struct ShiftCap {
 // reduces the per-case max |u|^2 of the updated velocities (getMaxSpeed,
 // src/fluid.cu:429-440) and advances every particle's path length since the hash.
 template <class Pol, bool VISCID, int BLOCK>
-__global__ void __launch_bounds__(BLOCK) k_fluid_pair_update(Dev<Pol> d, const TilePlan *plans, const int *ntiles) {
+__global__ void __launch_bounds__(BLOCK, (Pol::MIXED ? 4 : 1))
+    k_fluid_pair_update(Dev<Pol> d, const TilePlan *plans, const int *ntiles) {
   using R = typename Pol::R;
   using R2 = typename V2<R>::T;
   constexpr int CAP = PairCap<Pol, BLOCK>::value;

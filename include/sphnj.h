@@ -245,13 +245,14 @@ int sphnj_set_persistent_solid(sphnj_ctx *ctx, int enable);
  * the full pruned windows in every pass; mode 2 records the runs and never walks them.  All
  * three give the same results bit for bit.  SPHNJ_E_ARG for another mode. */
 int sphnj_set_shift_runs(sphnj_ctx *ctx, int mode);
-/* Autotuner: times the fluid pair kernel (family 0) and the shift kernel (family 1) at block
- * sizes 64, 128 and 256 -- a block owns that many consecutive sorted particles, so the block
- * size also sets the cells each LDS tile spans -- on the current state with HIP events, and
- * keeps the fastest.  The particle state is unchanged afterwards; the per-step transients
- * (colour, near flag, velocity gradient) are recomputed from it. */
+/* Autotuner: times the fluid pair kernel (family 0) and a step's shift passes (family 1) at
+ * block sizes 64, 128 and 256 -- a block owns that many sorted particles of a cell column, so
+ * the block size also sets the cells each LDS tile spans -- on the current state with HIP
+ * events, and keeps the fastest.  The particle state is unchanged afterwards; the per-step
+ * transients (colour, near flag, velocity gradient, position views) are recomputed from it. */
 int sphnj_autotune(sphnj_ctx *ctx);
-/* Milliseconds per launch measured by the last sphnj_autotune for block sizes 64, 128, 256. */
+/* Milliseconds per launch (per pass for the shift) measured by the last sphnj_autotune for
+ * block sizes 64, 128, 256. */
 int sphnj_autotune_report(const sphnj_ctx *ctx, int32_t family, double ms_per_launch[3]);
 /* Block size in use for a kernel family (0 = pair, 1 = shift, 2 = streaming). */
 int sphnj_get_block(const sphnj_ctx *ctx, int32_t family, int32_t *block);

@@ -839,10 +839,12 @@ struct Engine : EngineBase {
     return SPHNJ_OK;
   }
 
-  // Autotuner: times the pair and shift kernels at each candidate block size (= tile size in
-  // slots) on the current state with HIP events and keeps the fastest.  The pair kernel only
-  // writes the alternate buffers and the per-step transients; a case's last shift pass also
-  // updates u, v and (M32) x, y in place, which are saved before and restored after.
+  // Autotuner: times the pair kernel and a step's K shift passes at each candidate block size
+  // (= tile size in slots) on the current state with HIP events and keeps the fastest
+  // (tune_ms holds ms per launch, per pass for the shift).  The pair kernel only writes the
+  // alternate buffers and the per-step transients; the shift passes move the views and path
+  // lengths (rebuilt by the next hash) and a case's last pass also updates u, v and x, y in
+  // place, which are saved before and restored after.
   int autotune() override {
     if (d.nf == 0) return SPHNJ_OK;
     const int nf = d.nf;
@@ -863,25 +865,35 @@ struct Engine : EngineBase {
       return SPHNJ_E_HIP;
     }
     const int cand[3] = {64, 128, 256};
+    // the shift family runs a step's K passes as the stage does (the first records, the later
+    // ones walk the runs), buffer swaps included; their parity is restored below
+    int swaps = 0;
+    auto run = [&](int fam, int bs) -> int {
+      if (fam == 0) return launch_pair(bs);
+      for (int k = 0; k < K; ++k) {
+        const int r = launch_shift(bs, k);
+        if (r) return r;
+        swap_shift_buffers();
+        ++swaps;
+      }
+      return SPHNJ_OK;
+    };
     for (int fam = 0; fam < 2; ++fam) {
+      if (fam == 1 && K == 0) break;
       double best = 1e300;
       int best_b = 256;
       for (int bs : cand) {
-        for (int w = 0; w < 2; ++w) {
-          rc = fam == 0 ? launch_pair(bs) : launch_shift(bs, 0);
-          if (rc) return rc;
-        }
+        const int warm = fam == 0 ? 2 : 1, reps = fam == 0 ? 5 : 2;
+        for (int w = 0; w < warm; ++w)
+          if ((rc = run(fam, bs))) return rc;
         HIPC(hipEventRecord(a, stream));
-        const int reps = 5;
-        for (int r = 0; r < reps; ++r) {
-          rc = fam == 0 ? launch_pair(bs) : launch_shift(bs, 0);
-          if (rc) return rc;
-        }
+        for (int r = 0; r < reps; ++r)
+          if ((rc = run(fam, bs))) return rc;
         HIPC(hipEventRecord(b, stream));
         HIPC(hipEventSynchronize(b));
         float ms = 0.f;
         HIPC(hipEventElapsedTime(&ms, a, b));
-        tune_ms[fam][bs == 64 ? 0 : bs == 128 ? 1 : 2] = ms / reps;
+        tune_ms[fam][bs == 64 ? 0 : bs == 128 ? 1 : 2] = ms / reps / (fam == 0 ? 1 : K);
         if (ms < best) {
           best = ms;
           best_b = bs;
@@ -894,6 +906,7 @@ struct Engine : EngineBase {
     }
     ev_pool.push_back(a);
     ev_pool.push_back(b);
+    if (swaps & 1) swap_shift_buffers();
     HIPC(hipMemcpyAsync(d.fa.u, su, nf * sizeof(double), hipMemcpyDeviceToDevice, stream));
     HIPC(hipMemcpyAsync(d.fa.v, sv, nf * sizeof(double), hipMemcpyDeviceToDevice, stream));
     HIPC(hipMemcpyAsync(d.fa.x, sx_, nf * sizeof(double), hipMemcpyDeviceToDevice, stream));
@@ -971,20 +984,24 @@ struct Engine : EngineBase {
           for (int k = 0; k < K; ++k) {
             const int rc = launch_shift(blk_shift, k);
             if (rc) return rc;
-            // views and path lengths ping-pong (P64 views are the positions)
-            std::swap(d.fa.vx, d.fb.vx);
-            std::swap(d.fa.vy, d.fb.vy);
-            std::swap(d.fa.cum, d.fb.cum);
-            if (!Pol::MIXED) {
-              std::swap(d.fa.x, d.fb.x);
-              std::swap(d.fa.y, d.fb.y);
-            }
+            swap_shift_buffers();
           }
         }
         return SPHNJ_OK;
     }
     err = "unknown stage";
     return SPHNJ_E_ARG;
+  }
+
+  // after a shift pass: views and path lengths ping-pong (P64 views are the positions)
+  void swap_shift_buffers() {
+    std::swap(d.fa.vx, d.fb.vx);
+    std::swap(d.fa.vy, d.fb.vy);
+    std::swap(d.fa.cum, d.fb.cum);
+    if (!Pol::MIXED) {
+      std::swap(d.fa.x, d.fb.x);
+      std::swap(d.fa.y, d.fb.y);
+    }
   }
 
   int launches_of(int st) const {

@@ -648,19 +648,19 @@ __device__ __forceinline__ int wave_max_int(int v) {
   return v;
 }
 // Band height of a cell column of ny2 sub-rows for tiles of B lanes (all lanes of a wave call
-// it and get it): from the densest sub-row pair up while every band still holds at most B.
+// it and get it): B over its densest sub-row pair, or the largest of the 8 heights above that
+// whose bands all hold at most B (tested at once, 8 lanes per height).
 __device__ __forceinline__ int band_height(const int *lb0, const int *lb1, int ny2, int B, int lane) {
   int dm = 0;
   for (int r = lane; r < ny2; r += 64) dm = max(dm, band_lanes(lb0, lb1, r, r + 1));
   dm = wave_max_int(dm);
-  int R = max(1, min(kBandMax, B / max(dm, 1)));
-  for (int it = 0; it < 8 && R < kBandMax; ++it) {
-    int m = 0;
-    for (int a = lane * (R + 1); a < ny2; a += 64 * (R + 1)) m = max(m, band_lanes(lb0, lb1, a, min(a + R + 1, ny2)));
-    if (wave_max_int(m) > B) break;
-    ++R;
-  }
-  return R;
+  const int R0 = max(1, min(kBandMax, B / max(dm, 1))), h = R0 + 1 + (lane & 7);
+  int m = h <= kBandMax ? 0 : B + 1;
+  if (h <= kBandMax)
+    for (int a = (lane >> 3) * h; a < ny2; a += 8 * h) m = max(m, band_lanes(lb0, lb1, a, min(a + h, ny2)));
+  for (int o = 8; o < 64; o <<= 1) m = max(m, __shfl_xor(m, o, 64));  // over the lanes of height h
+  const unsigned long long ok = __ballot(lane < 8 && m <= B);
+  return ok ? R0 + 64 - __builtin_clzll(ok) : R0;
 }
 // Tiles of the bands of R sub-rows (all lanes of a wave; lane 0's sum is the count).
 __device__ __forceinline__ int band_tiles(const int *lb0, const int *lb1, int ny2, int R, int B, int lane) {

@@ -6,6 +6,7 @@ import pytest
 
 from conftest import rel_l2
 from helpers import case, fluid, lattice, params, solid, wall
+from sphnj_amd.sim import grid
 from sphnj_amd import Case, Simulation, _abi as A
 
 pytestmark = pytest.mark.gpu
@@ -220,6 +221,40 @@ def test_shift_runs_on_global_tiles(gpu):
     da, db = a.download()["fluid"], b.download()["fluid"]
     for k in ("x", "y", "u", "v", "rho", "p", "boundary_type"):
         assert np.array_equal(da[k], db[k]), k
+
+
+def test_band_tiles_and_per_subcolumn_cut(gpu, oracle_mod):
+    """A cell column holding a dense 10 x 7 cluster inside one sub-row (70 lanes in one sub-row
+    pair, past 64 and below 128) below a sparse line of single particles, beside a normal block:
+    at 64 slots its sub-row bands (the cluster's in two pieces) would need more tiles than its
+    budget, so it is cut per sub-column, while the other columns take band tiles (at 128 and 256
+    the cluster's column takes bands of 9 sub-rows).  Every block size matches the oracle, and
+    P64 is bit-identical across them."""
+    p = params(shift_per_step=2)
+    _, _, cdx, cdy = grid(p)
+    x0, y0 = -0.2 + 7 * cdx, -0.2 + 6 * cdy  # cell column 7, sub-row 12
+    xb, yb = lattice(10, 10, 0.02, 0.1, 0.1)
+    xc, yc = lattice(10, 7, 0.005, x0 + 0.01, y0 + 0.003)
+    ys = np.arange(y0 + 0.1, 0.95, 0.045)
+    xs = np.full_like(ys, x0 + 0.03)
+    x, y = np.r_[xb, xc, xs], np.r_[yb, yc, ys]
+    c = case(p, f=fluid(x, y, u=0.3 * (y - 0.4), v=0.2 * (x - 0.5), p=50.0 * x, rho=1000.0 + 0.3 * y))
+    o = oracle_mod.Oracle(c)
+    o.step(3)
+    ref = o.download()["fluid"]
+    out = {}
+    for prec, tol in (("p64", 1e-9), ("m32", 1e-4)):
+        for blk in (64, 128, 256):
+            s = Simulation(c, prec)
+            s.set_block(0, blk)
+            s.set_block(1, blk)
+            s.step(3)
+            out[prec, blk] = s.download()["fluid"]
+            for k in ("x", "y", "u", "v", "rho", "p"):
+                assert rel_l2(out[prec, blk][k], ref[k]) <= tol, (prec, blk, k)
+    for blk in (128, 256):
+        for k in ("x", "y", "u", "v", "rho", "p", "boundary_type"):
+            assert np.array_equal(out["p64", 64][k], out["p64", blk][k]), (blk, k)
 
 
 def test_solid_sort_follows_cell_changes(gpu, oracle_mod):

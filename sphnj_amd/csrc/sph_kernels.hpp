@@ -769,22 +769,14 @@ __global__ void __launch_bounds__(256) k_wall_rows(Dev<Pol> d, int G, const int 
   lb[cp.ny] = end;
 }
 
-// Row table of every sub-column (one block of NW waves per global sub-column): f_lb[k] for row
+// Row table of sub-column sx of case cp (all threads of a block of NW waves): f_lb[k] for row
 // k = the suffix minimum of the non-empty rows' starts, with the sub-column's end past the last
-// non-empty row.  The blocks of even sub-columns also audit their cell column's cells against
-// the reference's 64-entry cell capacity (include/global.h:9) and build its f_swp (when the
-// batch has solids).  With four waves (a few thousand sub-columns) the waves build the row
-// table together, then waves 1 and 2 audit and index the cell column; with one (many small
-// sub-columns, a batch of cases) the wave does all of it.
-template <class Pol, int NW>
-__global__ void __launch_bounds__(64 * NW) k_subcol_rows(Dev<Pol> d, int G, const int *col_case, int *reset_moved) {
+// non-empty row.
+template <class Pol, int NW, class CP>
+__device__ __forceinline__ void subcol_row_table(const Dev<Pol> &d, const CP &cp, int sx, int *s_min, int *s_end) {
   constexpr int NT = 64 * NW;
-  const int g = blockIdx.x, t = threadIdx.x, w = t >> 6, lane = t & 63;
-  if (g >= G) return;
-  if (reset_moved != nullptr && g == 0 && t == 0) *reset_moved = 0;  // read by the earlier kernels only
-  __shared__ int s_min[NW], s_end[NW];
-  const auto &cp = d.cases[col_case[g]];
-  const int sx = g - cp.col_off, ny2 = 2 * cp.ny;
+  const int t = threadIdx.x, w = t >> 6, lane = t & 63;
+  const int ny2 = 2 * cp.ny;
   const int kb = 4 * cp.cell_off + sx * ny2;
   int *lb = d.f_lb + cp.lb_off + sx * (ny2 + 1);
   // each thread takes a contiguous run of rows [k0, k1), held in registers when it has at most
@@ -835,7 +827,41 @@ __global__ void __launch_bounds__(64 * NW) k_subcol_rows(Dev<Pol> d, int G, cons
     }
   }
   if (t == 0) lb[ny2] = end;
-  if (w == 1 % NW && (sx & 1) == 0) {  // the block of an even sub-column audits its cell column
+  __syncthreads();  // s_min, s_end free again; the table visible to the block
+}
+
+// Row tables of every cell column's two sub-columns (one block of NW waves per cell column,
+// at its even global sub-column g = 2 blockIdx.x), then its tile counts for the njobs block
+// sizes (how it is cut and how many tiles, at g; 0 at g + 1).  The block also audits the cell
+// column's cells against the reference's 64-entry cell capacity (include/global.h:9) and
+// builds its f_swp (when the batch has solids).  With four waves (a few thousand sub-columns)
+// the waves build each row table together, then wave 0 counts the tiles while waves 1 and 2
+// audit and index the cell column; with one (many small sub-columns, a batch of cases) the
+// wave does all of it.
+template <class Pol, int NW>
+__global__ void __launch_bounds__(64 * NW) k_subcol_rows(Dev<Pol> d, int G, const int *col_case, PlanJobs jobs,
+                                                         int njobs, int *reset_moved) {
+  const int g = 2 * (int)blockIdx.x, t = threadIdx.x, w = t >> 6, lane = t & 63;
+  if (g >= G) return;
+  if (reset_moved != nullptr && g == 0 && t == 0) *reset_moved = 0;  // read by the earlier kernels only
+  __shared__ int s_min[NW], s_end[NW];
+  const auto &cp = d.cases[col_case[g]];
+  const int sx = g - cp.col_off, ny2 = 2 * cp.ny;
+  subcol_row_table<Pol, NW>(d, cp, sx, s_min, s_end);
+  subcol_row_table<Pol, NW>(d, cp, sx + 1, s_min, s_end);
+  if (w == 0) {  // the cell column's tiles
+    const int *lb0 = d.f_lb + cp.lb_off + sx * (ny2 + 1), *lb1 = lb0 + (ny2 + 1);
+    for (int j = 0; j < njobs; ++j) {
+      int R;
+      const int nt = cell_column_tiles(d, lb0, lb1, ny2, jobs.j[j].B, lane, &R);
+      if (lane == 0) {
+        jobs.j[j].cnt[g] = nt;
+        jobs.j[j].band[g] = R;
+        jobs.j[j].cnt[g + 1] = jobs.j[j].band[g + 1] = 0;
+      }
+    }
+  }
+  if (w == 1 % NW) {  // audits the cell column
     const int qc = cp.cell_off + (sx >> 1) * cp.ny, kc = 4 * cp.cell_off + sx * ny2;
     bool over = false;
     for (int k = lane; k < cp.ny; k += 64) {
@@ -848,7 +874,7 @@ __global__ void __launch_bounds__(64 * NW) k_subcol_rows(Dev<Pol> d, int G, cons
     }
     if (__ballot(over ? 1 : 0) != 0ull && lane == 0) atomicOr(d.status, kStCellOver64);
   }
-  if (w == 2 % NW && (sx & 1) == 0 && d.ns > 0) {  // ... and builds its column's f_swp
+  if (w == 2 % NW && d.ns > 0) {  // ... and builds its f_swp
     constexpr int kU = 8;
     const int qc = cp.cell_off + (sx >> 1) * cp.ny;
     int *swp = d.f_swp + cp.swp_off + (sx >> 1) * (cp.ny + 1);

@@ -749,14 +749,16 @@ struct Engine : EngineBase {
       k_cell_bounds<<<nblk(ns, 256), 256, 0, stream>>>(s_sorted_keys, ns, d.s_range, moved);
     }
     plan_ok[0] = plan_ok[1] = plan_ok[2] = false;
-    if (nf > 0) {  // row tables and the cell audit, then the tiles of both block sizes
+    if (nf > 0) {  // row tables, the cell audit and the tile counts of both block sizes
+      PlanJobs jobs{};
+      const int nj = plan_jobs(blk_pair, blk_shift, jobs);
       if (G <= kPlanPrefixMax)
-        k_subcol_rows<Pol, 4><<<G, 256, 0, stream>>>(d, G, col_case, moved);
+        k_subcol_rows<Pol, 4><<<G / 2, 256, 0, stream>>>(d, G, col_case, jobs, nj, moved);
       else
-        k_subcol_rows<Pol, 1><<<G, 64, 0, stream>>>(d, G, col_case, moved);
+        k_subcol_rows<Pol, 1><<<G / 2, 64, 0, stream>>>(d, G, col_case, jobs, nj, moved);
       LAUNCH_CHECK();
       int rc;
-      if ((rc = ensure_plans(blk_pair, blk_shift))) return rc;
+      if ((rc = ensure_plans(blk_pair, blk_shift, true))) return rc;
     } else {
       k_cell_audit<Pol><<<nblk(nc, 256), 256, 0, stream>>>(d, d.status);
       LAUNCH_CHECK();
@@ -777,12 +779,13 @@ struct Engine : EngineBase {
     }
     return nj;
   }
-  int ensure_plans(int B1, int B2 = 0) {
+  // counted: k_subcol_rows has just written these jobs' tile counts
+  int ensure_plans(int B1, int B2 = 0, bool counted = false) {
     if (d.nf == 0) return SPHNJ_OK;
     PlanJobs jobs{};
     const int nj = plan_jobs(B1, B2, jobs);
     if (nj == 0) return SPHNJ_OK;
-    k_tile_count<Pol><<<dim3(nblk(G, 4), nj), 256, 0, stream>>>(d, G, col_case, jobs);
+    if (!counted) k_tile_count<Pol><<<dim3(nblk(G, 4), nj), 256, 0, stream>>>(d, G, col_case, jobs);
     const int scanned = G > kPlanPrefixMax;
     if (scanned) k_tile_scan<<<nj, 1024, 0, stream>>>(jobs, G);
     k_tile_plan<Pol><<<dim3(nblk(G, 4), nj), 256, 0, stream>>>(d, G, col_case, jobs, scanned);

@@ -166,16 +166,36 @@ __device__ __forceinline__ Mat2<R> mmul(const Mat2<R> &x, const Mat2<R> &y) {
 // permute, cell ranges
 // ======================================================================================
 
+// The step size of each case for the step that begins (the first thing of its cell stage),
+// and the simulated time at its end: dt, or with adaptive_cfl C > 0
+// dt_n = min(dt, C h / (c_max + U), S C h / c_s), U = sqrt(max |u|^2 after the last update).
+// Threads t, t + n, ... of one block.
+template <class Pol>
+__device__ __forceinline__ void step_dt_body(const Dev<Pol> &d, int t, int n) {
+  for (int c = t; c < d.ncase; c += n) {
+    const auto &cp = d.cases[c];
+    double dt = cp.dt;
+    if (cp.adaptive) {
+      dt = fmin(dt, cp.cfl_h / (cp.c_max + sqrt(d.u2_last[c])));
+      if (cp.dt_solid_cap > 0.0) dt = fmin(dt, cp.dt_solid_cap);
+    }
+    d.dtv[c] = dt;
+    d.tsim[c] += dt;
+  }
+}
+
 // findPositionCell + fillFluidCellID (src/pair.cu:8-27) on the half-cell grid, and the count
 // of a counting sort (replacing thrust::sort_by_key, src/sort.cu:10-20): every fluid slot's
 // sub-cell key, (sy + sx * 2 ny) after the case's 4 ncell sub-cells offset, and its rank among
 // its key's slots.  Slots sorted last step hold equal keys in runs, so each run of equal keys
 // among a wave's consecutive lanes takes one atomic; the permute later fixes the order
 // inside each key, which the atomics leave open.  Every lane of the block must reach the
-// ballots.
+// ballots.  With new_step, block 0 also sets the step size (step_dt_body), which no other
+// block of this launch reads.
 template <class Pol>
-__global__ void __launch_bounds__(256) k_hash_fluid(Dev<Pol> d, const int *solids_moved) {
+__global__ void __launch_bounds__(256) k_hash_fluid(Dev<Pol> d, const int *solids_moved, int new_step) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63;
+  if (new_step && blockIdx.x == 0) step_dt_body(d, (int)threadIdx.x, (int)blockDim.x);
   // the solids' cell ranges start empty when the solids are sorted again (their bounds write the
   // rest); otherwise they stay as they are
   if (d.ns > 0 && (solids_moved == nullptr || *solids_moved != 0))
@@ -2834,21 +2854,10 @@ __global__ void __launch_bounds__(256) k_energy_partial(Dev<Pol> d, int flo, int
   if (t < 5) part[blockIdx.x * 5 + t] = sm[t][0];
 }
 
-// The step size of each case for the step that begins (the first thing of its cell stage),
-// and the simulated time at its end: dt, or with adaptive_cfl C > 0
-// dt_n = min(dt, C h / (c_max + U), S C h / c_s), U = sqrt(max |u|^2 after the last update).
+// step_dt_body as a launch of its own (a step without fluid).
 template <class Pol>
 __global__ void __launch_bounds__(64) k_step_dt(Dev<Pol> d) {
-  for (int c = threadIdx.x; c < d.ncase; c += 64) {
-    const auto &cp = d.cases[c];
-    double dt = cp.dt;
-    if (cp.adaptive) {
-      dt = fmin(dt, cp.cfl_h / (cp.c_max + sqrt(d.u2_last[c])));
-      if (cp.dt_solid_cap > 0.0) dt = fmin(dt, cp.dt_solid_cap);
-    }
-    d.dtv[c] = dt;
-    d.tsim[c] += dt;
-  }
+  step_dt_body(d, (int)threadIdx.x, 64);
 }
 
 // ---- probes of the reference's global outputs (src/output.cu:283-582) ----------------------

@@ -696,7 +696,7 @@ struct Engine : EngineBase {
       }
     }
     // hash and order the fluid and solid once, so every stage finds valid cells
-    if ((rc = stage_cells())) return rc;
+    if ((rc = stage_cells(false))) return rc;
     HIPC(hipStreamSynchronize(stream));
     return SPHNJ_OK;
   }
@@ -706,7 +706,9 @@ struct Engine : EngineBase {
     std::swap(d.fa, d.fb);
   }
 
-  int stage_cells() {
+  // new_step: a step begins (its size is set here: in the fluid hash, or on its own without
+  // fluid)
+  int stage_cells(bool new_step) {
     const int nf = d.nf, ns = d.ns, nc = d.total_cells;
     // every species' cell ranges start empty (the permute writes the fluid's, k_cell_bounds the
     // solids')
@@ -723,12 +725,13 @@ struct Engine : EngineBase {
       LAUNCH_CHECK();
     }
     if (nf == 0) {
+      if (new_step) k_step_dt<Pol><<<1, 64, 0, stream>>>(d);
       k_clear_ranges<<<nblk(4 * nc, 256), 256, 0, stream>>>(d.s_range, nc, d.f_srange, 4 * nc);
       LAUNCH_CHECK();
     }
     if (nf > 0) {  // counting sort on the sub-cell keys, then the permute
       const int nk = 4 * nc, nkb = nblk(nk, kScanTile);
-      k_hash_fluid<Pol><<<nblk(nf, 256), 256, 0, stream>>>(d, moved);
+      k_hash_fluid<Pol><<<nblk(nf, 256), 256, 0, stream>>>(d, moved, new_step ? 1 : 0);
       k_count_tiles<<<nkb, kSortThreads, 0, stream>>>(d.f_cnt, nk, cnt_tsum);
       if (nkb > kApplyPrefixMax) {
         k_exscan_small<<<1, 1024, 0, stream>>>(cnt_tsum, nkb, cnt_toff);
@@ -928,9 +931,7 @@ struct Engine : EngineBase {
     const int nf = d.nf, ns = d.ns, nw = d.nw;
     switch (st) {
       case SPHNJ_STAGE_CELLS:  // a step begins: its size first
-        k_step_dt<Pol><<<1, 64, 0, stream>>>(d);
-        LAUNCH_CHECK();
-        return stage_cells();
+        return stage_cells(true);
       case SPHNJ_STAGE_COUPLE:
         if (nf > 0 && ns > 0) {
           k_solid_couple<Pol><<<nblk(ns * kSolidLanes, 256), 256, 0, stream>>>(d);

@@ -99,12 +99,11 @@ struct Dev {
   FluidBuf<Pol> fa, fb;  // cur (a) / alt (b)
   int8_t *f_phase, *f_phase2, *f_cls, *f_cls2;  // cur / alt (swap in the permute only)
   int *f_orig;           // slot -> global original index (sorted values)
-  int *f_key;            // slot -> global sub-cell key (sorted keys)
   int *f_cxy;            // slot -> (sx << 16) | sy, sub-cell coordinates (cx = sx >> 1)
   // counting sort on the sub-cell keys: per key (4 total_cells + 1) the slots hashed there this
   // step (zero between steps); per slot of the order being replaced its key and its rank among
   // its key's slots in atomic order; per slot in atomic order the old slot, original index and
-  // key; per sorted slot the old slot and original index
+  // key (atomic order only permutes the slots inside a key, so f_pkey holds the sorted keys)
   int *f_cnt;
   int *f_hkey, *f_hrank;
   int *f_perm, *f_porig, *f_pkey;

@@ -189,39 +189,63 @@ __device__ __forceinline__ void step_dt_body(const Dev<Pol> &d, int t, int n) {
 // sub-cell key, (sy + sx * 2 ny) after the case's 4 ncell sub-cells offset, and its rank among
 // its key's slots.  Slots sorted last step hold equal keys in runs, so each run of equal keys
 // among a wave's consecutive lanes takes one atomic; the permute later fixes the order
-// inside each key, which the atomics leave open.  Every lane of the block must reach the
-// ballots.  With new_step, block 0 also sets the step size (step_dt_body), which no other
-// block of this launch reads.
+// inside each key, which the atomics leave open.  A wave takes kCellRounds rounds of 64
+// consecutive slots: the loads of all rounds, then the atomics of all rounds, then the
+// stores, so that it waits on memory once rather than once per round.  Every lane of the
+// block must reach the ballots.  With new_step, block 0 also sets the step size
+// (step_dt_body), which no other block of this launch reads.
+constexpr int kCellRounds = 4;
 template <class Pol>
 __global__ void __launch_bounds__(256) k_hash_fluid(Dev<Pol> d, const int *solids_moved, int new_step) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x, lane = threadIdx.x & 63;
+  constexpr int U = kCellRounds;
+  const int lane = threadIdx.x & 63;
+  const int s0 = (blockIdx.x * 4 + (int)(threadIdx.x >> 6)) * (64 * U) + lane;  // round u: s0 + 64 u
   if (new_step && blockIdx.x == 0) step_dt_body(d, (int)threadIdx.x, (int)blockDim.x);
   // the solids' cell ranges start empty when the solids are sorted again (their bounds write the
   // rest); otherwise they stay as they are
   if (d.ns > 0 && (solids_moved == nullptr || *solids_moved != 0))
-    for (int q = s; q < d.total_cells; q += gridDim.x * blockDim.x) d.s_range[q] = make_int2(0, 0);
-  const bool valid = s < d.nf;
-  int key = -1 - lane;  // distinct for the lanes past the end
-  if (valid) {
-    const int c = case_of(s, d.f_case_off, d.f_case_blk, d.ncase);
-    const auto &cp = d.cases[c];
-    unsigned fl = 0;
-    const int sx = axis_sub(d.fa.x[s], cp.cxmin, cp.inv_cdx, cp.nx, &fl);
-    const int sy = axis_sub(d.fa.y[s], cp.cymin, cp.inv_cdy, cp.ny, &fl);
-    key = 4 * cp.cell_off + sy + sx * (2 * cp.ny);
-    if (fl) atomicOr(d.status, kStOutOfGrid);
+    for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < d.total_cells; q += gridDim.x * blockDim.x)
+      d.s_range[q] = make_int2(0, 0);
+  double px[U], py[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int s = min(s0 + 64 * u, d.nf - 1);
+    px[u] = d.fa.x[s];
+    py[u] = d.fa.y[s];
   }
+  int key[U];
+  unsigned fl = 0;
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int s = s0 + 64 * u;
+    key[u] = -1 - lane;  // distinct for the lanes past the end
+    if (s < d.nf) {
+      const auto &cp = d.cases[case_of(s, d.f_case_off, d.f_case_blk, d.ncase)];
+      const int sx = axis_sub(px[u], cp.cxmin, cp.inv_cdx, cp.nx, &fl);
+      const int sy = axis_sub(py[u], cp.cymin, cp.inv_cdy, cp.ny, &fl);
+      key[u] = 4 * cp.cell_off + sy + sx * (2 * cp.ny);
+    }
+  }
+  if (fl) atomicOr(d.status, kStOutOfGrid);
   // runs of equal keys: a run starts where the key differs from the previous lane's
-  const int prev = __shfl_up(key, 1, 64);
-  const unsigned long long heads = __ballot((lane == 0 || prev != key) ? 1 : 0);
-  const unsigned long long above = lane == 63 ? 0ull : heads >> (lane + 1);
-  const int head = 63 - __builtin_clzll(heads & ((2ull << lane) - 1ull));  // this lane's run start
-  int base = 0;
-  if (valid && head == lane) base = atomicAdd(&d.f_cnt[key], above ? __builtin_ctzll(above) + 1 : 64 - lane);
-  base = __shfl(base, head, 64);
-  if (valid) {
-    d.f_hkey[s] = key;
-    d.f_hrank[s] = base + lane - head;
+  int head[U], base[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int prev = __shfl_up(key[u], 1, 64);
+    const unsigned long long heads = __ballot((lane == 0 || prev != key[u]) ? 1 : 0);
+    const unsigned long long above = lane == 63 ? 0ull : heads >> (lane + 1);
+    head[u] = 63 - __builtin_clzll(heads & ((2ull << lane) - 1ull));  // this lane's run start
+    base[u] = 0;
+    if (s0 + 64 * u < d.nf && head[u] == lane)
+      base[u] = atomicAdd(&d.f_cnt[key[u]], above ? __builtin_ctzll(above) + 1 : 64 - lane);
+  }
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int s = s0 + 64 * u, b = __shfl(base[u], head[u], 64);
+    if (s < d.nf) {
+      d.f_hkey[s] = key[u];
+      d.f_hrank[s] = b + lane - head[u];
+    }
   }
 }
 
@@ -296,15 +320,32 @@ __global__ void __launch_bounds__(kSortThreads) k_count_apply(int *cnt, int nk, 
 }
 
 // Counting sort, step 4: every slot to its place in atomic order, with its original index
-// and key.
+// and key (so f_pkey holds the sorted keys).  kCellRounds rounds of 256 slots per block,
+// the loads of all rounds first.
 template <class Pol>
 __global__ void __launch_bounds__(256) k_count_scatter(Dev<Pol> d) {
-  const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s >= d.nf) return;
-  const int k = d.f_hkey[s], p = d.f_srange[k].x + d.f_hrank[s];
-  d.f_perm[p] = s;
-  d.f_porig[p] = d.f_orig[s];
-  d.f_pkey[p] = k;
+  constexpr int U = kCellRounds;
+  const int s0 = blockIdx.x * (256 * U) + (int)threadIdx.x;  // round u: s0 + 256 u
+  int k[U], r[U], o[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int s = min(s0 + 256 * u, d.nf - 1);
+    k[u] = d.f_hkey[s];
+    r[u] = d.f_hrank[s];
+    o[u] = d.f_orig[s];
+  }
+  int p[U];
+#pragma unroll
+  for (int u = 0; u < U; ++u) p[u] = d.f_srange[k[u]].x + r[u];
+#pragma unroll
+  for (int u = 0; u < U; ++u) {
+    const int s = s0 + 256 * u;
+    if (s < d.nf) {
+      d.f_perm[p[u]] = s;
+      d.f_porig[p[u]] = o[u];
+      d.f_pkey[p[u]] = k[u];
+    }
+  }
 }
 
 // fillSolidCellID (src/pair.cu:29-36); solids stay in original order, the key array is the
@@ -367,21 +408,23 @@ __global__ void __launch_bounds__(256) k_permute_fluid(Dev<Pol> d) {
   // its key's start plus the number of the key's slots with a smaller original index -- so the
   // order is (sub-column, sub-row, original index) whatever order the atomics took
   const int k = d.f_pkey[i], o = d.f_porig[i], s = d.f_perm[i];
+  // the particle's state is loaded before the rank loop, which does not need it
+  const double x = d.fa.x[s], y = d.fa.y[s], u = d.fa.u[s], v = d.fa.v[s], rho = d.fa.rho[s];
+  const auto p = d.fa.p[s];
+  const int8_t phase = d.f_phase[s], cls = d.f_cls[s];
   const int2 r = d.f_srange[k];
   int rank = 0;
-  for (int u = r.x; u < r.y; ++u) rank += d.f_porig[u] < o ? 1 : 0;
+  for_run<4>(r.x, r.y, [&](int q) { return d.f_porig[q]; }, [&](int, int po) { rank += po < o ? 1 : 0; });
   const int t = r.x + rank;
-  d.f_key[t] = k;
-  const double x = d.fa.x[s], y = d.fa.y[s], u = d.fa.u[s], v = d.fa.v[s], rho = d.fa.rho[s];
   d.fb.x[t] = x;
   d.fb.y[t] = y;
   d.fb.u[t] = u;
   d.fb.v[t] = v;
   d.fb.rho[t] = rho;
-  d.fb.p[t] = d.fa.p[s];
+  d.fb.p[t] = p;
   d.fb.cum[t] = 0.f;
-  d.f_phase2[t] = d.f_phase[s];
-  d.f_cls2[t] = d.f_cls[s];
+  d.f_phase2[t] = phase;
+  d.f_cls2[t] = cls;
   d.f_orig[t] = o;
   const int c = case_of(t, d.f_case_off, d.f_case_blk, d.ncase);
   const auto &cp = d.cases[c];

@@ -442,7 +442,6 @@ struct Engine : EngineBase {
     if ((rc = dalloc(&d.f_phase, nf)) || (rc = dalloc(&d.f_phase2, nf)) || (rc = dalloc(&d.f_cls, nf)) ||
         (rc = dalloc(&d.f_cls2, nf)) || (rc = dalloc(&d.f_orig, nf)) || (rc = dalloc(&d.f_cxy, nf)) ||
         (rc = dalloc(&d.f_hkey, nf)) || (rc = dalloc(&d.f_hrank, nf)) || (rc = dalloc(&d.f_perm, nf)) ||
-        (rc = dalloc(&d.f_key, nf)) ||
         (rc = dalloc(&d.f_cnt, 4 * (size_t)nc + 1)) || (rc = dalloc(&d.f_porig, nf)) || (rc = dalloc(&d.f_pkey, nf)) ||
         (rc = dalloc(&cnt_tsum, (size_t)nblk(4 * nc, kScanTile) + 1)) ||
         (rc = dalloc(&cnt_toff, (size_t)nblk(4 * nc, kScanTile) + 1)) ||
@@ -731,7 +730,7 @@ struct Engine : EngineBase {
     }
     if (nf > 0) {  // counting sort on the sub-cell keys, then the permute
       const int nk = 4 * nc, nkb = nblk(nk, kScanTile);
-      k_hash_fluid<Pol><<<nblk(nf, 256), 256, 0, stream>>>(d, moved, new_step ? 1 : 0);
+      k_hash_fluid<Pol><<<nblk(nf, 256 * kCellRounds), 256, 0, stream>>>(d, moved, new_step ? 1 : 0);
       k_count_tiles<<<nkb, kSortThreads, 0, stream>>>(d.f_cnt, nk, cnt_tsum);
       if (nkb > kApplyPrefixMax) {
         k_exscan_small<<<1, 1024, 0, stream>>>(cnt_tsum, nkb, cnt_toff);
@@ -739,7 +738,7 @@ struct Engine : EngineBase {
       } else {
         k_count_apply<<<nkb, kSortThreads, 0, stream>>>(d.f_cnt, nk, cnt_tsum, nullptr, d.f_srange);
       }
-      k_count_scatter<Pol><<<nblk(nf, 256), 256, 0, stream>>>(d);
+      k_count_scatter<Pol><<<nblk(nf, 256 * kCellRounds), 256, 0, stream>>>(d);
       LAUNCH_CHECK();
       k_permute_fluid<Pol><<<nblk(std::max(nf, ncase * kSpread), blk_stream), blk_stream, 0, stream>>>(d);
       LAUNCH_CHECK();
@@ -1265,7 +1264,7 @@ struct Engine : EngineBase {
       n = f_off[c + 1] - lo;
       k.resize(n);
       o.resize(n);
-      if (n && (rc = d2h(k.data(), d.f_key + lo, n))) return rc;
+      if (n && (rc = d2h(k.data(), d.f_pkey + lo, n))) return rc;  // the scatter's keys are the sorted keys
       if ((rc = d2h(o.data(), d.f_orig + lo, n)) || (rc = d2h(rg.data(), d.f_srange + 4 * (size_t)cp.cell_off, nb)))
         return rc;
     } else if (species == SPHNJ_SOLID) {

@@ -1521,6 +1521,22 @@ __global__ void __launch_bounds__(BLOCK, (Pol::MIXED ? 4 : 1))
   const bool valid = t < P.count;
   const int i = valid ? lane_slot(P, t) : 0, lsx = lane_subcol(P, t);  // the lane's slot, sub-column
   const auto &A = d.fa;
+  // the lane's own state, loaded ahead of the staging so that the loads overlap it
+  int pk = 0, phi = 0;
+  R xi = R(0), yi = R(0), ui = R(0), vi = R(0), rhoi = R(1), pi = R(0);
+  double x0 = 0.0, y0 = 0.0;
+  if (valid) {
+    pk = d.f_cxy[i];
+    xi = A.vx[i];
+    yi = A.vy[i];
+    ui = A.vu[i];
+    vi = A.vv[i];
+    rhoi = A.vrho[i];
+    pi = A.p[i];
+    x0 = A.x[i];
+    y0 = A.y[i];
+    phi = d.f_phase[i];
+  }
   const bool staged = P.total <= CAP && P.nr <= TileRows<BLOCK>::value;
   int dmax = 0;  // bit pattern of the largest path length (non-negative floats order as ints)
   if (staged) {
@@ -1591,15 +1607,12 @@ __global__ void __launch_bounds__(BLOCK, (Pol::MIXED ? 4 : 1))
   unsigned long long my_u2 = 0ull;
   if (valid) {
     const auto &cp = d.cases[c];
-    const int pk = d.f_cxy[i], sy = cxy_y(pk), cx = P.cx, cy = sy >> 1;
-    const R xi = A.vx[i], yi = A.vy[i], ui = A.vu[i], vi = A.vv[i], rhoi = A.vrho[i], pi = A.p[i];
-    const double x0 = A.x[i], y0 = A.y[i];
+    const int sy = cxy_y(pk), cx = P.cx, cy = sy >> 1;
     LanePos lp;
     if constexpr (Pol::MIXED)
       lp = lane_pos_view(xi, yi, cx, cy, lsx, sy, cp);
     else
       lp = lane_pos(x0, y0, lsx, sy, cp);
-    const int phi = d.f_phase[i];
     const bool liq = phi == 0;
     const R cs = cp.c[phi], csinv = cp.cinv[phi], mu8 = cp.mu8[phi];
     const R rhoi_inv = krcp<R>(rhoi);

@@ -1522,9 +1522,10 @@ __global__ void __launch_bounds__(BLOCK, (Pol::MIXED ? 4 : 1))
   const int i = valid ? lane_slot(P, t) : 0, lsx = lane_subcol(P, t);  // the lane's slot, sub-column
   const auto &A = d.fa;
   // the lane's own state, loaded ahead of the staging so that the loads overlap it
-  int pk = 0, phi = 0;
+  int pk = 0, phi = 0, cls0 = 0;
   R xi = R(0), yi = R(0), ui = R(0), vi = R(0), rhoi = R(1), pi = R(0);
-  double x0 = 0.0, y0 = 0.0;
+  double x0 = 0.0, y0 = 0.0, u0 = 0.0, v0 = 0.0, r0 = 0.0;
+  float cum0 = 0.f;
   if (valid) {
     pk = d.f_cxy[i];
     xi = A.vx[i];
@@ -1535,7 +1536,12 @@ __global__ void __launch_bounds__(BLOCK, (Pol::MIXED ? 4 : 1))
     pi = A.p[i];
     x0 = A.x[i];
     y0 = A.y[i];
+    u0 = A.u[i];
+    v0 = A.v[i];
+    r0 = A.rho[i];
     phi = d.f_phase[i];
+    cls0 = d.f_cls[i];
+    cum0 = A.cum[i];
   }
   const bool staged = P.total <= CAP && P.nr <= TileRows<BLOCK>::value;
   int dmax = 0;  // bit pattern of the largest path length (non-negative floats order as ints)
@@ -1801,7 +1807,7 @@ __global__ void __launch_bounds__(BLOCK, (Pol::MIXED ? 4 : 1))
     d.f_near[i] = (int8_t)near;
     // the fluid class of this step (correctionUpdate, src/fluid.cu:369-427), a function of
     // phase, colour and near flag only; it is set only where the case shifts
-    int cls = d.f_cls[i];
+    int cls = cls0;
     if (cp.K > 0) {
       if (phi == 1)
         cls = 3;  // gas: INNER
@@ -1816,7 +1822,6 @@ __global__ void __launch_bounds__(BLOCK, (Pol::MIXED ? 4 : 1))
     d.f_L[4 * i + 2] = S.L10;
     d.f_L[4 * i + 3] = S.L11;
     // fluidUpdate, integrated in double
-    const double u0 = A.u[i], v0 = A.v[i], r0 = A.rho[i];
     double ax = (double)S.ax, ay = (double)S.ay;
     if (cp.steady) {
       ax += -0.1 * u0;
@@ -1849,7 +1854,7 @@ __global__ void __launch_bounds__(BLOCK, (Pol::MIXED ? 4 : 1))
     const double u2 = u * u + v * v;
     my_u2 = (unsigned long long)__double_as_longlong(u2);
     const double mx = x - x0, my = y - y0;
-    B.cum[i] = (float)((double)A.cum[i] + sqrt(mx * mx + my * my));
+    B.cum[i] = (float)((double)cum0 + sqrt(mx * mx + my * my));
     if (isnan(rho) || isnan(x) || isnan(y) || isnan(u2)) atomicOr(d.status, kStNaN);
   }
   // per-case max |u|^2 (non-negative doubles order like their bit patterns)

@@ -902,9 +902,9 @@ __device__ __forceinline__ void subcol_row_table(const Dev<Pol> &d, const CP &cp
 // audit and index the cell column; with one (many small sub-columns, a batch of cases) the
 // wave does all of it.
 template <class Pol, int NW>
-__global__ void __launch_bounds__(64 * NW) k_subcol_rows(Dev<Pol> d, int G, const int *col_case, PlanJobs jobs,
-                                                         int njobs, int *reset_moved) {
-  const int g = 2 * (int)blockIdx.x, t = threadIdx.x, w = t >> 6, lane = t & 63;
+__device__ __forceinline__ void subcol_rows_block(const Dev<Pol> &d, int G, const int *col_case, const PlanJobs &jobs,
+                                                  int njobs, int *reset_moved, int bid) {
+  const int g = 2 * bid, t = threadIdx.x, w = t >> 6, lane = t & 63;
   if (g >= G) return;
   if (reset_moved != nullptr && g == 0 && t == 0) *reset_moved = 0;  // read by the earlier kernels only
   __shared__ int s_min[NW], s_end[NW];
@@ -964,6 +964,11 @@ __global__ void __launch_bounds__(64 * NW) k_subcol_rows(Dev<Pol> d, int G, cons
     }
     if (lane == 0) swp[0] = 0;
   }
+}
+template <class Pol, int NW>
+__global__ void __launch_bounds__(64 * NW) k_subcol_rows(Dev<Pol> d, int G, const int *col_case, PlanJobs jobs,
+                                                         int njobs, int *reset_moved) {
+  subcol_rows_block<Pol, NW>(d, G, col_case, jobs, njobs, reset_moved, (int)blockIdx.x);
 }
 
 // Tiles per cell column, counted at its even global sub-column (0 at the odd one), and how it
@@ -2376,6 +2381,22 @@ __global__ void __launch_bounds__(256) k_couple_wall(Dev<Pol> d, int wall_blocks
     wall_update_block<Pol, kWallLanes>(d, (int)blockIdx.x);
   else
     solid_couple_block<Pol>(d, (int)blockIdx.x - wall_blocks);
+}
+// The cell columns' row tables (k_subcol_rows, 4 waves a cell column), the wall update and the
+// solid coupling in one launch at the end of a step's cell stage: the last two read the sorted
+// fluid and its sub-cell ranges, which the row tables leave as they are, and nothing they write
+// is read by the tables or the tile counts.  Blocks below row_blocks build the tables, the next
+// wall_blocks update walls, the rest (if any) couple solids.
+template <class Pol, int kWallLanes>
+__global__ void __launch_bounds__(256) k_rows_wall(Dev<Pol> d, int G, const int *col_case, PlanJobs jobs, int njobs,
+                                                   int *reset_moved, int row_blocks, int wall_blocks) {
+  const int b = (int)blockIdx.x;
+  if (b < row_blocks)
+    subcol_rows_block<Pol, 4>(d, G, col_case, jobs, njobs, reset_moved, b);
+  else if (b < row_blocks + wall_blocks)
+    wall_update_block<Pol, kWallLanes>(d, b - row_blocks);
+  else
+    solid_couple_block<Pol>(d, b - row_blocks - wall_blocks);
 }
 
 // plasticElasticFunction (src/solid.cu:80-108)

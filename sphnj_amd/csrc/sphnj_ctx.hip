@@ -104,6 +104,9 @@ struct EngineBase {
   bool persistent_solid = false;  // S solid sub-steps as one launch (k_solid_substeps)
   int shift_runs = 1;             // shift passes after the first walk its recorded runs
   int blk_pair = 256, blk_shift = 256, blk_stream = 256;
+  // set by enqueue_step around its cell stage: the row-table launch also runs the wall update
+  // and the solid coupling (k_rows_wall)
+  bool rows_wall = false;
   // per-stage timing: HIP events recorded on the context stream around each stage while on
   bool timing = false;
   struct StageRec {
@@ -754,7 +757,9 @@ struct Engine : EngineBase {
     if (nf > 0) {  // row tables, the cell audit and the tile counts of both block sizes
       PlanJobs jobs{};
       const int nj = plan_jobs(blk_pair, blk_shift, jobs);
-      if (G <= kPlanPrefixMax)
+      if (G <= kPlanPrefixMax && rows_wall)
+        launch_rows_wall(jobs, nj, moved);
+      else if (G <= kPlanPrefixMax)
         k_subcol_rows<Pol, 4><<<G / 2, 256, 0, stream>>>(d, G, col_case, jobs, nj, moved);
       else
         k_subcol_rows<Pol, 1><<<G / 2, 64, 0, stream>>>(d, G, col_case, jobs, nj, moved);
@@ -1024,6 +1029,22 @@ struct Engine : EngineBase {
     else
       k_wall_update<Pol, WL><<<wb, 256, 0, stream>>>(d);
   }
+  // The row tables with the wall update and the solid coupling (k_rows_wall).
+  template <int WL>
+  void launch_rows_wall_at(const PlanJobs &jobs, int nj, int *moved) {
+    const int wb = d.nw > 0 ? nblk(d.nw * WL, 256) : 0, cb = d.ns > 0 ? nblk(d.ns * kSolidLanes, 256) : 0;
+    k_rows_wall<Pol, WL><<<G / 2 + wb + cb, 256, 0, stream>>>(d, G, col_case, jobs, nj, moved, G / 2, wb);
+  }
+  void launch_rows_wall(const PlanJobs &jobs, int nj, int *moved) {
+    if (d.nw <= (1 << 16))
+      launch_rows_wall_at<8>(jobs, nj, moved);
+    else if (d.nw <= (1 << 17))
+      launch_rows_wall_at<4>(jobs, nj, moved);
+    else if (d.nw <= (1 << 18))
+      launch_rows_wall_at<2>(jobs, nj, moved);
+    else
+      launch_rows_wall_at<1>(jobs, nj, moved);
+  }
   void launch_wall(bool with_couple) {
     if (d.nw <= (1 << 16))
       launch_wall_at<8>(with_couple);
@@ -1036,9 +1057,13 @@ struct Engine : EngineBase {
   }
 
   int enqueue_step() {
-    // untimed steps run the coupling and the wall update as one launch (both read the fluid only)
-    const bool fuse = !timing && d.nf > 0 && d.ns > 0 && d.nw > 0;
+    // untimed steps run the wall update and the solid coupling in the cell stage's launch of the
+    // row tables where that launch gives a cell column 4 waves, else the coupling and the wall
+    // update as one launch (both read the fluid only)
+    const bool rw = !timing && d.nf > 0 && (d.nw > 0 || d.ns > 0) && G <= kPlanPrefixMax;
+    const bool fuse = !rw && !timing && d.nf > 0 && d.ns > 0 && d.nw > 0;
     for (int st = SPHNJ_STAGE_CELLS; st <= SPHNJ_STAGE_SHIFT; ++st) {
+      if (rw && (st == SPHNJ_STAGE_COUPLE || st == SPHNJ_STAGE_WALL)) continue;
       if (fuse && st == SPHNJ_STAGE_COUPLE) {
         launch_wall(true);
         LAUNCH_CHECK();
@@ -1055,7 +1080,9 @@ struct Engine : EngineBase {
         }
         HIPC(hipEventRecord(a, stream));
       }
+      rows_wall = rw && st == SPHNJ_STAGE_CELLS;
       const int rc = stage(st);
+      rows_wall = false;
       if (rc) return rc;
       if (timing) {
         HIPC(hipEventRecord(b, stream));

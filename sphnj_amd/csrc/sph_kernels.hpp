@@ -1723,6 +1723,7 @@ __global__ void __launch_bounds__(BLOCK, (Pol::MIXED ? 4 : 1))
           fluid_j(Inside{}, q.l == own, dx, dy, dx * dx + dy * dy, q.v.x, q.v.y, q.r.x, q.r.y, q.fl, true);
         };
         Cand cur = next();
+#pragma unroll 2
         while (m) {
           const Cand nxt = next();
           use(cur);

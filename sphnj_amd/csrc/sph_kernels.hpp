@@ -1693,12 +1693,27 @@ __global__ void __launch_bounds__(BLOCK, (Pol::MIXED ? 4 : 1))
           for_range_masked<2>(s0, s1, [&](int l, bool in) { staged_j(Cut{}, l, in); });
           return 0u;
         }
-        unsigned m = 0u;
-        for_range_4_2_1(s0, s1, [&](int l, bool) {
-          const R2 pj = s_pos[l];
+        // 4 candidates a round (then 2, then 1), their bits gathered in a nibble placed at
+        // once, so that the bit positions cost no instruction per candidate
+        auto in = [&](const R2 &pj) {
           const R dx = pj.x - xi, dy = pj.y - yg;
-          m |= (dx * dx + dy * dy < K.rc2 ? 1u : 0u) << ((l - s0) & 31);
-        });
+          return dx * dx + dy * dy < K.rc2;
+        };
+        unsigned m = 0u;
+        int s = s0, sh = 0;
+#pragma unroll 1
+        for (const int last = s1 - 3; s < last; s += 4, sh += 4) {
+          const R2 p0 = s_pos[s], p1 = s_pos[s + 1], p2 = s_pos[s + 2], p3 = s_pos[s + 3];
+          const unsigned nib = (in(p0) ? 1u : 0u) | (in(p1) ? 2u : 0u) | (in(p2) ? 4u : 0u) | (in(p3) ? 8u : 0u);
+          m |= nib << sh;
+        }
+        if (s + 1 < s1) {
+          const R2 p0 = s_pos[s], p1 = s_pos[s + 1];
+          m |= ((in(p0) ? 1u : 0u) | (in(p1) ? 2u : 0u)) << sh;
+          s += 2;
+          sh += 2;
+        }
+        if (s < s1) m |= (in(s_pos[s]) ? 1u : 0u) << sh;
         return m;
       };
       // the marked candidates of window (ma, sa0) and then of window (mb, sb0), in slot order;

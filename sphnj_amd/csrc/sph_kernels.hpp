@@ -2057,16 +2057,41 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
         const int sa = sc - P.sc0;
         const int lo = s_lb[sa][ra - P.r0], hi = s_lb[sa][rb + 1 - P.r0];
         if constexpr (MODE == kShiftRecord) {
-          // first and last candidate within rc (1 + s): f follows l until one is seen
-          int f = lo, la = lo;
-          bool seen = false;
-          for_range_4_2_1(lo, hi, [&](int l, bool) {
-            const bool in = staged_term(l) < rl2;
-            f = seen ? f : l;
-            seen = seen || in;
-            la = in ? l : la;
-          });
-          if (seen) put_run(f, la - f + 1);
+          // first and last candidate within rc (1 + s)
+          if (hi - lo <= 32) {  // from the mask of those candidates, 4 bits placed at a time
+            unsigned mk = 0u;
+            int l = lo, sh = 0;
+#pragma unroll 1
+            for (const int last = hi - 3; l < last; l += 4, sh += 4) {
+              unsigned nib = staged_term(l) < rl2 ? 1u : 0u;
+              nib |= staged_term(l + 1) < rl2 ? 2u : 0u;
+              nib |= staged_term(l + 2) < rl2 ? 4u : 0u;
+              nib |= staged_term(l + 3) < rl2 ? 8u : 0u;
+              mk |= nib << sh;
+            }
+            if (l + 1 < hi) {
+              unsigned nib = staged_term(l) < rl2 ? 1u : 0u;
+              nib |= staged_term(l + 1) < rl2 ? 2u : 0u;
+              mk |= nib << sh;
+              l += 2;
+              sh += 2;
+            }
+            if (l < hi) mk |= (staged_term(l) < rl2 ? 1u : 0u) << sh;
+            if (mk != 0u) {
+              const int f = __builtin_ctz(mk);
+              put_run(lo + f, 32 - __builtin_clz(mk) - f);
+            }
+          } else {  // f follows l until one is seen
+            int f = lo, la = lo;
+            bool seen = false;
+            for_range_4_2_1(lo, hi, [&](int q, bool) {
+              const bool in = staged_term(q) < rl2;
+              f = seen ? f : q;
+              seen = seen || in;
+              la = in ? q : la;
+            });
+            if (seen) put_run(f, la - f + 1);
+          }
         } else {
           for_range_4_2_1(lo, hi, staged_add);
         }

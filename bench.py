@@ -36,25 +36,27 @@ HBM_PEAK_GBS = 8000.0
 # inputs once and writes its outputs once, neighbour reads counted as ideal reuse.
 BYTES = {
     # stage: {precision: bytes per fluid particle per launch}
-    "fluid": {"m32": 74 + 90, "p64": 136 + 122},   # pair + update (fused)
-    # gradient pass + shift (fused), per pass, averaged over K = 10 passes.  M32: every pass
-    # reads the views, path length and class and writes the views and path length (25 B); the
-    # first also reads the sub-cell and writes the accumulated shift and the candidate runs
-    # (53 B); a middle pass reads the runs (12 B) and reads and writes the accumulated shift
-    # (53 B); the last also reads the sub-cell, the path since the hash kept with the runs, the
-    # double masters, L, u, v and writes the masters, u, v (133 B).  P64: the positions are
-    # the views, the accumulated shift and L are double (77, 85 and 141 B).
-    "shift": {"m32": 61, "p64": 90},
+    # pair + update (fused), with the shift's candidate runs it records (16 B written)
+    "fluid": {"m32": 74 + 106, "p64": 136 + 138},
+    # gradient pass + shift (fused), per pass, averaged over K = 10 passes, the runs recorded by
+    # the pair kernel.  M32: every pass reads the views, path length and class and writes the
+    # views and path length (25 B); the first also reads the runs (12 B) and writes the
+    # accumulated shift (45 B); a middle pass reads the runs and reads and writes the accumulated
+    # shift (53 B); the last also reads the sub-cell, the path since the hash kept with the runs,
+    # the double masters, L, u, v and writes the masters, u, v (133 B).  P64: the positions are
+    # the views, the accumulated shift and L are double (69, 85 and 141 B).
+    "shift": {"m32": 60, "p64": 89},
     "cells": {"m32": 16 + 142, "p64": 20 + 150},   # hash + sort/permute/cell ranges
 }
 WALL_BYTES = {"m32": 38, "p64": 66}  # wall Shepard, per wall particle
 
 
 def shift_step_bytes_m32(K):
-    """M32 shift bytes per fluid particle per step at K passes: 53 B for the first and each
-    middle pass and 133 B for the last (61 B a pass on average at K = 10); a single pass reads
-    the sub-cell, views, path, class, masters, L, u, v and writes views, path, masters, u, v."""
-    return 109 if K == 1 else 53 * K + 80
+    """M32 shift bytes per fluid particle per step at K passes, the runs recorded by the pair
+    kernel: 45 B for the first pass, 53 B for each middle one and 133 B for the last (60.2 B a
+    pass on average at K = 10); a single pass reads the sub-cell, views, path, class, masters,
+    L, u, v and writes views, path, masters, u, v."""
+    return 109 if K == 1 else 53 * K + 72
 
 
 def configs():

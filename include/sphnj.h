@@ -239,11 +239,13 @@ int sphnj_set_graph(sphnj_ctx *ctx, int enable);
  * write back and invalidate the L2 of every XCD, which costs more than a launch boundary. */
 int sphnj_set_persistent_solid(sphnj_ctx *ctx, int enable);
 /* Shift passes of a step (fluidCorrection's loop, src/fluid.cu:442-459; no counterpart of its
- * own in the reference): mode 1 (default) has the step's first pass record, per particle, the
- * runs of candidates within rc (1 + 0.02), and the later passes walk only those runs while
- * every path since the first pass stays within 0.02 rc (else the full windows); mode 0 searches
- * the full pruned windows in every pass; mode 2 records the runs and never walks them.  All
- * three give the same results bit for bit.  SPHNJ_E_ARG for another mode. */
+ * own in the reference): mode 1 (default) records, per particle, the runs of candidates within
+ * rc (1 + 0.02) -- in the fluid stage's pair kernel when the pair and the shift use the same
+ * block size (their tiles are then the same), else in the step's first shift pass -- and the
+ * shift passes walk only those runs while every path since the recording stays within 0.02 rc
+ * (else the full windows); mode 0 searches the full pruned windows in every pass; mode 2 has the
+ * first shift pass record the runs and never walks them.  All three give the same results bit
+ * for bit.  SPHNJ_E_ARG for another mode. */
 int sphnj_set_shift_runs(sphnj_ctx *ctx, int mode);
 /* Autotuner: times the fluid pair kernel (family 0) and a step's shift passes (family 1) at
  * block sizes 64, 128 and 256 -- a block owns that many sorted particles of a cell column, so

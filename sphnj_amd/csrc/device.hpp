@@ -52,8 +52,9 @@ struct CaseDev {
   // shift windows kept over a step's passes (skin s rc, s = kShiftSkin): the recording radius
   // as the first pass compares it (M32 scaled r'^2, P64 r^2) of rc (1 + s), the widening of
   // that pass's pruning radius, and the largest path the lane and its candidates may have
-  // taken since that pass for the windows to hold every neighbour within rc
-  R rl2_rec;
+  // taken since that pass for the windows to hold every neighbour within rc; rl2_pair is the
+  // recording radius as the pair kernel compares it (unscaled r^2), when it records the runs
+  R rl2_rec, rl2_pair;
   float skin_grow, skin_ok;
   float ssx, ssy, inv_ssx, inv_ssy;
   double dxmin, dymin, dxmax, dymax;

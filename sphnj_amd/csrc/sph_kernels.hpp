@@ -1341,8 +1341,41 @@ __device__ __forceinline__ bool span_window(const TilePlan &P, const CP &cp, int
 // full windows and the sums are the same bit for bit.  Any other lane searches the full
 // windows, with the tile's path bound rebuilt from the first pass's bound.  The last pass
 // writes the path since the hash again.  kShiftPlain is the single pass of a one-pass step.
+// When the pair kernel runs on the shift's tiles it records the runs itself, at the hash's
+// positions (with the lane's path since the hash and the tile's staged path bound, as the
+// first pass would), and every pass of the step is a kShiftCached one: the paths then count
+// from the hash, which bounds the paths since the recording.
 constexpr int kShiftPlain = 0, kShiftRecord = 1, kShiftCached = 2;
 constexpr double kShiftSkin = 0.02;
+
+// A lane's recorded candidate runs, in the packing of f_wc: up to 5 runs of (11-bit start,
+// 7-bit length) in three words; ok turns false at a sixth run, a run past 127 slots or a start
+// past 2047 (the lane then searches its full windows).
+struct RunRec {
+  unsigned w0 = 0u, w1 = 0u, w2 = 0u;
+  int n = 0;
+  bool ok = true;
+  __device__ __forceinline__ void put(int f, int len) {
+    if (n >= 5 || len > 127 || f > 2047) {
+      ok = false;
+      return;
+    }
+    const unsigned st = (unsigned)f, ln = (unsigned)len;
+    if (n == 0) w0 |= st | ln << 22;
+    if (n == 1) {
+      w0 |= st << 11;
+      w1 |= ln << 22;
+    }
+    if (n == 2) w1 |= st, w2 |= ln << 11;
+    if (n == 3) w1 |= st << 11, w2 |= ln << 18;
+    if (n == 4) w2 |= st | ln << 25;
+    ++n;
+  }
+  // the f_wc entry; path: the path since the hash at the recording
+  __device__ __forceinline__ uint4 entry(float path) const {
+    return uint4{w0 | (ok ? 0x80000000u : 0u), w1, w2, __float_as_uint(path)};
+  }
+};
 
 // Pruning radius: rc (1 + 1e-5) plus the tile's displacement bound D; a tile that is not
 // staged searches its lanes' whole 3x3 cells.
@@ -1500,10 +1533,12 @@ struct ShiftCap {
 // block.  Reads the current buffers (fa) of every neighbour and writes particle i's updated
 // state into the alternate buffers (fb), so neighbours never see a half-updated state.  Also
 // reduces the per-case max |u|^2 of the updated velocities (getMaxSpeed,
-// src/fluid.cu:429-440) and advances every particle's path length since the hash.
+// src/fluid.cu:429-440) and advances every particle's path length since the hash.  With
+// record (the shift runs on the same tiles), it also records the shift passes' candidate runs
+// (f_wc, tile_d0) from its windows widened by the skin.
 template <class Pol, bool VISCID, int BLOCK>
 __global__ void __launch_bounds__(BLOCK, (Pol::MIXED ? 4 : 1))
-    k_fluid_pair_update(Dev<Pol> d, const TilePlan *plans, const int *ntiles) {
+    k_fluid_pair_update(Dev<Pol> d, const TilePlan *plans, const int *ntiles, int record) {
   using R = typename Pol::R;
   using R2 = typename V2<R>::T;
   constexpr int CAP = PairCap<Pol, BLOCK>::value;
@@ -1669,7 +1704,13 @@ __global__ void __launch_bounds__(BLOCK, (Pol::MIXED ? 4 : 1))
                 (int)d.f_phase[s] | ((int)d.f_cls[s] << 4), true);
       });
     };
-    const float Rp = prune_radius(cp, staged, staged_bound<BLOCK>(s_dmax));
+    const float Dst = staged_bound<BLOCK>(s_dmax), Rp = prune_radius(cp, staged, Dst);
+    // a recording for the shift passes (record): the runs of every lane of a staged tile, the
+    // tile's staged path bound with them; a lane of a tile that is not staged has none
+    if (record) {
+      if (t == 0) d.tile_d0[tile] = Dst;
+      if (!staged) d.f_wc[i] = uint4{0u, 0u, 0u, 0u};
+    }
     // the lane in the tile frame (M32 staged positions are re-based on it)
     const R yg = Pol::MIXED ? yi + (R)(cy - P.cy_ref) * cp.cdy_r : yi;
     auto staged_j = [&](auto inside, int l, bool in) {
@@ -1687,33 +1728,44 @@ __global__ void __launch_bounds__(BLOCK, (Pol::MIXED ? 4 : 1))
       // together instead of each window for its own longest lane.
       unsigned mk[kWindowOrder];
       int w0[kWindowOrder];
-      // in-range mask of the run [s0, s1) (a run longer than 32 is evaluated right away)
-      auto mark = [&](int s0, int s1) -> unsigned {
-        if (s1 - s0 > 32) {
-          for_range_masked<2>(s0, s1, [&](int l, bool in) { staged_j(Cut{}, l, in); });
-          return 0u;
-        }
+      // in-range mask of the run [s0, s1) of at most 32 candidates; with REC also the mask ms
+      // of the candidates within rc (1 + s) (the recorded runs).  A longer run is evaluated
+      // whole (direct), in its place in the window order.
+      RunRec rr;
+      auto direct = [&](int s0, int s1) {
+        for_range_masked<2>(s0, s1, [&](int l, bool in) { staged_j(Cut{}, l, in); });
+      };
+      auto mark = [&](auto rec, int s0, int s1, unsigned &ms) -> unsigned {
+        constexpr bool REC = decltype(rec)::value;
+        ms = 0u;
         // 4 candidates a round (then 2, then 1), their bits gathered in a nibble placed at
         // once, so that the bit positions cost no instruction per candidate
-        auto in = [&](const R2 &pj) {
+        auto r2of = [&](const R2 &pj) {
           const R dx = pj.x - xi, dy = pj.y - yg;
-          return dx * dx + dy * dy < K.rc2;
+          return dx * dx + dy * dy;
         };
+        const R rc2 = K.rc2, rl2 = cp.rl2_pair;
         unsigned m = 0u;
         int s = s0, sh = 0;
 #pragma unroll 1
         for (const int last = s1 - 3; s < last; s += 4, sh += 4) {
           const R2 p0 = s_pos[s], p1 = s_pos[s + 1], p2 = s_pos[s + 2], p3 = s_pos[s + 3];
-          const unsigned nib = (in(p0) ? 1u : 0u) | (in(p1) ? 2u : 0u) | (in(p2) ? 4u : 0u) | (in(p3) ? 8u : 0u);
-          m |= nib << sh;
+          const R q0 = r2of(p0), q1 = r2of(p1), q2 = r2of(p2), q3 = r2of(p3);
+          m |= ((q0 < rc2 ? 1u : 0u) | (q1 < rc2 ? 2u : 0u) | (q2 < rc2 ? 4u : 0u) | (q3 < rc2 ? 8u : 0u)) << sh;
+          if (REC) ms |= ((q0 < rl2 ? 1u : 0u) | (q1 < rl2 ? 2u : 0u) | (q2 < rl2 ? 4u : 0u) | (q3 < rl2 ? 8u : 0u)) << sh;
         }
         if (s + 1 < s1) {
-          const R2 p0 = s_pos[s], p1 = s_pos[s + 1];
-          m |= ((in(p0) ? 1u : 0u) | (in(p1) ? 2u : 0u)) << sh;
+          const R q0 = r2of(s_pos[s]), q1 = r2of(s_pos[s + 1]);
+          m |= ((q0 < rc2 ? 1u : 0u) | (q1 < rc2 ? 2u : 0u)) << sh;
+          if (REC) ms |= ((q0 < rl2 ? 1u : 0u) | (q1 < rl2 ? 2u : 0u)) << sh;
           s += 2;
           sh += 2;
         }
-        if (s < s1) m |= (in(s_pos[s]) ? 1u : 0u) << sh;
+        if (s < s1) {
+          const R q0 = r2of(s_pos[s]);
+          m |= (q0 < rc2 ? 1u : 0u) << sh;
+          if (REC) ms |= (q0 < rl2 ? 1u : 0u) << sh;
+        }
         return m;
       };
       // the marked candidates of window (ma, sa0) and then of window (mb, sb0), in slot order;
@@ -1748,26 +1800,68 @@ __global__ void __launch_bounds__(BLOCK, (Pol::MIXED ? 4 : 1))
       };
       auto eval = [&](unsigned m, int s0) { eval2(m, s0, 0u, 0); };
       const int off = lsx - P.sc0;  // tile sub-column of the lane's own sub-column
+      // the windows are marked left to right (offsets -3 .. +3; window q has offset
+      // window_offset(q)), the order in which a recording keeps its runs for the shift passes;
+      // a recording widens them by the skin
+      const float Rpw = record ? Rp + cp.skin_grow : Rp;
+      unsigned dmask = 0u;  // windows evaluated whole
 #pragma unroll
-      for (int q = 0; q < kWindowOrder; ++q) {
+      for (int k = 0; k < kWindowOrder; ++k) {
+        const int o = k - 3, q = o < 0 ? -2 * o - 1 : 2 * o;
         mk[q] = 0u;
         w0[q] = 0;
-        const int sa = off + window_offset(q);
+        const int sa = off + o;
         int ra, rb;
-        if (!span_window(P, cp, lsx, sa, sy, cy, lp, Rp, ra, rb)) continue;
+        if (!span_window(P, cp, lsx, sa, sy, cy, lp, Rpw, ra, rb)) continue;
         const int s0 = s_lb[sa][ra - P.r0], s1 = s_lb[sa][rb + 1 - P.r0];
         w0[q] = s0;
-        mk[q] = mark(s0, s1);
+        if (s1 - s0 > 32) {  // evaluated whole below; its end in place of the mask
+          mk[q] = (unsigned)s1;
+          dmask |= 1u << q;
+          rr.ok = false;
+          continue;
+        }
+        unsigned ms;
+        if (record) {
+          mk[q] = mark(std::true_type{}, s0, s1, ms);
+          if (ms != 0u) {
+            const int f = __builtin_ctz(ms);
+            rr.put(s0 + f, 32 - __builtin_clz(ms) - f);
+          }
+        } else {
+          mk[q] = mark(std::false_type{}, s0, s1, ms);
+        }
       }
-      eval(mk[0], w0[0]);
+      if (record) d.f_wc[i] = rr.entry(cum0);
+      // windows in the order 0, -1, +1, ... (those at -k and +k in one loop unless one of them
+      // is evaluated whole)
+      auto evalq = [&](int q) {
+        if ((dmask >> q) & 1u)
+          direct(w0[q], (int)mk[q]);
+        else
+          eval(mk[q], w0[q]);
+      };
+      evalq(0);
 #pragma unroll
-      for (int q = 1; q < kWindowOrder; q += 2) eval2(mk[q], w0[q], mk[q + 1], w0[q + 1]);
+      for (int q = 1; q < kWindowOrder; q += 2) {
+        if (((dmask >> q) & 3u) != 0u) {
+          evalq(q);
+          evalq(q + 1);
+        } else {
+          eval2(mk[q], w0[q], mk[q + 1], w0[q + 1]);
+        }
+      }
       if (wstaged) {  // walls of the 3x3 cells, cell column by cell column
         const int lr0 = max(cy - 1, 0) - W.wcr0, lr1 = min(cy + 1, cp.ny - 1) - W.wcr0;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
-          const int s0 = s_wlb[a][lr0];
-          eval(mark(s0, s_wlb[a][lr1 + 1]), s0);
+          const int s0 = s_wlb[a][lr0], s1 = s_wlb[a][lr1 + 1];
+          if (s1 - s0 > 32) {
+            direct(s0, s1);
+          } else {
+            unsigned ms;
+            eval(mark(std::false_type{}, s0, s1, ms), s0);
+          }
         }
       }
     }
@@ -2023,25 +2117,8 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
   };
   auto staged_add = [&](int l, bool) { (void)staged_term(l); };
   // first pass of several: the runs to record (see kShiftRecord)
-  unsigned wc0 = 0u, wc1 = 0u, wc2 = 0u;
-  int nrun = 0;
-  bool runs_ok = staged;
-  auto put_run = [&](int f, int n) {
-    if (nrun >= 5 || n > 127) {
-      runs_ok = false;
-      return;
-    }
-    const unsigned st = (unsigned)f, ln = (unsigned)n;
-    if (nrun == 0) wc0 |= st | ln << 22;
-    if (nrun == 1) {
-      wc0 |= st << 11;
-      wc1 |= ln << 22;
-    }
-    if (nrun == 2) wc1 |= st, wc2 |= ln << 11;
-    if (nrun == 3) wc1 |= st << 11, wc2 |= ln << 18;
-    if (nrun == 4) wc2 |= st | ln << 25;
-    ++nrun;
-  };
+  RunRec rr;
+  rr.ok = staged;
   if (cached) {
     const unsigned x = own.wc.x, y = own.wc.y, z = own.wc.z;
     const int s0 = x & 0x7ff, s1 = (x >> 11) & 0x7ff, s2 = y & 0x7ff, s3 = (y >> 11) & 0x7ff, s4 = z & 0x7ff;
@@ -2079,7 +2156,7 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
             if (l < hi) mk |= (staged_term(l) < rl2 ? 1u : 0u) << sh;
             if (mk != 0u) {
               const int f = __builtin_ctz(mk);
-              put_run(lo + f, 32 - __builtin_clz(mk) - f);
+              rr.put(lo + f, 32 - __builtin_clz(mk) - f);
             }
           } else {  // f follows l until one is seen
             int f = lo, la = lo;
@@ -2090,7 +2167,7 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
               seen = seen || in;
               la = in ? q : la;
             });
-            if (seen) put_run(f, la - f + 1);
+            if (seen) rr.put(f, la - f + 1);
           }
         } else {
           for_range_4_2_1(lo, hi, staged_add);
@@ -2105,8 +2182,7 @@ __device__ __forceinline__ void shift_tile(const Dev<Pol> &d, const TilePlan &P,
       }
     });
   }
-  if constexpr (MODE == kShiftRecord)
-    d.f_wc[i] = uint4{wc0 | (runs_ok ? 0x80000000u : 0u), wc1, wc2, __float_as_uint(cum0)};
+  if constexpr (MODE == kShiftRecord) d.f_wc[i] = rr.entry(cum0);
   // walls of the 3x3 cells (cell column by cell column, rows in order), then solids
   if (wstaged) {
     const int lr0 = max(cy - 1, 0) - W.wcr0, lr1 = min(cy + 1, cp.ny - 1) - W.wcr0;

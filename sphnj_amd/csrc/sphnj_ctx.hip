@@ -102,11 +102,15 @@ struct EngineBase {
   hipStream_t stream = nullptr;
   bool use_graph = true;
   bool persistent_solid = false;  // S solid sub-steps as one launch (k_solid_substeps)
-  int shift_runs = 1;             // shift passes after the first walk its recorded runs
+  int shift_runs = 1;             // shift passes walk the recorded candidate runs
   int blk_pair = 256, blk_shift = 256, blk_stream = 256;
   // set by enqueue_step around its cell stage: the row-table launch also runs the wall update
   // and the solid coupling (k_rows_wall)
   bool rows_wall = false;
+  // the pair kernel records the shift passes' candidate runs (set around the fluid stage when
+  // the pair and the shift share their tiles), and f_wc holds such runs for the current hash
+  // and tiles: the step's first shift pass then walks them instead of recording
+  bool pair_rec = false, pair_runs = false;
   // per-stage timing: HIP events recorded on the context stream around each stage while on
   bool timing = false;
   struct StageRec {
@@ -293,6 +297,7 @@ struct Engine : EngineBase {
     {
       const double g = (1.0 + kShiftSkin) * (1.0 + kShiftSkin), rc = std::sqrt(D.rc2);
       c.rl2_rec = Pol::MIXED ? (R)(D.rc2 * D.hinv2 * 1.4426950408889634 * g) : (R)(D.rc2 * g);
+      c.rl2_pair = (R)(D.rc2 * g);
       c.skin_grow = (float)(kShiftSkin * rc);
       c.skin_ok = (float)(kShiftSkin * rc * (1.0 - 1e-3));
     }
@@ -711,6 +716,7 @@ struct Engine : EngineBase {
   // new_step: a step begins (its size is set here: in the fluid hash, or on its own without
   // fluid)
   int stage_cells(bool new_step) {
+    pair_runs = false;  // a new hash: new slots and tiles
     const int nf = d.nf, ns = d.ns, nc = d.total_cells;
     // every species' cell ranges start empty (the permute writes the fluid's, k_cell_bounds the
     // solids')
@@ -805,10 +811,11 @@ struct Engine : EngineBase {
   void pair_at() {
     const int b = bidx(B);
     const int *nt = tile_off[b] + G;
+    const int rec = pair_rec ? 1 : 0;
     if (viscid)
-      k_fluid_pair_update<Pol, true, B><<<grid_tiles(B), B, 0, stream>>>(d, plans[b], nt);
+      k_fluid_pair_update<Pol, true, B><<<grid_tiles(B), B, 0, stream>>>(d, plans[b], nt, rec);
     else
-      k_fluid_pair_update<Pol, false, B><<<grid_tiles(B), B, 0, stream>>>(d, plans[b], nt);
+      k_fluid_pair_update<Pol, false, B><<<grid_tiles(B), B, 0, stream>>>(d, plans[b], nt, rec);
   }
   int launch_pair(int blk) {
     int rc;
@@ -822,8 +829,9 @@ struct Engine : EngineBase {
     LAUNCH_CHECK();
     return SPHNJ_OK;
   }
-  // the step's first pass records the candidate runs of the later ones (kShiftRecord);
-  // shift_runs 0 searches the full windows in every pass, 2 records but never walks the runs
+  // the step's first pass records the candidate runs of the later ones (kShiftRecord) unless
+  // the pair kernel has recorded them (pair_runs: then every pass walks them); shift_runs 0
+  // searches the full windows in every pass, 2 records but never walks the runs
   template <int B>
   void shift_at(int pass) {
     const int b = bidx(B);
@@ -831,7 +839,7 @@ struct Engine : EngineBase {
     const dim3 g(grid_tiles(B));
     if (shift_runs == 0 || K == 1)
       k_fluid_shift<Pol, B, kShiftPlain><<<g, B, 0, stream>>>(d, plans[b], nt, pass, 0);
-    else if (pass == 0)
+    else if (pass == 0 && !pair_runs)
       k_fluid_shift<Pol, B, kShiftRecord><<<g, B, 0, stream>>>(d, plans[b], nt, pass, 0);
     else
       k_fluid_shift<Pol, B, kShiftCached><<<g, B, 0, stream>>>(d, plans[b], nt, pass, shift_runs == 1);
@@ -857,6 +865,7 @@ struct Engine : EngineBase {
   // place, which are saved before and restored after.
   int autotune() override {
     if (d.nf == 0) return SPHNJ_OK;
+    pair_runs = false;  // the shift family times its recording pass; its runs are not kept
     const int nf = d.nf;
     double *su = nullptr, *sv = nullptr, *sx_ = nullptr, *sy_ = nullptr;
     int8_t *scls = nullptr;
@@ -927,6 +936,7 @@ struct Engine : EngineBase {
       (void)hipFree(q);
       allocs.erase(std::remove(allocs.begin(), allocs.end(), q), allocs.end());
     }
+    pair_runs = false;
     drop_graphs();
     return SPHNJ_OK;
   }
@@ -950,7 +960,11 @@ struct Engine : EngineBase {
         return SPHNJ_OK;
       case SPHNJ_STAGE_FLUID:
         if (nf > 0) {
+          // the pair kernel records the shift's runs when both use the same tiles
+          pair_rec = shift_runs == 1 && K > 1 && blk_pair == blk_shift;
           const int rc = launch_pair(blk_pair);
+          pair_runs = pair_rec;
+          pair_rec = false;
           if (rc) return rc;
           k_umax_final<Pol><<<ncase, 64, 0, stream>>>(d);
           LAUNCH_CHECK();
@@ -1800,6 +1814,7 @@ int sphnj_set_shift_runs(sphnj_ctx *c, int mode) {
   if (c->eng) {
     if (c->eng->shift_runs != mode) c->eng->drop_graphs();
     c->eng->shift_runs = mode;
+    c->eng->pair_runs = false;
   }
   return SPHNJ_OK;
 }
@@ -1819,6 +1834,7 @@ int sphnj_set_block(sphnj_ctx *c, int32_t fam, int32_t block) {
     c->eng->blk_shift = block;
   else
     c->eng->blk_stream = block;
+  c->eng->pair_runs = false;
   c->eng->drop_graphs();
   return SPHNJ_OK;
 }

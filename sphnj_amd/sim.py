@@ -364,8 +364,9 @@ class Simulation(ContextBase):
         self._call("set_stage_timing", int(on))
 
     def set_shift_runs(self, mode: int) -> None:
-        """Shift passes after a step's first: 1 walk the runs it recorded (default), 0 search
-        the full windows, 2 record and never walk them (sphnj_set_shift_runs)."""
+        """Shift passes: 1 walk the candidate runs recorded by the pair kernel (same block sizes)
+        or by the first pass (default), 0 search the full windows, 2 record in the first pass and
+        never walk them (sphnj_set_shift_runs)."""
         self._call("set_shift_runs", int(mode))
 
     def stage_time(self, name: str) -> tuple[int, float]:

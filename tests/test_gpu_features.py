@@ -183,27 +183,35 @@ def test_dense_tiles_read_global_candidates(gpu, oracle_mod):
 
 @pytest.mark.parametrize("precision", ["p64", "m32"])
 def test_shift_runs_equal_full_windows(gpu, precision):
-    """The shift passes after a step's first walk the candidate runs it recorded (mode 1),
-    search the full pruned windows (mode 0), or record the runs and always fall back to the
-    windows rebuilt from the first pass's path bound (mode 2): the same non-zero terms in the
-    same order, so every field is bit-identical, through graph replays, on a batch whose cases
-    take 10 and 3 passes per step (the one with fewer carries its state over)."""
+    """The shift passes walk recorded candidate runs (mode 1: recorded by the pair kernel when
+    it shares the shift's tiles, else by the step's first shift pass), search the full pruned
+    windows (mode 0), or record the runs in the first pass and always fall back to the windows
+    rebuilt from its path bound (mode 2): the same non-zero terms in the same order, so every
+    field is bit-identical, through graph replays, on a batch whose cases take 10 and 3 passes
+    per step (the one with fewer carries its state over).  Pair tiles of 128 slots beside shift
+    tiles of 256 leave the recording to the first shift pass; they are compared with mode 0 on
+    the same tiles."""
     cases = [Case.generate("dam_break", dx=0.02, air=1, K=10),
              Case.generate("dam_break", dx=0.025, K=3)]
-    sims = {m: Simulation(cases, precision, shift_runs=m) for m in (0, 1, 2)}
+    sims = {}
+    for m, pair_block in ((0, 256), (1, 256), (2, 256), (0, 128), (1, 128)):
+        s = Simulation(cases, precision, shift_runs=m)
+        s.set_block(0, pair_block)
+        s.set_block(1, 256)
+        sims[m, pair_block] = s
     for n in (1, 9, 30):
         out = {}
-        for m, s in sims.items():
+        for key, s in sims.items():
             s.step(n)
-            out[m] = [s.download(c)["fluid"] for c in range(2)]
-        for m in (1, 2):
+            out[key] = [s.download(c)["fluid"] for c in range(2)]
+        for key, ref in (((1, 256), (0, 256)), ((2, 256), (0, 256)), ((1, 128), (0, 128))):
             for c in range(2):
                 for k in ("x", "y", "u", "v", "rho", "p", "boundary_type"):
-                    assert np.array_equal(out[m][c][k], out[0][c][k]), (n, m, c, k)
+                    assert np.array_equal(out[key][c][k], out[ref][c][k]), (n, key, c, k)
     for s in sims.values():
         assert s.status() == 0
     with pytest.raises(Exception):
-        sims[1].set_shift_runs(3)
+        sims[1, 256].set_shift_runs(3)
 
 
 def test_shift_runs_on_global_tiles(gpu):

@@ -1732,6 +1732,7 @@ __global__ void __launch_bounds__(BLOCK, (Pol::MIXED ? 4 : 1))
       // of the candidates within rc (1 + s) (the recorded runs).  A longer run is evaluated
       // whole (direct), in its place in the window order.
       RunRec rr;
+      const R rl2 = cp.rl2_pair;
       auto direct = [&](int s0, int s1) {
         for_range_masked<2>(s0, s1, [&](int l, bool in) { staged_j(Cut{}, l, in); });
       };
@@ -1744,7 +1745,7 @@ __global__ void __launch_bounds__(BLOCK, (Pol::MIXED ? 4 : 1))
           const R dx = pj.x - xi, dy = pj.y - yg;
           return dx * dx + dy * dy;
         };
-        const R rc2 = K.rc2, rl2 = cp.rl2_pair;
+        const R rc2 = K.rc2;
         unsigned m = 0u;
         int s = s0, sh = 0;
 #pragma unroll 1

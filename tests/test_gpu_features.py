@@ -373,10 +373,13 @@ def test_destroy_returns_device_memory(gpu):
 
 
 @pytest.mark.parametrize("precision", ["p64", "m32"])
-def test_stages_equal_step(gpu, precision):
-    """The six phases called one by one equal the step, where the coupling and the wall update
-    share one launch (a case with fluid, solids and walls), over three steps."""
-    c = gate()
+@pytest.mark.parametrize("make", [gate, two_phase_small])
+def test_stages_equal_step(gpu, precision, make):
+    """The six phases called one by one equal the step, whose cell stage also runs the wall
+    update and the coupling in its row-table launch (the FSI gate: fluid, solids and walls; a
+    two-phase dam break with 3 shift passes, whose pair kernel records the shift's runs in
+    both), over three steps."""
+    c = make()
     a = Simulation(c, precision, graph=False)
     b = Simulation(c, precision, graph=False)
     for _ in range(3):

@@ -1791,6 +1791,8 @@ __global__ void __launch_bounds__(BLOCK, (Pol::MIXED ? 4 : 1))
           fluid_j(Inside{}, q.l == own, dx, dy, dx * dx + dy * dy, q.v.x, q.v.y, q.r.x, q.r.y, q.fl, true);
         };
         Cand cur = next();
+        // unrolled by two, so that every other round reads into the registers the previous one
+        // used instead of copying nxt into cur
 #pragma unroll 2
         while (m) {
           const Cand nxt = next();

@@ -236,7 +236,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--config", default="two_phase", choices=["two_phase", "dam5k", "gate", "wedge", "ensemble"])
     ap.add_argument("--profile-steps", type=int, default=5, help="steps timed per stage with HIP events")
-    ap.add_argument("--cpu-steps", type=int, default=2, help="oracle steps in the CPU baseline sample (0 = skip)")
+    ap.add_argument("--cpu-steps", type=int, default=15,
+                    help="oracle steps in the CPU baseline sample, about 10 s on 16 cores (0 = skip)")
     ap.add_argument("--members", type=int, default=256, help="members of the batched ensemble")
     ap.add_argument("--serial-members", type=int, default=None,
                     help="members timed one by one in P64 (default: all; fewer are scaled to all)")
